@@ -1,0 +1,96 @@
+#if !defined(SBFS_H)
+#define SBFS_H
+
+#include <stddef.h>
+#include "domain.h"
+#include "array.h"
+#include "fluid.h"
+#include "interface.h"
+
+/*
+ * Library-level calls that have no counterpart in the reference.  They let
+ * a caller set up a case without NPY files, move fields between host and
+ * device, read the device diagnostics and choose the solver options.
+ * Every call returns 0 on success, non-zero on failure.
+ */
+SBFS_EXTERN_C_BEGIN
+
+/* text of the most recent failure ("" if none) */
+extern const char * sbfs_last_error(void);
+
+/* wait for all queued device work; reports any asynchronous error */
+extern int sbfs_synchronize(void);
+
+/* name of the device the library runs on (e.g. "AMD Instinct MI355X") */
+extern int sbfs_device_name(char * buf, size_t len);
+
+/* safety factors of the three time-step limits (overrides the environment
+ * variables coef_dt_adv, coef_dt_dif, coef_dt_int) */
+extern int sbfs_set_dt_coefs(double adv, double dif, double intf);
+
+/* integrate() execution path */
+#define SBFS_PATH_STAGED 0  /* one kernel group per reference stage */
+#define SBFS_PATH_FUSED  1  /* fused per-substep kernels (default) */
+extern int sbfs_set_path(int path);
+extern int sbfs_get_path(void);
+
+/* gauge of the singular (x-mode 0) periodic tridiagonal system */
+#define SBFS_GAUGE_PINNED    0 /* x_{Ny-1} = 0, the reference's singular
+                                  branch (src/tdm.c:94) taken for mode 0 */
+#define SBFS_GAUGE_REFERENCE 1 /* num/den closure with the DBL_EPSILON guard
+                                  exactly as src/tdm.c:92-94 */
+extern int sbfs_set_gauge(int gauge);
+
+/* y-direction tridiagonal algorithm */
+#define SBFS_TDM_SEQUENTIAL  0 /* one Thomas + Sherman-Morrison chain per
+                                  x-mode, the reference's order (src/tdm.c) */
+#define SBFS_TDM_PARTITIONED 1 /* chunked Thomas + reduced periodic system
+                                  (default for Ny >= 64) */
+extern int sbfs_set_tdm(int kind, int chunk);
+
+/* uniform-x domain built as the reference's IC generator does
+ * (initial_condition/main.py:15-24): xf = linspace(0, lx, Nx+1),
+ * xc = [0, face midpoints, lx]; metrics as src/domain.c:98-146 */
+extern int sbfs_domain_create(size_t nx, size_t ny, double lx, double ly,
+                              domain_t * domain);
+extern int sbfs_domain_destroy(domain_t * domain);
+
+/* allocate every array on the device (zero-filled) and set the parameters;
+ * refden = min(1, denr) as src/fluid/init.c:106 */
+extern int sbfs_fluid_create(const domain_t * domain, double Re, double Fr,
+                             double denr, double visr, fluid_t * fluid);
+extern int sbfs_fluid_destroy(fluid_t * fluid);
+extern int sbfs_interface_create(const domain_t * domain, double We,
+                                 interface_t * interface);
+extern int sbfs_interface_destroy(interface_t * interface);
+
+/* whole-buffer copies between host memory and array_t.data; `bytes` must
+ * equal array->datasize */
+extern int sbfs_array_upload(array_t * array, const void * host, size_t bytes);
+extern int sbfs_array_download(const array_t * array, void * host,
+                               size_t bytes);
+
+/*
+ * Diagnostics reduced on the device in FP64 (fixed reduction order, so the
+ * result is reproducible run to run):
+ *   out[0] gas volume      V   = sum vof * JDXC
+ *   out[1] centroid        x_c = sum vof * XC * JDXC / V
+ *   out[2] rise velocity   u_r = sum vof * (UX(i)+UX(i+1))/2 * JDXC / V
+ *   out[3] max |div u|         (src/logging/divergence.c:41-60)
+ *   out[4] min vof, out[5] max vof, out[6] mean vof
+ *   out[7] interface area      (src/logging/interface.c:120-137)
+ */
+#define SBFS_NDIAG 8
+extern int sbfs_diagnostics(const domain_t * domain, const fluid_t * fluid,
+                            const interface_t * interface,
+                            double out[SBFS_NDIAG]);
+
+/* the Poisson solver's row transforms on their own: FFTW REDFT10
+ * (inverse = 0) or REDFT01 (inverse = 1) of `howmany` rows of n doubles,
+ * n a power of two <= 4096; host buffers in and out */
+extern int sbfs_dct_rows(int inverse, size_t n, size_t howmany,
+                         const double * in, double * out);
+
+SBFS_EXTERN_C_END
+
+#endif /* SBFS_H */

@@ -1,0 +1,105 @@
+"""TEST INFRASTRUCTURE ONLY: Python binding of the CPU oracle.
+
+The oracle (oracle.c) is an FP64 C restatement of the reference time step.
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+import this package; the product path never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from simplebubblyflowsolver_amd import solver
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+DCT_DIRECT, DCT_FFT = 0, 1
+TDM_SEQUENTIAL, TDM_PARTITIONED = 0, 1
+GAUGE_PINNED, GAUGE_REFERENCE = 0, 1
+
+_lib = None
+
+
+def build(force: bool = False) -> Path:
+    if force or not LIB.exists():
+        subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        build()
+        _lib = C.CDLL(str(LIB))
+        _lib.oracle_set_poisson.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+        _lib.oracle_set_threads.argtypes = [C.c_int]
+        _lib.oracle_dct2.argtypes = [C.c_int, C.c_size_t, C.c_size_t, C.c_void_p, C.c_void_p]
+        _lib.oracle_dct3.argtypes = [C.c_int, C.c_size_t, C.c_size_t, C.c_void_p, C.c_void_p]
+        _lib.oracle_fft.argtypes = [C.c_size_t, C.c_void_p, C.c_void_p]
+        _lib.oracle_tdm_periodic.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_int]
+        _lib.oracle_tdm_partitioned.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_void_p]
+        _lib.oracle_poisson_evals.argtypes = [C.c_void_p, C.c_void_p]
+    return _lib
+
+
+def backend() -> solver.Backend:
+    return solver.Backend(lib(), "oracle_", "oracle_", False)
+
+
+def set_poisson(dct: int = DCT_FFT, tdm: int = TDM_SEQUENTIAL, chunk: int = 32,
+                gauge: int = GAUGE_PINNED) -> None:
+    lib().oracle_set_poisson(dct, tdm, chunk, gauge)
+
+
+def set_threads(n: int) -> None:
+    lib().oracle_set_threads(n)
+
+
+def dct2(x: np.ndarray, kind: int = DCT_FFT) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = x.shape[-1]
+    out = np.empty_like(x)
+    lib().oracle_dct2(kind, n, x.size // n, x.ctypes.data, out.ctypes.data)
+    return out
+
+
+def dct3(x: np.ndarray, kind: int = DCT_FFT) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = x.shape[-1]
+    out = np.empty_like(x)
+    lib().oracle_dct3(kind, n, x.size // n, x.ctypes.data, out.ctypes.data)
+    return out
+
+
+def fft(z: np.ndarray) -> np.ndarray:
+    z = np.ascontiguousarray(z, dtype=np.complex128)
+    out = np.empty_like(z)
+    lib().oracle_fft(z.size, z.ctypes.data, out.ctypes.data)
+    return out
+
+
+def tdm_periodic(q: np.ndarray, l: float, c: float, u: float, gauge: int = GAUGE_PINNED) -> np.ndarray:
+    q = np.ascontiguousarray(q, dtype=np.float64).copy()
+    rc = lib().oracle_tdm_periodic(q.size, l, c, u, q.ctypes.data, gauge)
+    if rc:
+        raise ValueError("tdm size too small")
+    return q
+
+
+def tdm_partitioned(q: np.ndarray, chunk: int, a: float, b: float) -> np.ndarray:
+    q = np.ascontiguousarray(q, dtype=np.float64).copy()
+    rc = lib().oracle_tdm_partitioned(q.size, chunk, a, b, q.ctypes.data)
+    if rc:
+        raise ValueError("no valid chunk for this size")
+    return q
+
+
+def new_case(nx, ny, lx=1.0, ly=2.0, Re=35.0, Fr=1.0, denr=1e-3, visr=1e-2, We=10.0,
+             dt_coefs=(0.2, 0.5, 0.95)) -> solver.Case:
+    be = backend()
+    be.set_dt_coefs(*dt_coefs)
+    return solver.Case(be, nx, ny, lx, ly, Re, Fr, denr, visr, We)

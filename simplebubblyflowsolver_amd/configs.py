@@ -1,0 +1,67 @@
+"""The benchmark and parity workloads.
+
+Physics defaults follow the reference's run script (exec/main.sh:18-30:
+coef_dt_adv/dif/int = 0.2/0.5/0.95, Re=35, We=10, Fr=1), with the density
+and viscosity ratios 1e-3 / 1e-2 of the workloads below.  Domains are 1 x 2
+(x wall-bounded and along gravity, y periodic), so cells are square with
+h = 1/Nx.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import ic
+
+DT_COEFS = (0.20, 0.50, 0.95)  # exec/main.sh:21-23
+
+
+@dataclass
+class Workload:
+    name: str
+    nx: int
+    ny: int
+    lx: float = 1.0
+    ly: float = 2.0
+    Re: float = 35.0
+    We: float = 10.0
+    Fr: float = 1.0
+    denr: float = 1.0e-3
+    visr: float = 1.0e-2
+    steps: int = 1
+    bubbles: list = field(default_factory=list)
+    seed: int | None = None
+
+    def vof(self) -> np.ndarray:
+        if len(self.bubbles) == 1:
+            cx, cy, r = self.bubbles[0]
+            return ic.bubble_vof(self.nx, self.ny, self.lx, self.ly, cx, cy, r)
+        return ic.swarm_vof(self.nx, self.ny, self.lx, self.ly, self.bubbles)
+
+    @property
+    def cells(self) -> int:
+        return self.nx * self.ny
+
+
+def rising_bubble(nx: int = 128, ny: int = 256, steps: int = 2000) -> Workload:
+    """One bubble of diameter 0.5 centred at (0.5, 0.5) in quiescent liquid
+    (128 x 256, 2000 steps in its full form)."""
+    return Workload("rising_bubble_%dx%d" % (nx, ny), nx, ny, steps=steps,
+                    bubbles=[(0.5, 0.5, 0.25)])
+
+
+def bubble_swarm(nx: int = 2048, ny: int = 4096, nbubbles: int = 64,
+                 steps: int = 200, seed: int = 1, Re: float = 35.0,
+                 We: float = 10.0) -> Workload:
+    """Non-overlapping bubbles, diameters uniform in [24, 48] cells,
+    dart-throwing placement with a 4-cell gap (2048 x 4096 with 64 bubbles
+    and 200 steps in its full form)."""
+    b = ic.poisson_disk_bubbles(nbubbles, nx, ny, 1.0, 2.0, 24.0, 48.0, 4.0, seed)
+    return Workload("bubble_swarm_%dx%d_n%d" % (nx, ny, nbubbles), nx, ny,
+                    Re=Re, We=We, steps=steps, bubbles=b, seed=seed)
+
+
+def large_swarm() -> Workload:
+    """4096 x 8192, 1024 bubbles, Re=100, We=5, 500 steps."""
+    return bubble_swarm(4096, 8192, 1024, 500, seed=4, Re=100.0, We=5.0)

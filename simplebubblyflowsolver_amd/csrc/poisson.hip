@@ -1,0 +1,465 @@
+// Split-pressure Poisson solver (reference src/fluid/compute_potential.c and
+// src/tdm.c) on one GPU.
+//
+//  1. k_dct2_rhs   one workgroup per row j: builds the right-hand side of
+//                  the row in LDS (assign_input, :285-363) and applies
+//                  FFTW's REDFT10 along x through an N/2-point complex
+//                  Stockham FFT in LDS; writes the row of modes r[j][m].
+//  2. tridiagonal  per x-mode m along the periodic y direction, directly on
+//                  the [j][m] layout (consecutive modes are consecutive
+//                  addresses, so every sweep step is a coalesced row access;
+//                  the reference's x1 <-> y1 pencil transposes disappear).
+//  3. k_dct3_psi   one workgroup per row: REDFT01 along x, then psi[1] with
+//                  its Neumann x ghosts and periodic y halo (extract_output,
+//                  :365-380, src/fluid/boundary/psi.c).
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "sbfs.h"
+#include "poisson.hpp"
+#include "sbfs_internal.hpp"
+#include "stencils.hpp"
+
+namespace sbfs {
+
+static const long double kPiL = 3.141592653589793238462643383279502884L;
+static const double kPiPoisson = 3.14159265358979324;  // compute_potential.c:21
+
+
+
+void poisson_destroy(PoissonPlan* p) {
+  if (!p) return;
+  (void)hipFree(p->buf);
+  (void)hipFree(p->v);
+  (void)hipFree(p->q1);
+  (void)hipFree(p->evals);
+  (void)hipFree(p->tw);
+  (void)hipFree(p->t1);
+  (void)hipFree(p->t2);
+  (void)hipFree(p->ptab);
+  (void)hipFree(p->pred);
+  (void)hipFree(p->gfl);
+  (void)hipFree(p->X);
+  (void)hipFree(p->rv);
+  (void)hipFree(p->rq1);
+  delete p;
+}
+
+// twiddle e^{-2 pi i e/m}, rounded from long double (the oracle builds its
+// tables with the same expression)
+static void twiddle(size_t e, size_t m, double* re, double* im) {
+  const long double a = 2.L * kPiL * (long double)e / (long double)m;
+  *re = (double)cosl(a);
+  *im = (double)(-sinl(a));
+}
+
+// largest chunk <= requested that divides n and leaves >= 3 chunks
+int partition_chunk(int n, int requested) {
+  int L = requested;
+  while (L >= 2 && (n % L != 0 || n / L < 3)) L /= 2;
+  return L >= 2 ? L : 0;
+}
+
+// ---------------------------------------------------------------------------
+// FFT in LDS: forward, Stockham autosort, radix-4 stages then one radix-2
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ double2* fft_lds(double2* x, double2* y, int m, const double2* __restrict__ tw) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  int n = m, s = 1, ls = 0;
+  while (n >= 4) {
+    const int q4 = n >> 2;
+    const int nb = m >> 2;
+    for (int b = tid; b < nb; b += nt) {
+      const int p = b >> ls, q = b & (s - 1);
+      const double2 a = x[q + s * p];
+      const double2 bb = x[q + s * (p + q4)];
+      const double2 c = x[q + s * (p + 2 * q4)];
+      const double2 d = x[q + s * (p + 3 * q4)];
+      const double t0r = a.x + c.x, t0i = a.y + c.y;
+      const double t1r = a.x - c.x, t1i = a.y - c.y;
+      const double t2r = bb.x + d.x, t2i = bb.y + d.y;
+      const double t3r = bb.x - d.x, t3i = bb.y - d.y;
+      const double2 x0 = make_double2(t0r + t2r, t0i + t2i);
+      const double2 x2 = make_double2(t0r - t2r, t0i - t2i);
+      const double2 x1 = make_double2(t1r + t3i, t1i - t3r);
+      const double2 x3 = make_double2(t1r - t3i, t1i + t3r);
+      double2* o = y + q + s * 4 * p;
+      o[0] = x0;
+      o[s] = cmul(x1, tw[p * s]);
+      o[2 * s] = cmul(x2, tw[2 * p * s]);
+      o[3 * s] = cmul(x3, tw[3 * p * s]);
+    }
+    __syncthreads();
+    double2* t = x;
+    x = y;
+    y = t;
+    n >>= 2;
+    s <<= 2;
+    ls += 2;
+  }
+  if (n == 2) {
+    for (int q = tid; q < s; q += nt) {
+      const double2 a = x[q], b = x[q + s];
+      y[q] = make_double2(a.x + b.x, a.y + b.y);
+      y[q + s] = cmul(make_double2(a.x - b.x, a.y - b.y), tw[0]);
+    }
+    __syncthreads();
+    double2* t = x;
+    x = y;
+    y = t;
+  }
+  return x;
+}
+
+// right-hand side of one cell (compute_potential.c:315-360)
+__device__ __forceinline__ double poisson_rhs(
+    int i, int j, int nx, const double* __restrict__ ux, const double* __restrict__ uy,
+    const double* __restrict__ psi, const double* __restrict__ den,
+    const double* __restrict__ hxxf, const double* __restrict__ jdxf,
+    const double* __restrict__ jdxc, double hy, double refden, double c0, double c1) {
+  const double hm = hxxf[i - 1], hp = hxxf[i];
+  const double jm = jdxf[i - 1], j0 = jdxc[i - 1], jp = jdxf[i];
+  const double div = 1. / j0 * (-jm / hm * ux[L::xf(i, j, nx)] + jp / hp * ux[L::xf(i + 1, j, nx)]
+                                - j0 / hy * uy[L::cc(i, j, nx)] + j0 / hy * uy[L::cc(i, j + 1, nx)]);
+  const double dxm = 0.5 * den[L::cc(i - 1, j, nx)] + 0.5 * den[L::cc(i, j, nx)];
+  const double dxp = 0.5 * den[L::cc(i, j, nx)] + 0.5 * den[L::cc(i + 1, j, nx)];
+  const double dym = 0.5 * den[L::cc(i, j - 1, nx)] + 0.5 * den[L::cc(i, j, nx)];
+  const double dyp = 0.5 * den[L::cc(i, j, nx)] + 0.5 * den[L::cc(i, j + 1, nx)];
+  const double pxm = -psi[L::cc(i - 1, j, nx)] + psi[L::cc(i, j, nx)];
+  const double pxp = -psi[L::cc(i, j, nx)] + psi[L::cc(i + 1, j, nx)];
+  const double pym = -psi[L::cc(i, j - 1, nx)] + psi[L::cc(i, j, nx)];
+  const double pyp = -psi[L::cc(i, j, nx)] + psi[L::cc(i, j + 1, nx)];
+  const double gxm = (refden / dxm - 1.) * pxm / hm;
+  const double gxp = (refden / dxp - 1.) * pxp / hp;
+  const double gym = (refden / dym - 1.) * pym / hy;
+  const double gyp = (refden / dyp - 1.) * pyp / hy;
+  const double add = 1. / j0 * (-jm / hm * gxm + jp / hp * gxp - j0 / hy * gym + j0 / hy * gyp);
+  return c0 * div + c1 * add;
+}
+
+// REDFT10 of a row held in LDS (`row`, n doubles aliasing B) -> out row
+__device__ void dct2_row_lds(double2* A, double2* B, int n, int m,
+                             const double2* __restrict__ tw, const double2* __restrict__ t1,
+                             const double2* __restrict__ t2, double* __restrict__ out) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const double* row = (const double*)B;
+  // v[k] = x[2k], v[n-1-k] = x[2k+1]; z[k] = v[2k] + i v[2k+1]
+  for (int k = tid; k < m; k += nt) {
+    const int e = 2 * k, o = 2 * k + 1;
+    const double ve = (e < m) ? row[2 * e] : row[2 * (n - 1 - e) + 1];
+    const double vo = (o < m) ? row[2 * o] : row[2 * (n - 1 - o) + 1];
+    A[k] = make_double2(ve, vo);
+  }
+  __syncthreads();
+  const double2* z = fft_lds(A, B, m, tw);
+  for (int k = tid; k <= m; k += nt) {
+    const int kk = (k == m) ? 0 : k;
+    const int kr = (k == 0) ? 0 : m - k;
+    const double ar = z[kk].x, ai = z[kk].y;
+    const double br = z[kr].x, bi = -z[kr].y;
+    const double er = (ar + br) * 0.5, ei = (ai + bi) * 0.5;
+    const double dr = ar - br, di = ai - bi;
+    const double2 od = make_double2(di * 0.5, -(dr * 0.5));
+    const double2 tt = cmul(od, t1[k]);
+    const double2 vv = make_double2(er + tt.x, ei + tt.y);
+    const double2 w = cmul(vv, t2[k]);
+    out[k] = 2. * w.x;
+    if (k != 0 && k != m) out[n - k] = -2. * w.y;
+  }
+}
+
+__global__ void k_dct2_rhs(const double* __restrict__ ux, const double* __restrict__ uy,
+                           const double* __restrict__ psi, const double* __restrict__ den,
+                           const double* __restrict__ hxxf, const double* __restrict__ jdxf,
+                           const double* __restrict__ jdxc, double hy, double refden,
+                           double c0, double c1, int nx, int m,
+                           const double2* __restrict__ tw, const double2* __restrict__ t1,
+                           const double2* __restrict__ t2, double* __restrict__ out) {
+  extern __shared__ double2 lds[];
+  double2* A = lds;
+  double2* B = lds + m;
+  const int j = blockIdx.x + 1;
+  double* row = (double*)B;
+  for (int i = threadIdx.x + 1; i <= nx; i += blockDim.x)
+    row[i - 1] = poisson_rhs(i, j, nx, ux, uy, psi, den, hxxf, jdxf, jdxc, hy, refden, c0, c1);
+  __syncthreads();
+  dct2_row_lds(A, B, nx, m, tw, t1, t2, out + (size_t)(j - 1) * nx);
+}
+
+// plain REDFT10 of rows (tests of the transform alone)
+__global__ void k_dct2_rows(const double* __restrict__ in, double* __restrict__ out, int n, int m,
+                            const double2* __restrict__ tw, const double2* __restrict__ t1,
+                            const double2* __restrict__ t2) {
+  extern __shared__ double2 lds[];
+  double2* A = lds;
+  double2* B = lds + m;
+  double* row = (double*)B;
+  const double* src = in + (size_t)blockIdx.x * n;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) row[t] = src[t];
+  __syncthreads();
+  dct2_row_lds(A, B, n, m, tw, t1, t2, out + (size_t)blockIdx.x * n);
+}
+
+// REDFT01 of a row held in LDS (`row` aliasing B) into LDS; returns the
+// buffer holding the n output doubles
+__device__ double* dct3_row_lds(double2* A, double2* B, int n, int m,
+                                const double2* __restrict__ tw, const double2* __restrict__ t1,
+                                const double2* __restrict__ t2) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const double* x = (const double*)B;
+  for (int k = tid; k < m; k += nt) {
+    const double pr = x[k], pim = -((k == 0) ? 0. : x[n - k]);
+    const double2 vk = cmul(make_double2(pr, pim), make_double2(t2[k].x, -t2[k].y));
+    const double qr = x[k + m], qim = -x[m - k];
+    const double2 vm = cmul(make_double2(qr, qim), make_double2(t2[k + m].x, -t2[k + m].y));
+    const double sr = vk.x + vm.x, si = vk.y + vm.y;
+    const double2 f = cmul(make_double2(vk.x - vm.x, vk.y - vm.y), make_double2(t1[k].x, -t1[k].y));
+    A[k] = make_double2(sr - f.y, -(si + f.x));
+  }
+  __syncthreads();
+  double2* z = fft_lds(A, B, m, tw);
+  double* y = (double*)((z == A) ? B : A);
+  for (int k = tid; k < m; k += nt) {
+    const double v0 = z[k].x, v1 = -z[k].y;
+    const int e = 2 * k, o = 2 * k + 1;
+    if (e < m) y[2 * e] = v0; else y[2 * (n - 1 - e) + 1] = v0;
+    if (o < m) y[2 * o] = v1; else y[2 * (n - 1 - o) + 1] = v1;
+  }
+  __syncthreads();
+  return y;
+}
+
+__global__ void k_dct3_rows(const double* __restrict__ in, double* __restrict__ out, int n, int m,
+                            const double2* __restrict__ tw, const double2* __restrict__ t1,
+                            const double2* __restrict__ t2) {
+  extern __shared__ double2 lds[];
+  double2* A = lds;
+  double2* B = lds + m;
+  double* row = (double*)B;
+  const double* src = in + (size_t)blockIdx.x * n;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) row[t] = src[t];
+  __syncthreads();
+  const double* y = dct3_row_lds(A, B, n, m, tw, t1, t2);
+  for (int t = threadIdx.x; t < n; t += blockDim.x) out[(size_t)blockIdx.x * n + t] = y[t];
+}
+
+// REDFT01 of mode row j, then psi[1](., j) with Neumann x ghosts and halo
+__global__ void k_dct3_psi(const double* __restrict__ in, double* __restrict__ psi, int nx,
+                           int ny, int m, const double2* __restrict__ tw,
+                           const double2* __restrict__ t1, const double2* __restrict__ t2) {
+  extern __shared__ double2 lds[];
+  double2* A = lds;
+  double2* B = lds + m;
+  const int j = blockIdx.x + 1;
+  double* row = (double*)B;
+  const double* src = in + (size_t)(j - 1) * nx;
+  for (int t = threadIdx.x; t < nx; t += blockDim.x) row[t] = src[t];
+  __syncthreads();
+  const double* y = dct3_row_lds(A, B, nx, m, tw, t1, t2);
+  auto idx = [nx](int a, int b) { return L::cc(a, b, nx); };
+  for (int i = threadIdx.x + 1; i <= nx; i += blockDim.x) {
+    const double v = y[i - 1];
+    store_with_halo(psi, i, j, ny, 1, v, idx);
+    if (i == 1) store_with_halo(psi, 0, j, ny, 1, v, idx);
+    if (i == nx) store_with_halo(psi, nx + 1, j, ny, 1, v, idx);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// tridiagonal solves, reference order: one Thomas + Sherman-Morrison chain
+// per x-mode (src/tdm.c:17-106 with constant l, c, u)
+// ---------------------------------------------------------------------------
+__global__ void k_tdm_sequential(double* __restrict__ q, double* __restrict__ v,
+                                 double* __restrict__ q1, int nx, int ny,
+                                 const double* __restrict__ evals, double l, double u,
+                                 int pin_mode0) {
+  const int mo = blockIdx.x * blockDim.x + threadIdx.x;
+  if (mo >= nx) return;
+  const double c = -l - u + evals[mo];
+  const int n = ny, N = ny - 1;
+#define Q(r) q[(size_t)(r) * nx + mo]
+#define V(r) v[(size_t)(r) * nx + mo]
+#define Q1(r) q1[(size_t)(r) * nx + mo]
+  for (int r = 0; r < N; r++) Q1(r) = (r == 0) ? -l : (r == n - 2) ? -u : 0.;
+  V(0) = u / c;
+  Q1(0) = Q1(0) / c;
+  Q(0) = Q(0) / c;
+  for (int r = 1; r < N - 1; r++) {
+    const double val = 1. / (c - l * V(r - 1));
+    V(r) = val * u;
+    Q1(r) = val * (Q1(r) - l * Q1(r - 1));
+    Q(r) = val * (Q(r) - l * Q(r - 1));
+  }
+  {
+    const double val = c - l * V(N - 2);
+    if (fabs(val) > DBL_EPSILON) {
+      Q1(N - 1) = 1. / val * (Q1(N - 1) - l * Q1(N - 2));
+      Q(N - 1) = 1. / val * (Q(N - 1) - l * Q(N - 2));
+    } else {
+      Q1(N - 1) = 0.;
+      Q(N - 1) = 0.;
+    }
+  }
+  for (int r = N - 2; r >= 0; r--) {
+    Q1(r) -= V(r) * Q1(r + 1);
+    Q(r) -= V(r) * Q(r + 1);
+  }
+  const double num = Q(n - 1) - u * Q(0) - l * Q(n - 2);
+  const double den = c + u * Q1(0) + l * Q1(n - 2);
+  const double xl = ((pin_mode0 && mo == 0) || fabs(den) < DBL_EPSILON) ? 0. : num / den;
+  Q(n - 1) = xl;
+  for (int r = 0; r < n - 1; r++) Q(r) = Q(r) + xl * Q1(r);
+#undef Q
+#undef V
+#undef Q1
+}
+
+// ---------------------------------------------------------------------------
+// plan
+// ---------------------------------------------------------------------------
+static int plan_build(PoissonPlan* p, const domain_t* d) {
+  const int nx = p->nx, ny = p->ny, m = p->m;
+  const size_t cells = (size_t)nx * ny;
+  SBFS_HIP(hipMalloc(&p->buf, sizeof(double) * cells));
+  SBFS_HIP(hipMalloc(&p->v, sizeof(double) * cells));
+  SBFS_HIP(hipMalloc(&p->q1, sizeof(double) * cells));
+  SBFS_HIP(hipMalloc(&p->evals, sizeof(double) * nx));
+  SBFS_HIP(hipMalloc(&p->tw, sizeof(double2) * (m > 0 ? m : 1)));
+  SBFS_HIP(hipMalloc(&p->t1, sizeof(double2) * (m + 1)));
+  SBFS_HIP(hipMalloc(&p->t2, sizeof(double2) * nx));
+  // eigenvalues of the x operator, compute_potential.c:233-261
+  std::vector<double> ev(nx);
+  const double sl = 2. * (double)d->glsizes[0];
+  const double gsz = d->lengths[0] / (double)d->glsizes[0];
+  for (int i = 0; i < nx; i++) {
+    const double s = sin(kPiPoisson * (double)i / sl);
+    ev[i] = -4. / (gsz * gsz) * (s * s);
+  }
+  std::vector<double> tw(2 * (m > 0 ? m : 1)), t1(2 * (m + 1)), t2(2 * nx);
+  for (int e = 0; e < m; e++) twiddle(e, m, &tw[2 * e], &tw[2 * e + 1]);
+  for (int k = 0; k <= m; k++) twiddle(k, nx, &t1[2 * k], &t1[2 * k + 1]);
+  for (int k = 0; k < nx; k++) twiddle(k, 4 * (size_t)nx, &t2[2 * k], &t2[2 * k + 1]);
+  SBFS_HIP(hipMemcpy(p->evals, ev.data(), sizeof(double) * nx, hipMemcpyHostToDevice));
+  SBFS_HIP(hipMemcpy(p->tw, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
+  SBFS_HIP(hipMemcpy(p->t1, t1.data(), sizeof(double) * t1.size(), hipMemcpyHostToDevice));
+  SBFS_HIP(hipMemcpy(p->t2, t2.data(), sizeof(double) * t2.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+
+static PoissonPlan* plan_get(const domain_t* d, DomainCtx* c) {
+  const int nx = (int)c->nx, ny = (int)c->ny;
+  if (c->poisson && c->poisson->nx == nx && c->poisson->ny == ny) return c->poisson;
+  if (c->poisson) {
+    poisson_destroy(c->poisson);
+    c->poisson = nullptr;
+  }
+  if (nx < 2 || (nx & (nx - 1)) != 0 || nx > 4096) {
+    set_error("Poisson solver: Nx = %d must be a power of two in [2, 4096]", nx);
+    return nullptr;
+  }
+  if (ny < 3) {
+    set_error("Poisson solver: Ny = %d must be at least 3", ny);
+    return nullptr;
+  }
+  PoissonPlan* p = new PoissonPlan();
+  p->nx = nx;
+  p->ny = ny;
+  p->m = nx / 2;
+  p->lx = d->lengths[0];
+  if (plan_build(p, d)) {
+    poisson_destroy(p);
+    return nullptr;
+  }
+  c->poisson = p;
+  return p;
+}
+
+static int row_threads(int m) { return m >= 1024 ? 512 : 256; }
+
+int poisson_solve(const domain_t* d, DomainCtx* c, double dt_old, double dt_new, fluid_t* f) {
+  PoissonPlan* p = plan_get(d, c);
+  if (!p) return 1;
+  const int nx = p->nx, ny = p->ny, m = p->m;
+  // coefficients incl. the FFT normalisation, compute_potential.c:303-314
+  const double norm = 2. * (double)d->glsizes[0];
+  const double refden = f->refden;
+  const double c0 = 1. / norm / dt_new * refden;
+  const double c1 = 1. / norm * -1. / dt_new * dt_old;
+  const size_t lds = sizeof(double2) * 2 * (size_t)(m > 0 ? m : 1);
+  hipLaunchKernelGGL(k_dct2_rhs, dim3(ny), dim3(row_threads(m)), lds, stream(),
+                     (const double*)f->ux.data, (const double*)f->uy.data,
+                     (const double*)f->psi[0].data, (const double*)f->den[1].data, c->hxxf,
+                     c->jdxf, c->jdxc, c->hy, refden, c0, c1, nx, m, p->tw, p->t1, p->t2, p->buf);
+  if (check_launch("poisson: rhs + DCT-II")) return 1;
+  // tridiagonal matrix l = u = 1/hy^2 (compute_potential.c:173-176)
+  const double l = 1. / d->hy / d->hy;
+  const int pin = options().gauge == SBFS_GAUGE_PINNED ? 1 : 0;
+  const int L = partition_chunk(ny, options().chunk);
+  if (options().tdm == SBFS_TDM_PARTITIONED && L != 0 && pin) {
+    if (p->L != L && partition_setup(p, d, L)) return 1;
+    if (partition_solve(p, pin)) return 1;
+  } else {
+    hipLaunchKernelGGL(k_tdm_sequential, dim3(cdiv(nx, 64)), dim3(64), 0, stream(), p->buf, p->v,
+                       p->q1, nx, ny, p->evals, l, l, pin);
+    if (check_launch("poisson: tridiagonal")) return 1;
+  }
+  hipLaunchKernelGGL(k_dct3_psi, dim3(ny), dim3(row_threads(m)), lds, stream(),
+                     (const double*)p->buf, (double*)f->psi[1].data, nx, ny, m, p->tw, p->t1,
+                     p->t2);
+  return check_launch("poisson: DCT-III + psi");
+}
+
+}  // namespace sbfs
+
+using namespace sbfs;
+
+// ---------------------------------------------------------------------------
+// transform entry points for tests (device in / out, howmany rows of n)
+// ---------------------------------------------------------------------------
+extern "C" int sbfs_dct_rows(int inverse, size_t n, size_t howmany, const double* in_host,
+                             double* out_host) {
+  if (init_runtime()) return 1;
+  SBFS_REQUIRE(n >= 2 && (n & (n - 1)) == 0 && n <= 4096, "n must be a power of two in [2, 4096]");
+  const int m = (int)(n / 2);
+  std::vector<double> tw(2 * m), t1(2 * (m + 1)), t2(2 * n);
+  for (int e = 0; e < m; e++) twiddle(e, m, &tw[2 * e], &tw[2 * e + 1]);
+  for (int k = 0; k <= m; k++) twiddle(k, n, &t1[2 * k], &t1[2 * k + 1]);
+  for (size_t k = 0; k < n; k++) twiddle(k, 4 * n, &t2[2 * k], &t2[2 * k + 1]);
+  double *din = nullptr, *dout = nullptr;
+  double2 *dtw = nullptr, *dt1 = nullptr, *dt2 = nullptr;
+  const size_t bytes = sizeof(double) * n * howmany;
+  SBFS_HIP(hipMalloc(&din, bytes));
+  SBFS_HIP(hipMalloc(&dout, bytes));
+  SBFS_HIP(hipMalloc(&dtw, sizeof(double) * tw.size()));
+  SBFS_HIP(hipMalloc(&dt1, sizeof(double) * t1.size()));
+  SBFS_HIP(hipMalloc(&dt2, sizeof(double) * t2.size()));
+  SBFS_HIP(hipMemcpy(din, in_host, bytes, hipMemcpyHostToDevice));
+  SBFS_HIP(hipMemcpy(dtw, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
+  SBFS_HIP(hipMemcpy(dt1, t1.data(), sizeof(double) * t1.size(), hipMemcpyHostToDevice));
+  SBFS_HIP(hipMemcpy(dt2, t2.data(), sizeof(double) * t2.size(), hipMemcpyHostToDevice));
+  const size_t lds = sizeof(double2) * 2 * (size_t)m;
+  if (inverse)
+    hipLaunchKernelGGL(k_dct3_rows, dim3((unsigned)howmany), dim3(row_threads(m)), lds, stream(), din, dout,
+                       (int)n, m, dtw, dt1, dt2);
+  else
+    hipLaunchKernelGGL(k_dct2_rows, dim3((unsigned)howmany), dim3(row_threads(m)), lds, stream(), din, dout,
+                       (int)n, m, dtw, dt1, dt2);
+  int rc = check_launch("sbfs_dct_rows");
+  if (!rc) {
+    SBFS_HIP(hipStreamSynchronize(stream()));
+    SBFS_HIP(hipMemcpy(out_host, dout, bytes, hipMemcpyDeviceToHost));
+  }
+  (void)hipFree(din);
+  (void)hipFree(dout);
+  (void)hipFree(dtw);
+  (void)hipFree(dt1);
+  (void)hipFree(dt2);
+  return rc;
+}

@@ -1,0 +1,35 @@
+// Poisson solver state shared by poisson.hip and tdm_partition.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "sbfs_internal.hpp"
+
+namespace sbfs {
+
+struct PoissonPlan {
+  int nx = 0, ny = 0, m = 0;
+  double lx = 0.;
+  double* buf = nullptr;     // nx * ny, modes r[j][m]
+  double* v = nullptr;       // nx * ny, sequential-solver factors
+  double* q1 = nullptr;      // nx * ny, sequential-solver spike
+  double* evals = nullptr;   // nx
+  double2* tw = nullptr;     // m      FFT twiddles e^{-2 pi i e/m}
+  double2* t1 = nullptr;     // m + 1  e^{-2 pi i k/n}
+  double2* t2 = nullptr;     // n      e^{-i pi k/(2n)}
+  // partitioned solver
+  int L = 0, P = 0;
+  double a = 0.;             // off-diagonal 1/hy^2
+  double* ptab = nullptr;    // [4][K][nx]: inv, v, w, z per (row-in-chunk, mode)
+  double* pred = nullptr;    // [3][nx]: reduced lower / diagonal / upper
+  double* gfl = nullptr;     // [2P][nx]: first / last interior of each chunk
+  double* X = nullptr;       // [P][nx]: separators
+  double* rv = nullptr;      // [P][nx]: reduced-solver factors
+  double* rq1 = nullptr;     // [P][nx]: reduced-solver spike
+};
+
+int partition_chunk(int n, int requested);
+int partition_setup(PoissonPlan* p, const domain_t* d, int L);
+int partition_solve(PoissonPlan* p, int pin_mode0);
+
+}  // namespace sbfs

@@ -1,0 +1,160 @@
+// Internal declarations of libsbfs: device context, layouts, error handling.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "domain.h"
+#include "array.h"
+#include "fluid.h"
+#include "interface.h"
+
+namespace sbfs {
+
+// ---------------------------------------------------------------------------
+// errors: every entry point returns non-zero and records a message
+// ---------------------------------------------------------------------------
+void set_error(const char* fmt, ...);
+const char* last_error();
+
+#define SBFS_HIP(expr)                                                        \
+  do {                                                                        \
+    hipError_t sbfs_e_ = (expr);                                              \
+    if (sbfs_e_ != hipSuccess) {                                              \
+      ::sbfs::set_error("%s:%d %s: %s", __FILE__, __LINE__, #expr,             \
+                        hipGetErrorString(sbfs_e_));                          \
+      return 1;                                                               \
+    }                                                                         \
+  } while (0)
+
+#define SBFS_REQUIRE(cond, ...)                                               \
+  do {                                                                        \
+    if (!(cond)) {                                                            \
+      ::sbfs::set_error(__VA_ARGS__);                                         \
+      return 1;                                                               \
+    }                                                                         \
+  } while (0)
+
+// check the launch status of the kernels just queued
+int check_launch(const char* what);
+
+// ---------------------------------------------------------------------------
+// field layouts (reference include/array_macros/): x fastest, 1-based
+// interior indices, per-field ghost widths.  `nx` is the interior width.
+// ---------------------------------------------------------------------------
+struct L {
+  // ux, visux, fluxx, txy, dvof: i = 1..nx+1, j = 0..ny+1 (dvof: ..ny+2)
+  __host__ __device__ static inline size_t xf(int i, int j, int nx) {
+    return (size_t)(i - 1) + (size_t)(nx + 1) * (size_t)j;
+  }
+  // uy, p, psi, den, visuy, curv, fluxy: i = 0..nx+1, j = 0..ny+1
+  __host__ __device__ static inline size_t cc(int i, int j, int nx) {
+    return (size_t)i + (size_t)(nx + 2) * (size_t)j;
+  }
+  // vof: i = 0..nx+1, j = -1..ny+2
+  __host__ __device__ static inline size_t vof(int i, int j, int nx) {
+    return (size_t)i + (size_t)(nx + 2) * (size_t)(j + 1);
+  }
+  // txx, tyy, normal: i = 1..nx, j = 0..ny+1
+  __host__ __device__ static inline size_t cx(int i, int j, int nx) {
+    return (size_t)(i - 1) + (size_t)nx * (size_t)j;
+  }
+  // srcux, ifrcx: i = 2..nx, j = 1..ny
+  __host__ __device__ static inline size_t fx(int i, int j, int nx) {
+    return (size_t)(i - 2) + (size_t)(nx - 1) * (size_t)(j - 1);
+  }
+  // srcuy, ifrcy, src: i = 1..nx, j = 1..ny
+  __host__ __device__ static inline size_t ci(int i, int j, int nx) {
+    return (size_t)(i - 1) + (size_t)nx * (size_t)(j - 1);
+  }
+};
+
+// ghost widths of each field: {{x lo, x hi}, {y lo, y hi}}
+struct Nadds {
+  int v[2][2];
+};
+extern const Nadds NA_UX, NA_UY, NA_P, NA_DEN, NA_VISUX, NA_VISUY, NA_TXX,
+    NA_TXY, NA_TYY, NA_SRCUX, NA_SRCUY, NA_VOF, NA_IFRCX, NA_IFRCY, NA_DVOF,
+    NA_NORMAL, NA_CURV, NA_FLUXX, NA_FLUXY, NA_SRC;
+
+// verify that an array_t matches the layout the kernels assume
+int check_array(const domain_t* d, const array_t* a, const Nadds& na,
+                size_t elem, const char* name);
+
+// ---------------------------------------------------------------------------
+// constants (reference src/runge_kutta.c, src/interface/common.c)
+// ---------------------------------------------------------------------------
+struct RK {
+  double a, b, g;
+};
+RK rk(size_t stage);
+
+constexpr double kVofMin = 1.e-8;   // src/interface/common.c:14
+constexpr double kVofBeta = 1.;     // src/interface/common.c:13
+constexpr double kGaussP = 0.2886751345948129;  // src/interface/common.c:4-7
+constexpr double kGaussW = 0.5;                  // src/interface/common.c:8-11
+constexpr double kUyXm = 0.;        // src/param/boundary-condition.c:3
+constexpr double kUyXp = 0.;        // src/param/boundary-condition.c:4
+
+// ---------------------------------------------------------------------------
+// device state per domain
+// ---------------------------------------------------------------------------
+struct PoissonPlan;  // poisson.hip
+
+struct DomainCtx {
+  // identity of the host domain this mirrors
+  size_t nx = 0, ny = 0;
+  double lx = 0., ly = 0., hy = 0.;
+  uint64_t hash = 0;
+  // device copies of the metrics (1-based through -1 offsets, as the host)
+  double* hxxf = nullptr;  // nx+1
+  double* hxxc = nullptr;  // nx
+  double* jdxf = nullptr;  // nx+1
+  double* jdxc = nullptr;  // nx
+  double* xc = nullptr;    // nx+2
+  // scratch for reductions
+  double* red = nullptr;   // kRedSlots doubles
+  double* dt_dev = nullptr;  // device-resident dt (1 double)
+  double* host_pin = nullptr;  // pinned host staging (16 doubles)
+  PoissonPlan* poisson = nullptr;
+};
+
+constexpr int kRedBlocks = 1024;          // partial-reduction blocks
+constexpr int kRedSlots = kRedBlocks * 8 + 64;
+
+struct Options {
+  int path = 1;        // SBFS_PATH_FUSED
+  int gauge = 0;       // SBFS_GAUGE_PINNED
+  int tdm = 1;         // SBFS_TDM_PARTITIONED
+  int chunk = 32;
+  bool coefs_set = false;
+  double cadv = 0., cdif = 0., cint = 0.;
+};
+
+Options& options();
+hipStream_t stream();
+int init_runtime();
+// device context of a host domain (created / refreshed on demand)
+DomainCtx* domain_ctx(const domain_t* d);
+int load_dt_coefs();
+
+// grid-dimension helpers
+inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
+
+// ---------------------------------------------------------------------------
+// stage launchers (stages.hip, poisson.hip, step.hip); all queue on stream()
+// ---------------------------------------------------------------------------
+int launch_density(const domain_t* d, DomainCtx* c, const double* vof,
+                   double* den, double denr);
+int poisson_solve(const domain_t* d, DomainCtx* c, double dt_old,
+                  double dt_new, fluid_t* f);
+int compute_dt_device(const domain_t* d, DomainCtx* c, const fluid_t* f,
+                      const interface_t* s);
+void dt_limits(const domain_t* d, const fluid_t* f, const interface_t* s,
+               double* dif0, double* dif1, double* dint);
+
+}  // namespace sbfs
