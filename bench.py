@@ -1,0 +1,169 @@
+"""Throughput of the two-phase time step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--path fused|staged]
+                    [--workload swarm|rising] [--no-cpu-baseline]
+
+Workload (default): plain FP64 bubble swarm, 2048 x 4096 cells on a 1 x 2
+domain (walls in x, periodic y), 64 bubbles of diameter 24-48 cells placed by
+seeded dart throwing, Re=35, We=10, Fr=1, density / viscosity ratios 1e-3 /
+1e-2, fluid at rest.  A step is one full time step (decide_dt + 3 Runge-Kutta
+stages).  `value` is interior cells x steps / wall seconds of the timed loop,
+in Mcell-steps/s, with the state resident in HBM.
+
+`roofline` is for the kernel with the largest share of GPU time, timed with
+HIP events on the library stream during the timed loop; its algorithmic
+bytes per cell (distinct fields read + written once, FP64) are listed in
+BYTES_PER_CELL.  `cpu_baseline` times the CPU oracle (the FP64 C
+restatement, OpenMP) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+from simplebubblyflowsolver_amd import configs, solver  # noqa: E402
+
+METRIC = "Mcell-steps/s, mixed+learned vs plain FP64 at <=1% rise-velocity error"
+HBM_PEAK_GBS = 8000.0
+
+# compulsory HBM bytes per interior cell of one launch (FP64 fields read once
+# and written once); the band-only normal is counted in full
+BYTES_PER_CELL = {
+    "k_density": 16, "k_visux": 16, "k_visuy": 16, "k_dvof": 24, "k_normal_curv": 56,
+    "k_force": 40, "k_fluxx": 48, "k_fluxy": 48, "k_vof_advance": 48, "k_mass_flux": 48,
+    "k_stress": 56, "k_rhs_ux": 80, "k_rhs_uy": 72, "k_predict_ux": 56, "k_predict_uy": 56,
+    "k_dct2_rhs": 40, "k_tdm_sequential": 48, "k_part_reduce": 8, "k_part_separators": 0,
+    "k_part_expand": 16, "k_dct3_psi": 16, "k_correct_ux": 40, "k_correct_uy": 40,
+    "k_update_p": 24, "k_dt_partial": 16, "k_dt_final": 0,
+    # fused path
+    "k_pre": 0, "k_post": 0,
+}
+
+
+def workload(name: str) -> configs.Workload:
+    if name == "swarm":
+        return configs.bubble_swarm(2048, 4096, 64, steps=200, seed=1)
+    if name == "rising":
+        return configs.rising_bubble(128, 256, steps=2000)
+    if name == "large":
+        return configs.large_swarm()
+    raise SystemExit(f"unknown workload {name}")
+
+
+def profile_entries(lib) -> list:
+    lib.sbfs_profile_entry.argtypes = [C.c_int, C.c_char_p, C.c_size_t, C.POINTER(C.c_int),
+                                       C.POINTER(C.c_double)]
+    out = []
+    for k in range(lib.sbfs_profile_count()):
+        name = C.create_string_buffer(128)
+        cnt = C.c_int(0)
+        ms = C.c_double(0.0)
+        lib.sbfs_profile_entry(k, name, 128, C.byref(cnt), C.byref(ms))
+        out.append((name.value.decode(), cnt.value, ms.value))
+    return out
+
+
+def cpu_baseline(w: configs.Workload, steps: int, threads: int) -> dict:
+    import oracle
+    oracle.build()
+    oracle.set_threads(threads)
+    oracle.set_poisson(dct=oracle.DCT_FFT, tdm=oracle.TDM_SEQUENTIAL)
+    o = oracle.new_case(w.nx, w.ny, w.lx, w.ly, w.Re, w.Fr, w.denr, w.visr, w.We)
+    o.load_initial_condition(w.vof())
+    o.integrate()  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.integrate()
+    t = time.perf_counter() - t0
+    o.close()
+    return {"value": round(w.cells * steps / t / 1e6, 3), "unit": "Mcell-steps/s",
+            "cores": threads, "kind": "port",
+            "sample": f"{steps} steps of {w.name} after 1 warm-up step, "
+                      f"CPU oracle (FP64 C restatement, OpenMP, {threads} threads), {t:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--path", choices=["fused", "staged"], default="fused")
+    ap.add_argument("--workload", default="swarm")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--no-profile", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the time step runs on a single MI355X "
+              "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
+        return 2
+
+    w = workload(a.workload)
+    be = solver.product_backend()
+    lib = be.lib
+    be.set_dt_coefs(*configs.DT_COEFS)
+    lib.sbfs_set_path(1 if a.path == "fused" else 0)
+    g = solver.Case(be, w.nx, w.ny, w.lx, w.ly, w.Re, w.Fr, w.denr, w.visr, w.We)
+    g.load_initial_condition(w.vof())
+    d0 = g.diagnostics()
+    for _ in range(a.warmup):
+        g.integrate()
+    be.check(lib.sbfs_synchronize(), "sync")
+    if not a.no_profile:
+        be.check(lib.sbfs_profile_start(), "profile_start")
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        g.integrate()
+    be.check(lib.sbfs_synchronize(), "sync")
+    t = time.perf_counter() - t0
+    prof = []
+    if not a.no_profile:
+        be.check(lib.sbfs_profile_stop(), "profile_stop")
+        prof = profile_entries(lib)
+    d1 = g.diagnostics()
+    g.close()
+
+    value = w.cells * a.steps / t / 1e6
+    roof = None
+    if prof:
+        name, cnt, ms = max(prof, key=lambda e: e[2])
+        avg_s = ms / cnt / 1e3
+        bpc = BYTES_PER_CELL.get(name)
+        achieved = bpc * w.cells / avg_s / 1e9 if bpc else None
+        roof = {"bound": "hbm", "kernel": name, "avg_us": round(avg_s * 1e6, 2),
+                "bytes_per_cell": bpc,
+                "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                "traffic": None,
+                "share_of_gpu_time": round(ms / sum(e[2] for e in prof), 3)}
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Mcell-steps/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(t / a.steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": w.name, "grid": [w.nx, w.ny], "bubbles": len(w.bubbles),
+                   "Re": w.Re, "We": w.We, "precision": "plain FP64", "path": a.path,
+                   "volume_drift": abs(d1["volume"] - d0["volume"]) / d0["volume"],
+                   "divmax": d1["divmax"]},
+        "roofline": roof,
+        "kernels": [{"name": n, "launches": c, "total_ms": round(m, 3)} for n, c, m in prof],
+    }
+    if not a.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(w, a.cpu_steps, threads)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

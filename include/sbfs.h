@@ -85,6 +85,14 @@ extern int sbfs_diagnostics(const domain_t * domain, const fluid_t * fluid,
                             const interface_t * interface,
                             double out[SBFS_NDIAG]);
 
+/* kernel timing: while on, every launch is bracketed by HIP events on the
+ * library stream; stop() synchronises and aggregates per kernel name */
+extern int sbfs_profile_start(void);
+extern int sbfs_profile_stop(void);
+extern int sbfs_profile_count(void);
+extern int sbfs_profile_entry(int k, char * name, size_t len, int * count,
+                              double * total_ms);
+
 /* the Poisson solver's row transforms on their own: FFTW REDFT10
  * (inverse = 0) or REDFT01 (inverse = 1) of `howmany` rows of n doubles,
  * n a power of two <= 4096; host buffers in and out */

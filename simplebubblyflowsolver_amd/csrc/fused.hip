@@ -1,30 +1,605 @@
-// Fused integrator (placeholder until the fused kernels land): runs the
-// staged composition.
+// Fused integrator: per Runge-Kutta stage
+//
+//   k_pre        one LDS tile of cells per workgroup: corner gradients,
+//                normals, curvature, THINC fluxes, mass fluxes, the vof
+//                update, surface force, stress tensor, explicit sources and
+//                the predicted velocity (every stage of the reference from
+//                fluid_compute_density to fluid_predict_field).  Halo values
+//                are recomputed per tile instead of being stored, so only
+//                vof, u, p and the beta sources are read and only the new
+//                vof, the predicted u and the alpha sources are written.
+//   k_dct2_rhs_fused  Poisson right-hand side + DCT-II per row
+//   k_part_*     partitioned tridiagonal solve (tdm_partition.hip)
+//   k_dct3_psi   DCT-III per row -> psi[1]
+//   k_post       projection of u and the pressure update
+//
+// Every value is computed with the same device functions and operation
+// order as the staged kernels (stencils.hpp), so the fused path reproduces
+// the staged path bitwise (up to the sign of exact zeros; see k_pre's beta
+// term at stage 0).  dt stays in device memory for the whole step.
+#include <cfloat>
+#include <cmath>
+
 #include "decide_dt.h"
 #include "fluid_solver.h"
 #include "integrate.h"
 #include "interface_solver.h"
+#include "poisson.hpp"
+#include "sbfs.h"
 #include "sbfs_internal.hpp"
+#include "stencils.hpp"
 
 namespace sbfs {
 
-int integrate_fused(const domain_t* domain, fluid_t* fluid, interface_t* interface, double* dt) {
-  if (decide_dt(domain, fluid, interface, dt)) return 1;
-  for (size_t k = 0; k < 3; k++) {
-    if (fluid_compute_density(domain, fluid, interface, 0)) return 1;
-    if (fluid_compute_viscosity(domain, fluid, interface)) return 1;
-    if (interface_compute_curvature_tensor(domain, interface)) return 1;
-    if (interface_compute_force(domain, fluid, interface)) return 1;
-    if (interface_update_vof(domain, k, *dt, fluid, interface)) return 1;
-    if (fluid_compute_density(domain, fluid, interface, 1)) return 1;
-    if (interface_compute_mass_flux(domain, fluid, interface)) return 1;
-    if (fluid_predict_field(domain, k, *dt, fluid, interface)) return 1;
-    const double dt_old = rk((k + 2) % 3).g * (*dt);
-    const double dt_new = rk(k).g * (*dt);
-    if (fluid_compute_potential(domain, dt_old, dt_new, fluid)) return 1;
-    if (fluid_correct_velocity(domain, dt_old, dt_new, fluid)) return 1;
-    if (fluid_update_pressure(domain, fluid)) return 1;
+struct PreArgs {
+  // inputs (state at the start of the stage)
+  const double* vof;
+  const double* ux;
+  const double* uy;
+  const double* p;
+  const double* srcb_vof;
+  const double* srcb_ux;
+  const double* srcb_uy;
+  // outputs
+  double* vofn;
+  double* uxs;
+  double* uys;
+  double* srca_vof;
+  double* srca_ux;
+  double* srca_uy;
+  double* den1;  // written on the last stage only (else null)
+  double* txy;   // written on the last stage only (else null)
+  // metrics
+  const double* hxxf;
+  const double* hxxc;
+  const double* jdxf;
+  const double* jdxc;
+  double hy;
+  // physics
+  double denr, dlo, dhi, visr, vlo, vhi;
+  double refdeninv, tension, diffusivity, grav;
+  // time stepping
+  const double* dt;
+  double ca, cb, cg;
+  int stage;
+  int nx, ny;
+};
+
+__device__ __forceinline__ int wrap_row(int j, int ny) {
+  int r = (j - 1) % ny;
+  if (r < 0) r += ny;
+  return r + 1;
+}
+
+// tile of TX x TY output cells; regions (inclusive, relative to i0, j0):
+//   vof     [i0-3, i0+TX+1] x [j0-3, j0+TY+1]
+//   corner  [i0-2, i0+TX+1] x [j0-2, j0+TY+1]
+//   normal  [i0-2, i0+TX  ] x [j0-2, j0+TY  ]
+//   curv    [i0-1, i0+TX-1] x [j0-1, j0+TY-1]
+//   fx      [i0-1, i0+TX  ] x [j0-1, j0+TY-1]   (vof flux and mass flux)
+//   fy      [i0-1, i0+TX-1] x [j0-1, j0+TY  ]
+//   vofn    [i0-1, i0+TX-1] x [j0-1, j0+TY-1]   (new vof and its source)
+//   txx     [i0-1, i0+TX-1] x [j0  , j0+TY-1]
+//   tyy     [i0  , i0+TX-1] x [j0-1, j0+TY-1]
+//   txy     [i0  , i0+TX  ] x [j0  , j0+TY  ]
+template <int TX, int TY>
+struct PreTile {
+  static constexpr int VX = TX + 5, VY = TY + 5;
+  static constexpr int CX = TX + 4, CY = TY + 4;
+  static constexpr int NX = TX + 3, NY = TY + 3;
+  static constexpr int KX = TX + 1, KY = TY + 1;
+  static constexpr int FXX = TX + 2, FXY = TY + 1;
+  static constexpr int FYX = TX + 1, FYY = TY + 2;
+  static constexpr int WX = TX + 1, WY = TY + 1;
+  static constexpr int TXXX = TX + 1, TXXY = TY;
+  static constexpr int TYYX = TX, TYYY = TY + 1;
+  static constexpr int TXYX = TX + 1, TXYY = TY + 1;
+};
+
+template <int TX, int TY>
+__global__ __launch_bounds__(TX* TY) void k_pre(PreArgs a) {
+  using T = PreTile<TX, TY>;
+  // phase A arrays (vof, corners, normals, curvature)
+  __shared__ double s_vof[T::VY][T::VX];
+  __shared__ double s_cx[T::CY][T::CX];
+  __shared__ double s_cy[T::CY][T::CX];
+  __shared__ double s_n[3][T::NY][T::NX];
+  __shared__ double s_curv[T::KY][T::KX];
+  // phase B arrays
+  __shared__ double s_fx[T::FXY][T::FXX];
+  __shared__ double s_mfx[T::FXY][T::FXX];
+  __shared__ double s_fy[T::FYY][T::FYX];
+  __shared__ double s_mfy[T::FYY][T::FYX];
+  __shared__ double s_vofn[T::WY][T::WX];
+  __shared__ double s_src[T::WY][T::WX];
+  __shared__ double s_txx[T::TXXY][T::TXXX];
+  __shared__ double s_tyy[T::TYYY][T::TYYX];
+  __shared__ double s_txy[T::TXYY][T::TXYX];
+
+  const int nx = a.nx, ny = a.ny;
+  const int i0 = 1 + blockIdx.x * TX;
+  const int j0 = 1 + blockIdx.y * TY;
+  const int tid = threadIdx.x + threadIdx.y * TX;
+  constexpr int NT = TX * TY;
+  const double hy = a.hy;
+  auto UX = [&](int i, int j) { return a.ux[L::xf(i, wrap_row(j, ny), nx)]; };
+  auto UY = [&](int i, int j) { return a.uy[L::cc(i, wrap_row(j, ny), nx)]; };
+  auto VOF = [&](int i, int j) -> double { return s_vof[j - (j0 - 3)][i - (i0 - 3)]; };
+  auto CRN = [&](int i, int j) -> V2 {
+    return {s_cx[j - (j0 - 2)][i - (i0 - 2)], s_cy[j - (j0 - 2)][i - (i0 - 2)]};
+  };
+  auto NRM = [&](int i, int j) -> V3 {
+    const int y = j - (j0 - 2), x = i - (i0 - 2);
+    return {s_n[0][y][x], s_n[1][y][x], s_n[2][y][x]};
+  };
+  auto CURV = [&](int i, int j) -> double { return s_curv[j - (j0 - 1)][i - (i0 - 1)]; };
+  auto FX = [&](int i, int j) -> double { return s_fx[j - (j0 - 1)][i - (i0 - 1)]; };
+  auto MFX = [&](int i, int j) -> double { return s_mfx[j - (j0 - 1)][i - (i0 - 1)]; };
+  auto FY = [&](int i, int j) -> double { return s_fy[j - (j0 - 1)][i - (i0 - 1)]; };
+  auto MFY = [&](int i, int j) -> double { return s_mfy[j - (j0 - 1)][i - (i0 - 1)]; };
+  auto VOFN = [&](int i, int j) -> double { return s_vofn[j - (j0 - 1)][i - (i0 - 1)]; };
+  auto TXX = [&](int i, int j) -> double { return s_txx[j - j0][i - (i0 - 1)]; };
+  auto TYY = [&](int i, int j) -> double { return s_tyy[j - (j0 - 1)][i - i0]; };
+  auto TXY = [&](int i, int j) -> double { return s_txy[j - j0][i - i0]; };
+  auto DEN0 = [&](int i, int j) { return mix_clamp(a.denr, a.dlo, a.dhi, VOF(i, j)); };
+  auto DEN1 = [&](int i, int j) { return mix_clamp(a.denr, a.dlo, a.dhi, VOFN(i, j)); };
+  // viscosity at x faces / y faces (src/fluid/viscosity.c:37-83)
+  auto VISUX = [&](int i, int j) {
+    double v;
+    if (i == 1) v = VOF(0, j);
+    else if (i == nx + 1) v = VOF(nx + 1, j);
+    else v = 0.5 * VOF(i - 1, j) + 0.5 * VOF(i, j);
+    return mix_clamp(a.visr, a.vlo, a.vhi, v);
+  };
+  auto VISUY = [&](int i, int j) {
+    return mix_clamp(a.visr, a.vlo, a.vhi, 0.5 * VOF(i, j - 1) + 0.5 * VOF(i, j));
+  };
+
+  // --- vof tile ----------------------------------------------------------
+  for (int k = tid; k < T::VX * T::VY; k += NT) {
+    const int x = k % T::VX, y = k / T::VX;
+    const int i = i0 - 3 + x, j = j0 - 3 + y;
+    s_vof[y][x] = (i >= 0 && i <= nx + 1) ? a.vof[L::vof(i, wrap_row(j, ny), nx)] : 0.;
   }
+  __syncthreads();
+  // --- corner unit gradients (curvature_tensor.c:16-49) --------------------
+  for (int k = tid; k < T::CX * T::CY; k += NT) {
+    const int x = k % T::CX, y = k / T::CX;
+    const int i = i0 - 2 + x, j = j0 - 2 + y;
+    V2 g = {0., 0.};
+    if (i >= 1 && i <= nx + 1) {
+      const double mm = VOF(i - 1, j - 1), pm = VOF(i, j - 1), mp = VOF(i - 1, j), pp = VOF(i, j);
+      if (!(mm == pm && mm == mp && mm == pp)) g = corner_gradient(mm, pm, mp, pp, a.hxxf[i - 1], hy);
+    }
+    s_cx[y][x] = g.x;
+    s_cy[y][x] = g.y;
+  }
+  __syncthreads();
+  // --- normals inside the band (:51-97) and curvature (:100-135) ----------
+  for (int k = tid; k < T::NX * T::NY; k += NT) {
+    const int x = k % T::NX, y = k / T::NX;
+    const int i = i0 - 2 + x, j = j0 - 2 + y;
+    V3 n = {0., 0., 0.};
+    if (i >= 1 && i <= nx) {
+      const double v = VOF(i, j);
+      if (in_band(v)) n = cell_normal(CRN(i, j), CRN(i + 1, j), CRN(i, j + 1), CRN(i + 1, j + 1), a.hxxc[i - 1], hy, v);
+    }
+    s_n[0][y][x] = n.x;
+    s_n[1][y][x] = n.y;
+    s_n[2][y][x] = n.z;
+  }
+  for (int k = tid; k < T::KX * T::KY; k += NT) {
+    const int x = k % T::KX, y = k / T::KX;
+    const int i = i0 - 1 + x, j = j0 - 1 + y;
+    double c = 0.;
+    if (i >= 1 && i <= nx)
+      c = curvature(CRN(i, j), CRN(i + 1, j), CRN(i, j + 1), CRN(i + 1, j + 1), a.hxxf[i - 1], a.hxxf[i],
+                    a.jdxf[i - 1], a.jdxc[i - 1], a.jdxf[i], hy);
+    s_curv[y][x] = c;
+  }
+  __syncthreads();
+  // --- THINC fluxes and mass fluxes (update/fluxx.c, fluxy.c, mass_flux.c)
+  for (int k = tid; k < T::FXX * T::FXY; k += NT) {
+    const int x = k % T::FXX, y = k / T::FXX;
+    const int i = i0 - 1 + x, j = j0 - 1 + y;
+    double f = 0., mf = 0.;
+    if (i >= 1 && i <= nx + 1) {
+      const double vel = UX(i, j);
+      if (i != 1 && i != nx + 1) {
+        const int ii = vel < 0. ? i : i - 1;
+        const double v = VOF(ii, j);
+        f = thinc_flux_x(vel, v, in_band(v) ? NRM(ii, j) : V3{0., 0., 0.});
+      }
+      mf = vel + (a.denr - 1.) * f;
+    }
+    s_fx[y][x] = f;
+    s_mfx[y][x] = mf;
+  }
+  for (int k = tid; k < T::FYX * T::FYY; k += NT) {
+    const int x = k % T::FYX, y = k / T::FYX;
+    const int i = i0 - 1 + x, j = j0 - 1 + y;
+    double f = 0., mf = 0.;
+    if (i >= 0 && i <= nx + 1) {
+      const double vel = UY(i, j);
+      if (i != 0 && i != nx + 1) {
+        const int jj = vel < 0. ? j : j - 1;
+        const double v = VOF(i, jj);
+        f = thinc_flux_y(vel, v, in_band(v) ? NRM(i, jj) : V3{0., 0., 0.});
+      }
+      mf = vel + (a.denr - 1.) * f;
+    }
+    s_fy[y][x] = f;
+    s_mfy[y][x] = mf;
+  }
+  // --- stress tensor (predict/txx.c, txy.c, tyy.c) -------------------------
+  for (int k = tid; k < T::TXXX * T::TXXY; k += NT) {
+    const int x = k % T::TXXX, y = k / T::TXXX;
+    const int i = i0 - 1 + x, j = j0 + y;
+    double t = 0.;
+    if (i >= 1 && i <= nx) {
+      const double vx = 0.5 * VISUX(i, j) + 0.5 * VISUX(i + 1, j);
+      const double l = 1. / a.hxxc[i - 1] * (-UX(i, j) + UX(i + 1, j));
+      t = vx * l + vx * l;
+    }
+    s_txx[y][x] = t;
+  }
+  for (int k = tid; k < T::TYYX * T::TYYY; k += NT) {
+    const int x = k % T::TYYX, y = k / T::TYYX;
+    const int i = i0 + x, j = j0 - 1 + y;
+    double t = 0.;
+    if (i <= nx) {
+      const double vy = 0.5 * VISUY(i, j) + 0.5 * VISUY(i, j + 1);
+      const double l = 1. / hy * (-UY(i, j) + UY(i, j + 1));
+      t = vy * l + vy * l;
+    }
+    s_tyy[y][x] = t;
+  }
+  for (int k = tid; k < T::TXYX * T::TXYY; k += NT) {
+    const int x = k % T::TXYX, y = k / T::TXYX;
+    const int i = i0 + x, j = j0 + y;
+    double t = 0.;
+    if (i <= nx + 1) {
+      const double vx = 0.5 * VISUX(i, j - 1) + 0.5 * VISUX(i, j);
+      const double vy = 0.5 * VISUY(i - 1, j) + 0.5 * VISUY(i, j);
+      const double lxy = 1. / hy * (-UX(i, j - 1) + UX(i, j));
+      const double lyx = 1. / a.hxxf[i - 1] * (-UY(i - 1, j) + UY(i, j));
+      t = vx * lxy + vy * lyx;
+    }
+    s_txy[y][x] = t;
+  }
+  __syncthreads();
+  // --- vof update on the tile and its one-cell skirt (update/main.c:54-120)
+  const double dt = *a.dt;
+  const double dtca = dt * a.ca, dtcb = dt * a.cb;
+  for (int k = tid; k < T::WX * T::WY; k += NT) {
+    const int x = k % T::WX, y = k / T::WX;
+    const int i = i0 - 1 + x, j = j0 - 1 + y;
+    double v = 0., s = 0.;
+    if (i >= 1 && i <= nx) {
+      const double hm = a.hxxf[i - 1], hp = a.hxxf[i];
+      const double jm = a.jdxf[i - 1], j0c = a.jdxc[i - 1], jp = a.jdxf[i];
+      s = 1. / j0c * (+jm / hm * FX(i, j) - jp / hp * FX(i + 1, j) + j0c / hy * FY(i, j) - j0c / hy * FY(i, j + 1));
+      v = VOF(i, j);
+      v += dtca * s;
+      if (a.stage != 0) v += dtcb * a.srcb_vof[L::ci(i, wrap_row(j, ny), nx)];
+    }
+    s_vofn[y][x] = v;
+    s_src[y][x] = s;
+  }
+  __syncthreads();
+
+  // --- outputs of this thread's cell --------------------------------------
+  const int i = i0 + (int)threadIdx.x, j = j0 + (int)threadIdx.y;
+  if (i > nx || j > ny) return;
+  auto ivof = [nx](int p, int q) { return L::vof(p, q, nx); };
+  auto icc = [nx](int p, int q) { return L::cc(p, q, nx); };
+  auto ixf = [nx](int p, int q) { return L::xf(p, q, nx); };
+  {
+    const double v = VOFN(i, j);
+    store_with_halo(a.vofn, i, j, ny, 2, v, ivof);
+    if (i == 1) store_with_halo(a.vofn, 0, j, ny, 2, 0., ivof);
+    if (i == nx) store_with_halo(a.vofn, nx + 1, j, ny, 2, 0., ivof);
+    if (a.srca_vof) a.srca_vof[L::ci(i, j, nx)] = s_src[j - (j0 - 1)][i - (i0 - 1)];
+    if (a.den1) {
+      store_with_halo(a.den1, i, j, ny, 1, DEN1(i, j), icc);
+      if (i == 1) store_with_halo(a.den1, 0, j, ny, 1, DEN1(0, j), icc);
+      if (i == nx) store_with_halo(a.den1, nx + 1, j, ny, 1, DEN1(nx + 1, j), icc);
+    }
+    if (a.txy) {
+      store_with_halo(a.txy, i, j, ny, 1, TXY(i, j), ixf);
+      if (i == nx) store_with_halo(a.txy, nx + 1, j, ny, 1, TXY(nx + 1, j), ixf);
+    }
+  }
+  // ux at face i (predict/ux.c)
+  {
+    double u = 0.;
+    if (i >= 2) {
+      const double* hxxf = a.hxxf;
+      const double* hxxc = a.hxxc;
+      const double* jdxf = a.jdxf;
+      const double* jdxc = a.jdxc;
+      double s = 0.;
+      {
+        const double mm = 0.5 * jdxf[i - 2] / hxxf[i - 2] * MFX(i - 1, j) + 0.5 * jdxf[i - 1] / hxxf[i - 1] * MFX(i, j);
+        const double mp = 0.5 * jdxf[i - 1] / hxxf[i - 1] * MFX(i, j) + 0.5 * jdxf[i] / hxxf[i] * MFX(i + 1, j);
+        const double um = 0.5 * UX(i - 1, j) + 0.5 * UX(i, j);
+        const double up = 0.5 * UX(i, j) + 0.5 * UX(i + 1, j);
+        s -= 1. / jdxf[i - 1] * (-mm * um + mp * up);
+      }
+      {
+        const double mm = 0.5 * jdxc[i - 2] / hy * MFY(i - 1, j) + 0.5 * jdxc[i - 1] / hy * MFY(i, j);
+        const double mp = 0.5 * jdxc[i - 2] / hy * MFY(i - 1, j + 1) + 0.5 * jdxc[i - 1] / hy * MFY(i, j + 1);
+        const double um = 0.5 * UX(i, j - 1) + 0.5 * UX(i, j);
+        const double up = 0.5 * UX(i, j) + 0.5 * UX(i, j + 1);
+        s -= 1. / jdxf[i - 1] * (-mm * um + mp * up);
+      }
+      s += a.diffusivity / jdxf[i - 1] * (-jdxc[i - 2] / hxxc[i - 2] * TXX(i - 1, j) + jdxc[i - 1] / hxxc[i - 1] * TXX(i, j));
+      s += a.diffusivity / jdxf[i - 1] * (-jdxf[i - 1] / hy * TXY(i, j) + jdxf[i - 1] / hy * TXY(i, j + 1));
+      double g = 0.;
+      g -= 1. / hxxf[i - 1] * (-a.p[icc(i - 1, j)] + a.p[icc(i, j)]);
+      s += surface_force(DEN0(i - 1, j), DEN0(i, j), CURV(i - 1, j), CURV(i, j), VOF(i - 1, j), VOF(i, j),
+                         hxxf[i - 1], a.refdeninv, a.tension);
+      s += a.grav * (0.5 * DEN0(i - 1, j) + 0.5 * DEN0(i, j));
+      if (a.srca_ux) a.srca_ux[L::fx(i, j, nx)] = s;
+      const double sb = a.stage != 0 ? a.srcb_ux[L::fx(i, j, nx)] : 0.;
+      const double d0 = 0.5 * DEN0(i - 1, j) + 0.5 * DEN0(i, j);
+      u = d0 * UX(i, j) + a.ca * dt * s + a.cb * dt * sb + a.cg * dt * g;
+      const double d1 = 0.5 * DEN1(i - 1, j) + 0.5 * DEN1(i, j);
+      u /= d1;
+    }
+    store_with_halo(a.uxs, i, j, ny, 1, u, ixf);
+    if (i == nx) store_with_halo(a.uxs, nx + 1, j, ny, 1, 0., ixf);
+  }
+  // uy at face j (predict/uy.c)
+  {
+    const double* hxxf = a.hxxf;
+    const double* jdxf = a.jdxf;
+    const double* jdxc = a.jdxc;
+    double s = 0.;
+    {
+      const double mm = 0.5 * jdxf[i - 1] / hxxf[i - 1] * MFX(i, j - 1) + 0.5 * jdxf[i - 1] / hxxf[i - 1] * MFX(i, j);
+      const double mp = 0.5 * jdxf[i] / hxxf[i] * MFX(i + 1, j - 1) + 0.5 * jdxf[i] / hxxf[i] * MFX(i + 1, j);
+      const double um = 0.5 * UY(i - 1, j) + 0.5 * UY(i, j);
+      const double up = 0.5 * UY(i, j) + 0.5 * UY(i + 1, j);
+      s -= 1. / jdxc[i - 1] * (-mm * um + mp * up);
+    }
+    {
+      const double mm = 0.5 * jdxc[i - 1] / hy * MFY(i, j - 1) + 0.5 * jdxc[i - 1] / hy * MFY(i, j);
+      const double mp = 0.5 * jdxc[i - 1] / hy * MFY(i, j) + 0.5 * jdxc[i - 1] / hy * MFY(i, j + 1);
+      const double um = 0.5 * UY(i, j - 1) + 0.5 * UY(i, j);
+      const double up = 0.5 * UY(i, j) + 0.5 * UY(i, j + 1);
+      s -= 1. / jdxc[i - 1] * (-mm * um + mp * up);
+    }
+    s += a.diffusivity / jdxc[i - 1] * (-jdxf[i - 1] / hxxf[i - 1] * TXY(i, j) + jdxf[i] / hxxf[i] * TXY(i + 1, j));
+    s += a.diffusivity / jdxc[i - 1] * (-jdxc[i - 1] / hy * TYY(i, j - 1) + jdxc[i - 1] / hy * TYY(i, j));
+    double g = 0.;
+    g -= 1. / hy * (-a.p[icc(i, j - 1)] + a.p[icc(i, j)]);
+    s += surface_force(DEN0(i, j - 1), DEN0(i, j), CURV(i, j - 1), CURV(i, j), VOF(i, j - 1), VOF(i, j), hy,
+                       a.refdeninv, a.tension);
+    if (a.srca_uy) a.srca_uy[L::ci(i, j, nx)] = s;
+    const double sb = a.stage != 0 ? a.srcb_uy[L::ci(i, j, nx)] : 0.;
+    const double d0 = 0.5 * DEN0(i, j - 1) + 0.5 * DEN0(i, j);
+    double u = d0 * UY(i, j) + a.ca * dt * s + a.cb * dt * sb + a.cg * dt * g;
+    const double d1 = 0.5 * DEN1(i, j - 1) + 0.5 * DEN1(i, j);
+    u /= d1;
+    store_with_halo(a.uys, i, j, ny, 1, u, icc);
+    if (i == 1) store_with_halo(a.uys, 0, j, ny, 1, kUyXm, icc);
+    if (i == nx) store_with_halo(a.uys, nx + 1, j, ny, 1, kUyXp, icc);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// projection + pressure (correct_velocity/ux.c, uy.c; update_pressure.c)
+// ---------------------------------------------------------------------------
+struct PostArgs {
+  const double* uxs;
+  const double* uys;
+  const double* psi1;
+  const double* psi0;
+  const double* vofn;
+  double* ux;
+  double* uy;
+  double* p;
+  const double* hxxf;
+  double hy;
+  double denr, dlo, dhi, refden;
+  const double* dt;
+  double g_new, g_old;
+  int nx, ny;
+};
+
+__global__ __launch_bounds__(256) void k_post(PostArgs a) {
+  const int nx = a.nx, ny = a.ny;
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);  // 0..nx+1
+  const int j = 1 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (i > nx + 1 || j > ny) return;
+  const double dt = *a.dt;
+  const double dtn = a.g_new * dt;
+  const double dto = a.g_old * dt;
+  const double coef = -1. / dtn * dto;
+  auto icc = [nx](int p, int q) { return L::cc(p, q, nx); };
+  auto ixf = [nx](int p, int q) { return L::xf(p, q, nx); };
+  auto DEN1 = [&](int p, int q) { return mix_clamp(a.denr, a.dlo, a.dhi, a.vofn[L::vof(p, q, nx)]); };
+  if (i >= 1) {
+    double u = 0.;
+    if (i != 1 && i != nx + 1) {
+      const double h = a.hxxf[i - 1];
+      u = a.uxs[ixf(i, j)];
+      u -= dtn / a.refden / h * (-a.psi1[icc(i - 1, j)] + a.psi1[icc(i, j)]);
+      const double d = 0.5 * DEN1(i - 1, j) + 0.5 * DEN1(i, j);
+      u += dtn * coef * (1. / d - 1. / a.refden) / h * (-a.psi0[icc(i - 1, j)] + a.psi0[icc(i, j)]);
+    }
+    store_with_halo(a.ux, i, j, ny, 1, u, ixf);
+  }
+  {
+    double u;
+    if (i == 0) u = kUyXm;
+    else if (i == nx + 1) u = kUyXp;
+    else {
+      u = a.uys[icc(i, j)];
+      u -= dtn / a.refden / a.hy * (-a.psi1[icc(i, j - 1)] + a.psi1[icc(i, j)]);
+      const double d = 0.5 * DEN1(i, j - 1) + 0.5 * DEN1(i, j);
+      u += dtn * coef * (1. / d - 1. / a.refden) / a.hy * (-a.psi0[icc(i, j - 1)] + a.psi0[icc(i, j)]);
+    }
+    store_with_halo(a.uy, i, j, ny, 1, u, icc);
+  }
+  if (i >= 1 && i <= nx) {
+    const double v = a.p[icc(i, j)] + a.psi1[icc(i, j)];
+    store_with_halo(a.p, i, j, ny, 1, v, icc);
+    if (i == 1) store_with_halo(a.p, 0, j, ny, 1, v, icc);
+    if (i == nx) store_with_halo(a.p, nx + 1, j, ny, 1, v, icc);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+constexpr int kTX = 64, kTY = 4;
+
+struct FusedScratch {
+  size_t nx = 0, ny = 0;
+  double* vof = nullptr;
+  double* ux = nullptr;
+  double* uy = nullptr;
+};
+static FusedScratch g_scr;
+
+static int scratch(const domain_t* d, const interface_t* s, const fluid_t* f) {
+  if (g_scr.nx == d->mysizes[0] && g_scr.ny == d->mysizes[1]) return 0;
+  (void)hipStreamSynchronize(stream());
+  (void)hipFree(g_scr.vof);
+  (void)hipFree(g_scr.ux);
+  (void)hipFree(g_scr.uy);
+  g_scr = FusedScratch();
+  SBFS_HIP(hipMalloc(&g_scr.vof, s->vof.datasize));
+  SBFS_HIP(hipMalloc(&g_scr.ux, f->ux.datasize));
+  SBFS_HIP(hipMalloc(&g_scr.uy, f->uy.datasize));
+  SBFS_HIP(hipMemset(g_scr.vof, 0, s->vof.datasize));
+  SBFS_HIP(hipMemset(g_scr.ux, 0, f->ux.datasize));
+  SBFS_HIP(hipMemset(g_scr.uy, 0, f->uy.datasize));
+  g_scr.nx = d->mysizes[0];
+  g_scr.ny = d->mysizes[1];
+  return 0;
+}
+
+int poisson_solve_fused(const domain_t* d, DomainCtx* c, const double* dt_dev, double g_old,
+                        double g_new, const double* uxs, const double* uys, const double* vofn,
+                        fluid_t* f);  // poisson.hip
+
+#define D(a) ((double*)(a).data)
+
+static void swap_data(array_t& x, array_t& y) {
+  void* t = x.data;
+  x.data = y.data;
+  y.data = t;
+}
+
+int integrate_fused(const domain_t* domain, fluid_t* fluid, interface_t* interface, double* dt) {
+  DomainCtx* c = domain_ctx(domain);
+  if (!c) return 1;
+  const Nadds* nas[] = {&NA_UX, &NA_UY, &NA_P, &NA_P, &NA_P, &NA_DEN, &NA_DEN, &NA_TXY};
+  const array_t* arrs[] = {&fluid->ux, &fluid->uy, &fluid->p, &fluid->psi[0], &fluid->psi[1],
+                           &fluid->den[0], &fluid->den[1], &fluid->txy};
+  for (int k = 0; k < 8; k++)
+    if (check_array(domain, arrs[k], *nas[k], 8, "fluid array")) return 1;
+  for (int n = 0; n < 3; n++) {
+    if (check_array(domain, &fluid->srcux[n], NA_SRCUX, 8, "srcux")) return 1;
+    if (check_array(domain, &fluid->srcuy[n], NA_SRCUY, 8, "srcuy")) return 1;
+  }
+  if (check_array(domain, &interface->vof, NA_VOF, 8, "vof")) return 1;
+  for (int n = 0; n < 2; n++)
+    if (check_array(domain, &interface->src[n], NA_SRC, 8, "src")) return 1;
+  if (scratch(domain, interface, fluid)) return 1;
+  const int nx = (int)c->nx, ny = (int)c->ny;
+  if (compute_dt_device(domain, c, fluid, interface)) return 1;
+  const double* dtp = c->dt_dev;
+  for (int k = 0; k < 3; k++) {
+    if (k != 0) {
+      swap_data(interface->src[0], interface->src[1]);
+      swap_data(fluid->srcux[0], fluid->srcux[1]);
+      swap_data(fluid->srcuy[0], fluid->srcuy[1]);
+    }
+    const RK r = rk(k);
+    PreArgs a;
+    a.vof = D(interface->vof);
+    a.ux = D(fluid->ux);
+    a.uy = D(fluid->uy);
+    a.p = D(fluid->p);
+    a.srcb_vof = D(interface->src[1]);
+    a.srcb_ux = D(fluid->srcux[1]);
+    a.srcb_uy = D(fluid->srcuy[1]);
+    a.vofn = g_scr.vof;
+    a.uxs = g_scr.ux;
+    a.uys = g_scr.uy;
+    // the alpha sources of the last stage are never read again (beta of
+    // stage 0 is zero), so they are not written
+    a.srca_vof = k < 2 ? D(interface->src[0]) : nullptr;
+    a.srca_ux = k < 2 ? D(fluid->srcux[0]) : nullptr;
+    a.srca_uy = k < 2 ? D(fluid->srcuy[0]) : nullptr;
+    a.den1 = k == 2 ? D(fluid->den[1]) : nullptr;
+    a.txy = k == 2 ? D(fluid->txy) : nullptr;
+    a.hxxf = c->hxxf;
+    a.hxxc = c->hxxc;
+    a.jdxf = c->jdxf;
+    a.jdxc = c->jdxc;
+    a.hy = c->hy;
+    a.denr = fluid->denr;
+    a.dlo = fmin(1., fluid->denr);
+    a.dhi = fmax(1., fluid->denr);
+    a.visr = fluid->visr;
+    a.vlo = fmin(1., fluid->visr);
+    a.vhi = fmax(1., fluid->visr);
+    a.refdeninv = 2. / (1. + fluid->denr);
+    a.tension = 1. / interface->We;
+    a.diffusivity = 1. / fluid->Re;
+    a.grav = -1. / fluid->Fr / fluid->Fr;
+    a.dt = dtp;
+    a.ca = r.a;
+    a.cb = r.b;
+    a.cg = r.g;
+    a.stage = k;
+    a.nx = nx;
+    a.ny = ny;
+    const dim3 grid(cdiv(nx, kTX), cdiv(ny, kTY)), block(kTX, kTY);
+    SBFS_TIMED("k_pre", hipLaunchKernelGGL((k_pre<kTX, kTY>), grid, block, 0, stream(), a));
+    if (check_launch("integrate: k_pre")) return 1;
+    // the new vof becomes the state; the old one is the next scratch
+    {
+      void* t = interface->vof.data;
+      interface->vof.data = g_scr.vof;
+      g_scr.vof = (double*)t;
+    }
+    const double g_old = rk((k + 2) % 3).g, g_new = r.g;
+    if (poisson_solve_fused(domain, c, dtp, g_old, g_new, g_scr.ux, g_scr.uy, D(interface->vof), fluid))
+      return 1;
+    PostArgs q;
+    q.uxs = g_scr.ux;
+    q.uys = g_scr.uy;
+    q.psi1 = D(fluid->psi[1]);
+    q.psi0 = D(fluid->psi[0]);
+    q.vofn = D(interface->vof);
+    q.ux = D(fluid->ux);
+    q.uy = D(fluid->uy);
+    q.p = D(fluid->p);
+    q.hxxf = c->hxxf;
+    q.hy = c->hy;
+    q.denr = fluid->denr;
+    q.dlo = fmin(1., fluid->denr);
+    q.dhi = fmax(1., fluid->denr);
+    q.refden = fluid->refden;
+    q.dt = dtp;
+    q.g_new = g_new;
+    q.g_old = g_old;
+    q.nx = nx;
+    q.ny = ny;
+    const dim3 pg(cdiv(nx + 2, 64), cdiv(ny, 4)), pb(64, 4);
+    SBFS_TIMED("k_post", hipLaunchKernelGGL(k_post, pg, pb, 0, stream(), q));
+    if (check_launch("integrate: k_post")) return 1;
+    swap_data(fluid->psi[0], fluid->psi[1]);
+  }
+  SBFS_HIP(hipMemcpyAsync(c->host_pin, c->dt_dev, sizeof(double), hipMemcpyDeviceToHost, stream()));
+  SBFS_HIP(hipStreamSynchronize(stream()));
+  *dt = c->host_pin[0];
   return 0;
 }
 

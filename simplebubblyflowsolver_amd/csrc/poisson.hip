@@ -42,8 +42,7 @@ void poisson_destroy(PoissonPlan* p) {
   (void)hipFree(p->pred);
   (void)hipFree(p->gfl);
   (void)hipFree(p->X);
-  (void)hipFree(p->rv);
-  (void)hipFree(p->rq1);
+  (void)hipFree(p->rtab);
   delete p;
 }
 
@@ -117,20 +116,23 @@ __device__ double2* fft_lds(double2* x, double2* y, int m, const double2* __rest
   return x;
 }
 
-// right-hand side of one cell (compute_potential.c:315-360)
+// right-hand side of one cell (compute_potential.c:315-360); `den(i, j)`
+// supplies den[1] (read from the array, or recomputed from the new vof)
+template <typename Den>
 __device__ __forceinline__ double poisson_rhs(
     int i, int j, int nx, const double* __restrict__ ux, const double* __restrict__ uy,
-    const double* __restrict__ psi, const double* __restrict__ den,
-    const double* __restrict__ hxxf, const double* __restrict__ jdxf,
-    const double* __restrict__ jdxc, double hy, double refden, double c0, double c1) {
+    const double* __restrict__ psi, Den den, const double* __restrict__ hxxf,
+    const double* __restrict__ jdxf, const double* __restrict__ jdxc, double hy, double refden,
+    double c0, double c1) {
   const double hm = hxxf[i - 1], hp = hxxf[i];
   const double jm = jdxf[i - 1], j0 = jdxc[i - 1], jp = jdxf[i];
   const double div = 1. / j0 * (-jm / hm * ux[L::xf(i, j, nx)] + jp / hp * ux[L::xf(i + 1, j, nx)]
                                 - j0 / hy * uy[L::cc(i, j, nx)] + j0 / hy * uy[L::cc(i, j + 1, nx)]);
-  const double dxm = 0.5 * den[L::cc(i - 1, j, nx)] + 0.5 * den[L::cc(i, j, nx)];
-  const double dxp = 0.5 * den[L::cc(i, j, nx)] + 0.5 * den[L::cc(i + 1, j, nx)];
-  const double dym = 0.5 * den[L::cc(i, j - 1, nx)] + 0.5 * den[L::cc(i, j, nx)];
-  const double dyp = 0.5 * den[L::cc(i, j, nx)] + 0.5 * den[L::cc(i, j + 1, nx)];
+  const double d00 = den(i, j);
+  const double dxm = 0.5 * den(i - 1, j) + 0.5 * d00;
+  const double dxp = 0.5 * d00 + 0.5 * den(i + 1, j);
+  const double dym = 0.5 * den(i, j - 1) + 0.5 * d00;
+  const double dyp = 0.5 * d00 + 0.5 * den(i, j + 1);
   const double pxm = -psi[L::cc(i - 1, j, nx)] + psi[L::cc(i, j, nx)];
   const double pxp = -psi[L::cc(i, j, nx)] + psi[L::cc(i + 1, j, nx)];
   const double pym = -psi[L::cc(i, j - 1, nx)] + psi[L::cc(i, j, nx)];
@@ -186,8 +188,36 @@ __global__ void k_dct2_rhs(const double* __restrict__ ux, const double* __restri
   double2* B = lds + m;
   const int j = blockIdx.x + 1;
   double* row = (double*)B;
+  auto dd = [den, nx](int p, int q) { return den[L::cc(p, q, nx)]; };
   for (int i = threadIdx.x + 1; i <= nx; i += blockDim.x)
-    row[i - 1] = poisson_rhs(i, j, nx, ux, uy, psi, den, hxxf, jdxf, jdxc, hy, refden, c0, c1);
+    row[i - 1] = poisson_rhs(i, j, nx, ux, uy, psi, dd, hxxf, jdxf, jdxc, hy, refden, c0, c1);
+  __syncthreads();
+  dct2_row_lds(A, B, nx, m, tw, t1, t2, out + (size_t)(j - 1) * nx);
+}
+
+// fused-path variant: predicted velocity from scratch buffers, den[1] from
+// the new vof, coefficients from the device-resident dt
+__global__ void k_dct2_rhs_fused(const double* __restrict__ ux, const double* __restrict__ uy,
+                                 const double* __restrict__ psi, const double* __restrict__ vofn,
+                                 const double* __restrict__ hxxf, const double* __restrict__ jdxf,
+                                 const double* __restrict__ jdxc, double hy, double refden,
+                                 double denr, double dlo, double dhi, double norm,
+                                 const double* __restrict__ dtp, double g_old, double g_new,
+                                 int nx, int m, const double2* __restrict__ tw,
+                                 const double2* __restrict__ t1, const double2* __restrict__ t2,
+                                 double* __restrict__ out) {
+  extern __shared__ double2 lds[];
+  double2* A = lds;
+  double2* B = lds + m;
+  const int j = blockIdx.x + 1;
+  const double dt = *dtp;
+  const double dt_old = g_old * dt, dt_new = g_new * dt;
+  const double c0 = 1. / norm / dt_new * refden;
+  const double c1 = 1. / norm * -1. / dt_new * dt_old;
+  double* row = (double*)B;
+  auto dd = [=](int p, int q) { return mix_clamp(denr, dlo, dhi, vofn[L::vof(p, q, nx)]); };
+  for (int i = threadIdx.x + 1; i <= nx; i += blockDim.x)
+    row[i - 1] = poisson_rhs(i, j, nx, ux, uy, psi, dd, hxxf, jdxf, jdxc, hy, refden, c0, c1);
   __syncthreads();
   dct2_row_lds(A, B, nx, m, tw, t1, t2, out + (size_t)(j - 1) * nx);
 }
@@ -383,6 +413,8 @@ static PoissonPlan* plan_get(const domain_t* d, DomainCtx* c) {
 
 static int row_threads(int m) { return m >= 1024 ? 512 : 256; }
 
+static int poisson_tail(const domain_t* d, PoissonPlan* p, fluid_t* f);
+
 int poisson_solve(const domain_t* d, DomainCtx* c, double dt_old, double dt_new, fluid_t* f) {
   PoissonPlan* p = plan_get(d, c);
   if (!p) return 1;
@@ -393,11 +425,20 @@ int poisson_solve(const domain_t* d, DomainCtx* c, double dt_old, double dt_new,
   const double c0 = 1. / norm / dt_new * refden;
   const double c1 = 1. / norm * -1. / dt_new * dt_old;
   const size_t lds = sizeof(double2) * 2 * (size_t)(m > 0 ? m : 1);
-  hipLaunchKernelGGL(k_dct2_rhs, dim3(ny), dim3(row_threads(m)), lds, stream(),
-                     (const double*)f->ux.data, (const double*)f->uy.data,
-                     (const double*)f->psi[0].data, (const double*)f->den[1].data, c->hxxf,
-                     c->jdxf, c->jdxc, c->hy, refden, c0, c1, nx, m, p->tw, p->t1, p->t2, p->buf);
+  SBFS_TIMED("k_dct2_rhs",
+             hipLaunchKernelGGL(k_dct2_rhs, dim3(ny), dim3(row_threads(m)), lds, stream(),
+                                (const double*)f->ux.data, (const double*)f->uy.data,
+                                (const double*)f->psi[0].data, (const double*)f->den[1].data,
+                                c->hxxf, c->jdxf, c->jdxc, c->hy, refden, c0, c1, nx, m, p->tw,
+                                p->t1, p->t2, p->buf));
   if (check_launch("poisson: rhs + DCT-II")) return 1;
+  return poisson_tail(d, p, f);
+}
+
+// tridiagonal solves + DCT-III into psi[1]
+static int poisson_tail(const domain_t* d, PoissonPlan* p, fluid_t* f) {
+  const int nx = p->nx, ny = p->ny, m = p->m;
+  const size_t lds = sizeof(double2) * 2 * (size_t)(m > 0 ? m : 1);
   // tridiagonal matrix l = u = 1/hy^2 (compute_potential.c:173-176)
   const double l = 1. / d->hy / d->hy;
   const int pin = options().gauge == SBFS_GAUGE_PINNED ? 1 : 0;
@@ -406,14 +447,34 @@ int poisson_solve(const domain_t* d, DomainCtx* c, double dt_old, double dt_new,
     if (p->L != L && partition_setup(p, d, L)) return 1;
     if (partition_solve(p, pin)) return 1;
   } else {
-    hipLaunchKernelGGL(k_tdm_sequential, dim3(cdiv(nx, 64)), dim3(64), 0, stream(), p->buf, p->v,
-                       p->q1, nx, ny, p->evals, l, l, pin);
+    SBFS_TIMED("k_tdm_sequential",
+               hipLaunchKernelGGL(k_tdm_sequential, dim3(cdiv(nx, 64)), dim3(64), 0, stream(),
+                                  p->buf, p->v, p->q1, nx, ny, p->evals, l, l, pin));
     if (check_launch("poisson: tridiagonal")) return 1;
   }
-  hipLaunchKernelGGL(k_dct3_psi, dim3(ny), dim3(row_threads(m)), lds, stream(),
-                     (const double*)p->buf, (double*)f->psi[1].data, nx, ny, m, p->tw, p->t1,
-                     p->t2);
+  SBFS_TIMED("k_dct3_psi",
+             hipLaunchKernelGGL(k_dct3_psi, dim3(ny), dim3(row_threads(m)), lds, stream(),
+                                (const double*)p->buf, (double*)f->psi[1].data, nx, ny, m, p->tw,
+                                p->t1, p->t2));
   return check_launch("poisson: DCT-III + psi");
+}
+
+int poisson_solve_fused(const domain_t* d, DomainCtx* c, const double* dt_dev, double g_old,
+                        double g_new, const double* uxs, const double* uys, const double* vofn,
+                        fluid_t* f) {
+  PoissonPlan* p = plan_get(d, c);
+  if (!p) return 1;
+  const int nx = p->nx, ny = p->ny, m = p->m;
+  const double norm = 2. * (double)d->glsizes[0];
+  const size_t lds = sizeof(double2) * 2 * (size_t)(m > 0 ? m : 1);
+  SBFS_TIMED("k_dct2_rhs",
+             hipLaunchKernelGGL(k_dct2_rhs_fused, dim3(ny), dim3(row_threads(m)), lds, stream(),
+                                uxs, uys, (const double*)f->psi[0].data, vofn, c->hxxf, c->jdxf,
+                                c->jdxc, c->hy, f->refden, f->denr, fmin(1., f->denr),
+                                fmax(1., f->denr), norm, dt_dev, g_old, g_new, nx, m, p->tw,
+                                p->t1, p->t2, p->buf));
+  if (check_launch("poisson: rhs + DCT-II")) return 1;
+  return poisson_tail(d, p, f);
 }
 
 }  // namespace sbfs

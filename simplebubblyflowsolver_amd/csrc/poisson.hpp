@@ -21,11 +21,11 @@ struct PoissonPlan {
   int L = 0, P = 0;
   double a = 0.;             // off-diagonal 1/hy^2
   double* ptab = nullptr;    // [4][K][nx]: inv, v, w, z per (row-in-chunk, mode)
-  double* pred = nullptr;    // [3][nx]: reduced lower / diagonal / upper
+  double* pred = nullptr;    // [6][nx]: reduced lower / diagonal / upper,
+                             //   1/(last pivot), last-pivot flag, closure denominator
+  double* rtab = nullptr;    // [3][P][nx]: reduced-solver pivots, factors, spike
   double* gfl = nullptr;     // [2P][nx]: first / last interior of each chunk
   double* X = nullptr;       // [P][nx]: separators
-  double* rv = nullptr;      // [P][nx]: reduced-solver factors
-  double* rq1 = nullptr;     // [P][nx]: reduced-solver spike
 };
 
 int partition_chunk(int n, int requested);

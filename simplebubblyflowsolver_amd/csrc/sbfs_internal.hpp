@@ -142,6 +142,16 @@ int init_runtime();
 DomainCtx* domain_ctx(const domain_t* d);
 int load_dt_coefs();
 
+// kernel timing (profiler.hip): events around launches while profiling
+void prof_begin(const char* name);
+void prof_end();
+#define SBFS_TIMED(name, ...) \
+  do {                        \
+    ::sbfs::prof_begin(name); \
+    __VA_ARGS__;              \
+    ::sbfs::prof_end();       \
+  } while (0)
+
 // grid-dimension helpers
 inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
 

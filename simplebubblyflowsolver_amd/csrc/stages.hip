@@ -21,11 +21,12 @@ namespace sbfs {
   if (i > i1 || j > j1) return
 
 template <typename K, typename... A>
-static void launch2(K kernel, int i0, int i1, int j0, int j1, A... args) {
+static void launch2n(const char* name, K kernel, int i0, int i1, int j0, int j1, A... args) {
   const dim3 block(64, 4);
   const dim3 grid(cdiv(i1 - i0 + 1, 64), cdiv(j1 - j0 + 1, 4));
-  hipLaunchKernelGGL(kernel, grid, block, 0, stream(), i0, i1, j0, j1, args...);
+  SBFS_TIMED(name, hipLaunchKernelGGL(kernel, grid, block, 0, stream(), i0, i1, j0, j1, args...));
 }
+#define launch2(k, ...) launch2n(#k, k, __VA_ARGS__)
 
 // index helpers bound to an interior width
 #define IXF(i, j) L::xf(i, j, nx)

@@ -111,11 +111,14 @@ int compute_dt_device(const domain_t* d, DomainCtx* c, const fluid_t* f,
   dt_limits(d, f, s, &dif0, &dif1, &dint);
   const long total = (long)nx * ny;
   const int nblk = (int)std::min<long>(kRedBlocks, (total + kRedThreads - 1) / kRedThreads);
-  hipLaunchKernelGGL(k_dt_partial, dim3(nblk), dim3(kRedThreads), 0, stream(),
-                     (const double*)f->ux.data, (const double*)f->uy.data, c->hxxf, c->hy, nx,
-                     ny, c->red);
-  hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(kRedThreads), 0, stream(), (const double*)c->red,
-                     nblk, options().cadv, dif0, dif1, dint, c->dt_dev);
+  SBFS_TIMED("k_dt_partial",
+             hipLaunchKernelGGL(k_dt_partial, dim3(nblk), dim3(kRedThreads), 0, stream(),
+                                (const double*)f->ux.data, (const double*)f->uy.data, c->hxxf,
+                                c->hy, nx, ny, c->red));
+  SBFS_TIMED("k_dt_final",
+             hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(kRedThreads), 0, stream(),
+                                (const double*)c->red, nblk, options().cadv, dif0, dif1, dint,
+                                c->dt_dev));
   return check_launch("decide_dt");
 }
 

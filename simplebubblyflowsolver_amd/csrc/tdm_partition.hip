@@ -96,11 +96,16 @@ __global__ void k_part_reduce(const double* __restrict__ q, const double* __rest
   gfl[(size_t)(2 * ch + 1) * nx + mo] = g[K - 1];
 }
 
+// Separators of every mode: assemble the reduced right-hand side and solve
+// the periodic reduced system with the reference algorithm.  Its pivots,
+// factors and spike depend only on the mode, so they come from tables
+// (rtab, pred) built with the same arithmetic; the only loop-carried
+// dependence left is the running value in a register.
 template <int L>
 __global__ void k_part_separators(const double* __restrict__ q, const double* __restrict__ gfl,
-                                  const double* __restrict__ pred, double* __restrict__ X,
-                                  double* __restrict__ rv, double* __restrict__ rq1, int nx,
-                                  int P, double a, int pin_mode0) {
+                                  const double* __restrict__ pred,
+                                  const double* __restrict__ rtab, double* __restrict__ X,
+                                  int nx, int P, double a, int pin_mode0) {
   constexpr int K = L - 1;
   const int mo = blockIdx.x * blockDim.x + threadIdx.x;
   if (mo >= nx) return;
@@ -109,8 +114,33 @@ __global__ void k_part_separators(const double* __restrict__ q, const double* __
     const double gf = gfl[(size_t)(2 * ((c + 1) % P)) * nx + mo];
     X[(size_t)c * nx + mo] = (q[(size_t)(c * L + K) * nx + mo] - a * gl) - a * gf;
   }
-  const double lo = pred[mo], di = pred[nx + mo], up = pred[2 * nx + mo];
-  periodic_tdm(X + mo, rv + mo, rq1 + mo, (size_t)nx, P, lo, di, up, pin_mode0 && mo == 0);
+  const double lo = pred[mo], di = pred[(size_t)nx + mo], up = pred[(size_t)2 * nx + mo];
+  const double ilast = pred[(size_t)3 * nx + mo], ok = pred[(size_t)4 * nx + mo];
+  const double den = pred[(size_t)5 * nx + mo];
+  const double* val = rtab;
+  const double* vv = rtab + (size_t)P * nx;
+  const double* q1 = rtab + (size_t)2 * P * nx;
+  const size_t st = nx;
+  double* x = X + mo;
+  const int N = P - 1;
+  // gtsv on the first N rows (src/tdm.c:26-48)
+  double prev = x[0] / di;
+  x[0] = prev;
+  for (int r = 1; r < N - 1; r++) {
+    prev = val[r * st + mo] * (x[r * st] - lo * prev);
+    x[r * st] = prev;
+  }
+  double nxt = ok != 0. ? ilast * (x[(N - 1) * st] - lo * prev) : 0.;
+  x[(N - 1) * st] = nxt;
+  for (int r = N - 2; r >= 0; r--) {
+    nxt = x[r * st] - vv[r * st + mo] * nxt;
+    x[r * st] = nxt;
+  }
+  // closure (src/tdm.c:92-98); the singular mode pins the last separator
+  const double num = x[(P - 1) * st] - up * x[0] - lo * x[(P - 2) * st];
+  const double xl = ((pin_mode0 && mo == 0) || fabs(den) < DBL_EPSILON) ? 0. : num / den;
+  x[(P - 1) * st] = xl;
+  for (int r = 0; r < P - 1; r++) x[r * st] = x[r * st] + xl * q1[r * st + mo];
 }
 
 template <int L>
@@ -151,14 +181,14 @@ int partition_setup(PoissonPlan* p, const domain_t* d, int L) {
   (void)hipFree(p->pred);
   (void)hipFree(p->gfl);
   (void)hipFree(p->X);
-  (void)hipFree(p->rv);
-  (void)hipFree(p->rq1);
-  p->ptab = p->pred = p->gfl = p->X = p->rv = p->rq1 = nullptr;
+  (void)hipFree(p->rtab);
+  p->ptab = p->pred = p->gfl = p->X = p->rtab = nullptr;
   std::vector<double> ev(nx);
   SBFS_HIP(hipMemcpy(ev.data(), p->evals, sizeof(double) * nx, hipMemcpyDeviceToHost));
   const double l = 1. / d->hy / d->hy;
   const double u = l;
-  std::vector<double> tab((size_t)4 * K * nx), red((size_t)3 * nx);
+  std::vector<double> tab((size_t)4 * K * nx), red((size_t)6 * nx), rtab((size_t)3 * P * nx, 0.);
+  std::vector<double> rv(P), rq(P), rval(P);
   std::vector<double> inv(K), vv(K), w(K), z(K), f(K);
   for (int mo = 0; mo < nx; mo++) {
     const double a = l;
@@ -181,16 +211,52 @@ int partition_setup(PoissonPlan* p, const domain_t* d, int L) {
       tab[(size_t)(2 * K + k) * nx + mo] = w[k];
       tab[(size_t)(3 * K + k) * nx + mo] = z[k];
     }
-    red[mo] = a * w[K - 1];
-    red[nx + mo] = (b + a * z[K - 1]) + a * w[0];
-    red[2 * nx + mo] = a * z[0];
+    const double lo = a * w[K - 1];
+    const double di = (b + a * z[K - 1]) + a * w[0];
+    const double up = a * z[0];
+    // data-independent part of the reduced periodic solve (src/tdm.c:78-94
+    // with l = lo, c = di, u = up): pivots, factors, spike q1, denominator
+    const int N = P - 1;
+    std::fill(rv.begin(), rv.end(), 0.);
+    std::fill(rval.begin(), rval.end(), 0.);
+    for (int r = 0; r < N; r++) rq[r] = (r == 0) ? -lo : (r == P - 2) ? -up : 0.;
+    rq[N] = 0.;
+    rv[0] = up / di;
+    rq[0] = rq[0] / di;
+    for (int r = 1; r < N - 1; r++) {
+      rval[r] = 1. / (di - lo * rv[r - 1]);
+      rv[r] = rval[r] * up;
+      rq[r] = rval[r] * (rq[r] - lo * rq[r - 1]);
+    }
+    const double lval = di - lo * rv[N - 2];
+    double ilast = 0., ok = 0.;
+    if (fabs(lval) > DBL_EPSILON) {
+      ilast = 1. / lval;
+      ok = 1.;
+      rq[N - 1] = ilast * (rq[N - 1] - lo * rq[N - 2]);
+    } else {
+      rq[N - 1] = 0.;
+    }
+    for (int r = N - 2; r >= 0; r--) rq[r] -= rv[r] * rq[r + 1];
+    const double den = di + up * rq[0] + lo * rq[P - 2];
+    for (int r = 0; r < P; r++) {
+      rtab[(size_t)(0 * P + r) * nx + mo] = rval[r];
+      rtab[(size_t)(1 * P + r) * nx + mo] = rv[r];
+      rtab[(size_t)(2 * P + r) * nx + mo] = rq[r];
+    }
+    red[mo] = lo;
+    red[nx + mo] = di;
+    red[2 * nx + mo] = up;
+    red[3 * nx + mo] = ilast;
+    red[4 * nx + mo] = ok;
+    red[5 * nx + mo] = den;
   }
   SBFS_HIP(hipMalloc(&p->ptab, sizeof(double) * tab.size()));
   SBFS_HIP(hipMalloc(&p->pred, sizeof(double) * red.size()));
   SBFS_HIP(hipMalloc(&p->gfl, sizeof(double) * 2 * (size_t)P * nx));
   SBFS_HIP(hipMalloc(&p->X, sizeof(double) * (size_t)P * nx));
-  SBFS_HIP(hipMalloc(&p->rv, sizeof(double) * (size_t)P * nx));
-  SBFS_HIP(hipMalloc(&p->rq1, sizeof(double) * (size_t)P * nx));
+  SBFS_HIP(hipMalloc(&p->rtab, sizeof(double) * rtab.size()));
+  SBFS_HIP(hipMemcpy(p->rtab, rtab.data(), sizeof(double) * rtab.size(), hipMemcpyHostToDevice));
   SBFS_HIP(hipMemcpy(p->ptab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
   SBFS_HIP(hipMemcpy(p->pred, red.data(), sizeof(double) * red.size(), hipMemcpyHostToDevice));
   p->L = L;
@@ -203,10 +269,15 @@ template <int L>
 static void part_launch(PoissonPlan* p, double a, int pin) {
   const int nx = p->nx, P = p->P;
   const dim3 blk(64, 4), grd(cdiv(nx, 64), cdiv(P, 4));
-  hipLaunchKernelGGL(k_part_reduce<L>, grd, blk, 0, stream(), p->buf, p->ptab, p->gfl, nx, P, a);
-  hipLaunchKernelGGL(k_part_separators<L>, dim3(cdiv(nx, 64)), dim3(64), 0, stream(), p->buf,
-                     p->gfl, p->pred, p->X, p->rv, p->rq1, nx, P, a, pin);
-  hipLaunchKernelGGL(k_part_expand<L>, grd, blk, 0, stream(), p->buf, p->ptab, p->X, nx, P, a);
+  SBFS_TIMED("k_part_reduce",
+             hipLaunchKernelGGL(k_part_reduce<L>, grd, blk, 0, stream(), p->buf, p->ptab, p->gfl,
+                                nx, P, a));
+  SBFS_TIMED("k_part_separators",
+             hipLaunchKernelGGL(k_part_separators<L>, dim3(cdiv(nx, 64)), dim3(64), 0, stream(),
+                                p->buf, p->gfl, p->pred, p->rtab, p->X, nx, P, a, pin));
+  SBFS_TIMED("k_part_expand",
+             hipLaunchKernelGGL(k_part_expand<L>, grd, blk, 0, stream(), p->buf, p->ptab, p->X,
+                                nx, P, a));
 }
 
 int partition_solve(PoissonPlan* p, int pin) {

@@ -7,7 +7,8 @@ bitwise (same operation order, no FMA contraction, correctly rounded FP64
 division and sqrt on both sides).  The THINC intercept (log) and the THINC
 fluxes (tanh) use the GPU's and glibc's own libm: they may differ in the
 last bits, so those arrays, and the vof update built from the fluxes, are
-held to a relative 1e-13.
+held to 1e-13 of the array's largest magnitude (element-wise relative
+differences are larger where the flux divergence cancels).
 """
 import ctypes as C
 
@@ -49,7 +50,8 @@ def test_stages_isolated(oracle_mod, product, nx, ny, seed):
                 run_stage(g, st, k, dt)
                 for name, (dmax, rmax, nd) in compare(o, g).items():
                     if name in LIBM_ARRAYS.get(st, ()):
-                        if rmax > 1e-13:
+                        scale = np.abs(o.get(name)).max()
+                        if dmax > 1e-13 * scale:
                             failures.append((k, st, name, dmax, rmax, nd))
                     elif dmax != 0.0:
                         failures.append((k, st, name, dmax, rmax, nd))
