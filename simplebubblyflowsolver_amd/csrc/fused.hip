@@ -19,17 +19,31 @@
 // term at stage 0).  dt stays in device memory for the whole step.
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
+#include <vector>
 
 #include "decide_dt.h"
 #include "fluid_solver.h"
 #include "integrate.h"
 #include "interface_solver.h"
+#include "march.hpp"
 #include "poisson.hpp"
 #include "sbfs.h"
 #include "sbfs_internal.hpp"
 #include "stencils.hpp"
 
 namespace sbfs {
+
+// Per-column quotients of the metrics (struct Coef, march.hpp), each
+// evaluated on the host with the expression (and so the rounding) the
+// reference uses inside its loops:
+//   rhf = 1./HXXF  rhc = 1./HXXC  rjf = 1./JDXF  rjc = 1./JDXC
+//   qf  = JDXF/HXXF   qc = JDXC/HXXC   eqf = 0.5*JDXF/HXXF
+//   ecy = 0.5*JDXC/hy  cyc = JDXC/hy  cyf = JDXF/hy
+//   dfx = (1/Re)/JDXF  dcx = (1/Re)/JDXC
+// Indexed by the 1-based column i.  Multiplying by a stored quotient gives
+// the same bits as the reference's left-to-right `a / b * x`, so the fused
+// kernels keep bitwise parity while only data-dependent divisions remain.
 
 struct PreArgs {
   // inputs (state at the start of the stage)
@@ -55,6 +69,7 @@ struct PreArgs {
   const double* jdxf;
   const double* jdxc;
   double hy;
+  Coef cf;
   // physics
   double denr, dlo, dhi, visr, vlo, vhi;
   double refdeninv, tension, diffusivity, grav;
@@ -65,10 +80,9 @@ struct PreArgs {
   int nx, ny;
 };
 
+// periodic image of a row index within a few rows of [1, ny]
 __device__ __forceinline__ int wrap_row(int j, int ny) {
-  int r = (j - 1) % ny;
-  if (r < 0) r += ny;
-  return r + 1;
+  return j < 1 ? j + ny : (j > ny ? j - ny : j);
 }
 
 // tile of TX x TY output cells; regions (inclusive, relative to i0, j0):
@@ -169,7 +183,14 @@ __global__ __launch_bounds__(TX* TY) void k_pre(PreArgs a) {
     V2 g = {0., 0.};
     if (i >= 1 && i <= nx + 1) {
       const double mm = VOF(i - 1, j - 1), pm = VOF(i, j - 1), mp = VOF(i - 1, j), pp = VOF(i, j);
-      if (!(mm == pm && mm == mp && mm == pp)) g = corner_gradient(mm, pm, mp, pp, a.hxxf[i - 1], hy);
+      if (!(mm == pm && mm == mp && mm == pp)) {
+        // curvature_tensor.c:29-45
+        const double gx = a.cf.rhf[i] * (-mm + pm - mp + pp);
+        const double gy = a.cf.rhy * (-mm - pm + mp + pp);
+        const double norm = sqrt(gx * gx + gy * gy);
+        const double ninv = 1. / fmax(norm, DBL_EPSILON);
+        g = {gx * ninv, gy * ninv};
+      }
     }
     s_cx[y][x] = g.x;
     s_cy[y][x] = g.y;
@@ -192,9 +213,16 @@ __global__ __launch_bounds__(TX* TY) void k_pre(PreArgs a) {
     const int x = k % T::KX, y = k / T::KX;
     const int i = i0 - 1 + x, j = j0 - 1 + y;
     double c = 0.;
-    if (i >= 1 && i <= nx)
-      c = curvature(CRN(i, j), CRN(i + 1, j), CRN(i, j + 1), CRN(i + 1, j + 1), a.hxxf[i - 1], a.hxxf[i],
-                    a.jdxf[i - 1], a.jdxc[i - 1], a.jdxf[i], hy);
+    if (i >= 1 && i <= nx) {
+      // curvature_tensor.c:119-131
+      const V2 c00 = CRN(i, j), c10 = CRN(i + 1, j), c01 = CRN(i, j + 1), c11 = CRN(i + 1, j + 1);
+      const double nxm = 0.5 * c00.x + 0.5 * c01.x;
+      const double nxp = 0.5 * c10.x + 0.5 * c11.x;
+      const double nym = 0.5 * c00.y + 0.5 * c10.y;
+      const double nyp = 0.5 * c01.y + 0.5 * c11.y;
+      const double div = a.cf.rjc[i] * (-a.cf.qf[i] * nxm + a.cf.qf[i + 1] * nxp - a.cf.cyc[i] * nym + a.cf.cyc[i] * nyp);
+      c = -1. * div;
+    }
     s_curv[y][x] = c;
   }
   __syncthreads();
@@ -238,7 +266,7 @@ __global__ __launch_bounds__(TX* TY) void k_pre(PreArgs a) {
     double t = 0.;
     if (i >= 1 && i <= nx) {
       const double vx = 0.5 * VISUX(i, j) + 0.5 * VISUX(i + 1, j);
-      const double l = 1. / a.hxxc[i - 1] * (-UX(i, j) + UX(i + 1, j));
+      const double l = a.cf.rhc[i] * (-UX(i, j) + UX(i + 1, j));
       t = vx * l + vx * l;
     }
     s_txx[y][x] = t;
@@ -249,7 +277,7 @@ __global__ __launch_bounds__(TX* TY) void k_pre(PreArgs a) {
     double t = 0.;
     if (i <= nx) {
       const double vy = 0.5 * VISUY(i, j) + 0.5 * VISUY(i, j + 1);
-      const double l = 1. / hy * (-UY(i, j) + UY(i, j + 1));
+      const double l = a.cf.rhy * (-UY(i, j) + UY(i, j + 1));
       t = vy * l + vy * l;
     }
     s_tyy[y][x] = t;
@@ -261,8 +289,8 @@ __global__ __launch_bounds__(TX* TY) void k_pre(PreArgs a) {
     if (i <= nx + 1) {
       const double vx = 0.5 * VISUX(i, j - 1) + 0.5 * VISUX(i, j);
       const double vy = 0.5 * VISUY(i - 1, j) + 0.5 * VISUY(i, j);
-      const double lxy = 1. / hy * (-UX(i, j - 1) + UX(i, j));
-      const double lyx = 1. / a.hxxf[i - 1] * (-UY(i - 1, j) + UY(i, j));
+      const double lxy = a.cf.rhy * (-UX(i, j - 1) + UX(i, j));
+      const double lyx = a.cf.rhf[i] * (-UY(i - 1, j) + UY(i, j));
       t = vx * lxy + vy * lyx;
     }
     s_txy[y][x] = t;
@@ -276,9 +304,9 @@ __global__ __launch_bounds__(TX* TY) void k_pre(PreArgs a) {
     const int i = i0 - 1 + x, j = j0 - 1 + y;
     double v = 0., s = 0.;
     if (i >= 1 && i <= nx) {
-      const double hm = a.hxxf[i - 1], hp = a.hxxf[i];
-      const double jm = a.jdxf[i - 1], j0c = a.jdxc[i - 1], jp = a.jdxf[i];
-      s = 1. / j0c * (+jm / hm * FX(i, j) - jp / hp * FX(i + 1, j) + j0c / hy * FY(i, j) - j0c / hy * FY(i, j + 1));
+      // update/main.c:70-84
+      s = a.cf.rjc[i] * (+a.cf.qf[i] * FX(i, j) - a.cf.qf[i + 1] * FX(i + 1, j) + a.cf.cyc[i] * FY(i, j) -
+                         a.cf.cyc[i] * FY(i, j + 1));
       v = VOF(i, j);
       v += dtca * s;
       if (a.stage != 0) v += dtcb * a.srcb_vof[L::ci(i, wrap_row(j, ny), nx)];
@@ -302,43 +330,46 @@ __global__ __launch_bounds__(TX* TY) void k_pre(PreArgs a) {
     if (a.srca_vof) a.srca_vof[L::ci(i, j, nx)] = s_src[j - (j0 - 1)][i - (i0 - 1)];
     if (a.den1) {
       store_with_halo(a.den1, i, j, ny, 1, DEN1(i, j), icc);
-      if (i == 1) store_with_halo(a.den1, 0, j, ny, 1, DEN1(0, j), icc);
-      if (i == nx) store_with_halo(a.den1, nx + 1, j, ny, 1, DEN1(nx + 1, j), icc);
+      const double dg = mix_clamp(a.denr, a.dlo, a.dhi, 0.);  // vof = 0 in the wall ghosts
+      if (i == 1) store_with_halo(a.den1, 0, j, ny, 1, dg, icc);
+      if (i == nx) store_with_halo(a.den1, nx + 1, j, ny, 1, dg, icc);
     }
     if (a.txy) {
       store_with_halo(a.txy, i, j, ny, 1, TXY(i, j), ixf);
       if (i == nx) store_with_halo(a.txy, nx + 1, j, ny, 1, TXY(nx + 1, j), ixf);
     }
   }
-  // ux at face i (predict/ux.c)
+  // ux at face i (predict/ux.c:29-273)
   {
+    const Coef& cf = a.cf;
     double u = 0.;
     if (i >= 2) {
-      const double* hxxf = a.hxxf;
-      const double* hxxc = a.hxxc;
-      const double* jdxf = a.jdxf;
-      const double* jdxc = a.jdxc;
       double s = 0.;
       {
-        const double mm = 0.5 * jdxf[i - 2] / hxxf[i - 2] * MFX(i - 1, j) + 0.5 * jdxf[i - 1] / hxxf[i - 1] * MFX(i, j);
-        const double mp = 0.5 * jdxf[i - 1] / hxxf[i - 1] * MFX(i, j) + 0.5 * jdxf[i] / hxxf[i] * MFX(i + 1, j);
+        const double mm = cf.eqf[i - 1] * MFX(i - 1, j) + cf.eqf[i] * MFX(i, j);
+        const double mp = cf.eqf[i] * MFX(i, j) + cf.eqf[i + 1] * MFX(i + 1, j);
         const double um = 0.5 * UX(i - 1, j) + 0.5 * UX(i, j);
         const double up = 0.5 * UX(i, j) + 0.5 * UX(i + 1, j);
-        s -= 1. / jdxf[i - 1] * (-mm * um + mp * up);
+        s -= cf.rjf[i] * (-mm * um + mp * up);
       }
       {
-        const double mm = 0.5 * jdxc[i - 2] / hy * MFY(i - 1, j) + 0.5 * jdxc[i - 1] / hy * MFY(i, j);
-        const double mp = 0.5 * jdxc[i - 2] / hy * MFY(i - 1, j + 1) + 0.5 * jdxc[i - 1] / hy * MFY(i, j + 1);
+        const double mm = cf.ecy[i - 1] * MFY(i - 1, j) + cf.ecy[i] * MFY(i, j);
+        const double mp = cf.ecy[i - 1] * MFY(i - 1, j + 1) + cf.ecy[i] * MFY(i, j + 1);
         const double um = 0.5 * UX(i, j - 1) + 0.5 * UX(i, j);
         const double up = 0.5 * UX(i, j) + 0.5 * UX(i, j + 1);
-        s -= 1. / jdxf[i - 1] * (-mm * um + mp * up);
+        s -= cf.rjf[i] * (-mm * um + mp * up);
       }
-      s += a.diffusivity / jdxf[i - 1] * (-jdxc[i - 2] / hxxc[i - 2] * TXX(i - 1, j) + jdxc[i - 1] / hxxc[i - 1] * TXX(i, j));
-      s += a.diffusivity / jdxf[i - 1] * (-jdxf[i - 1] / hy * TXY(i, j) + jdxf[i - 1] / hy * TXY(i, j + 1));
+      s += cf.dfx[i] * (-cf.qc[i - 1] * TXX(i - 1, j) + cf.qc[i] * TXX(i, j));
+      s += cf.dfx[i] * (-cf.cyf[i] * TXY(i, j) + cf.cyf[i] * TXY(i, j + 1));
       double g = 0.;
-      g -= 1. / hxxf[i - 1] * (-a.p[icc(i - 1, j)] + a.p[icc(i, j)]);
-      s += surface_force(DEN0(i - 1, j), DEN0(i, j), CURV(i - 1, j), CURV(i, j), VOF(i - 1, j), VOF(i, j),
-                         hxxf[i - 1], a.refdeninv, a.tension);
+      g -= cf.rhf[i] * (-a.p[icc(i - 1, j)] + a.p[icc(i, j)]);
+      {
+        // force.c:37-49
+        const double d = 0.5 * DEN0(i - 1, j) + 0.5 * DEN0(i, j);
+        const double k = 0.5 * CURV(i - 1, j) + 0.5 * CURV(i, j);
+        const double delta = cf.rhf[i] * (-VOF(i - 1, j) + VOF(i, j));
+        s += d * a.refdeninv * a.tension * k * delta;
+      }
       s += a.grav * (0.5 * DEN0(i - 1, j) + 0.5 * DEN0(i, j));
       if (a.srca_ux) a.srca_ux[L::fx(i, j, nx)] = s;
       const double sb = a.stage != 0 ? a.srcb_ux[L::fx(i, j, nx)] : 0.;
@@ -350,32 +381,35 @@ __global__ __launch_bounds__(TX* TY) void k_pre(PreArgs a) {
     store_with_halo(a.uxs, i, j, ny, 1, u, ixf);
     if (i == nx) store_with_halo(a.uxs, nx + 1, j, ny, 1, 0., ixf);
   }
-  // uy at face j (predict/uy.c)
+  // uy at face j (predict/uy.c:29-251)
   {
-    const double* hxxf = a.hxxf;
-    const double* jdxf = a.jdxf;
-    const double* jdxc = a.jdxc;
+    const Coef& cf = a.cf;
     double s = 0.;
     {
-      const double mm = 0.5 * jdxf[i - 1] / hxxf[i - 1] * MFX(i, j - 1) + 0.5 * jdxf[i - 1] / hxxf[i - 1] * MFX(i, j);
-      const double mp = 0.5 * jdxf[i] / hxxf[i] * MFX(i + 1, j - 1) + 0.5 * jdxf[i] / hxxf[i] * MFX(i + 1, j);
+      const double mm = cf.eqf[i] * MFX(i, j - 1) + cf.eqf[i] * MFX(i, j);
+      const double mp = cf.eqf[i + 1] * MFX(i + 1, j - 1) + cf.eqf[i + 1] * MFX(i + 1, j);
       const double um = 0.5 * UY(i - 1, j) + 0.5 * UY(i, j);
       const double up = 0.5 * UY(i, j) + 0.5 * UY(i + 1, j);
-      s -= 1. / jdxc[i - 1] * (-mm * um + mp * up);
+      s -= cf.rjc[i] * (-mm * um + mp * up);
     }
     {
-      const double mm = 0.5 * jdxc[i - 1] / hy * MFY(i, j - 1) + 0.5 * jdxc[i - 1] / hy * MFY(i, j);
-      const double mp = 0.5 * jdxc[i - 1] / hy * MFY(i, j) + 0.5 * jdxc[i - 1] / hy * MFY(i, j + 1);
+      const double mm = cf.ecy[i] * MFY(i, j - 1) + cf.ecy[i] * MFY(i, j);
+      const double mp = cf.ecy[i] * MFY(i, j) + cf.ecy[i] * MFY(i, j + 1);
       const double um = 0.5 * UY(i, j - 1) + 0.5 * UY(i, j);
       const double up = 0.5 * UY(i, j) + 0.5 * UY(i, j + 1);
-      s -= 1. / jdxc[i - 1] * (-mm * um + mp * up);
+      s -= cf.rjc[i] * (-mm * um + mp * up);
     }
-    s += a.diffusivity / jdxc[i - 1] * (-jdxf[i - 1] / hxxf[i - 1] * TXY(i, j) + jdxf[i] / hxxf[i] * TXY(i + 1, j));
-    s += a.diffusivity / jdxc[i - 1] * (-jdxc[i - 1] / hy * TYY(i, j - 1) + jdxc[i - 1] / hy * TYY(i, j));
+    s += cf.dcx[i] * (-cf.qf[i] * TXY(i, j) + cf.qf[i + 1] * TXY(i + 1, j));
+    s += cf.dcx[i] * (-cf.cyc[i] * TYY(i, j - 1) + cf.cyc[i] * TYY(i, j));
     double g = 0.;
-    g -= 1. / hy * (-a.p[icc(i, j - 1)] + a.p[icc(i, j)]);
-    s += surface_force(DEN0(i, j - 1), DEN0(i, j), CURV(i, j - 1), CURV(i, j), VOF(i, j - 1), VOF(i, j), hy,
-                       a.refdeninv, a.tension);
+    g -= cf.rhy * (-a.p[icc(i, j - 1)] + a.p[icc(i, j)]);
+    {
+      // force.c:72-84
+      const double d = 0.5 * DEN0(i, j - 1) + 0.5 * DEN0(i, j);
+      const double k = 0.5 * CURV(i, j - 1) + 0.5 * CURV(i, j);
+      const double delta = cf.rhy * (-VOF(i, j - 1) + VOF(i, j));
+      s += d * a.refdeninv * a.tension * k * delta;
+    }
     if (a.srca_uy) a.srca_uy[L::ci(i, j, nx)] = s;
     const double sb = a.stage != 0 ? a.srcb_uy[L::ci(i, j, nx)] : 0.;
     const double d0 = 0.5 * DEN0(i, j - 1) + 0.5 * DEN0(i, j);
@@ -387,7 +421,6 @@ __global__ __launch_bounds__(TX* TY) void k_pre(PreArgs a) {
     if (i == nx) store_with_halo(a.uys, nx + 1, j, ny, 1, kUyXp, icc);
   }
 }
-
 // ---------------------------------------------------------------------------
 // projection + pressure (correct_velocity/ux.c, uy.c; update_pressure.c)
 // ---------------------------------------------------------------------------
@@ -482,6 +515,80 @@ static int scratch(const domain_t* d, const interface_t* s, const fluid_t* f) {
   return 0;
 }
 
+struct CoefTables {
+  const DomainCtx* ctx = nullptr;
+  uint64_t hash = 0;
+  double re = 0.;
+  double* buf = nullptr;
+  Coef cf{};
+};
+static CoefTables g_coef;
+
+static int coef_tables(const domain_t* d, const DomainCtx* c, double Re, Coef* out) {
+  if (g_coef.ctx == c && g_coef.hash == c->hash && g_coef.re == Re && g_coef.buf) {
+    *out = g_coef.cf;
+    return 0;
+  }
+  const size_t nx = d->mysizes[0], w = nx + 2;
+  std::vector<double> h(14 * w, 0.);
+  double* t[14];
+  for (int k = 0; k < 14; k++) t[k] = h.data() + k * w;
+  const double hy = d->hy;
+  const double diffusivity = 1. / Re;
+  for (size_t i = 1; i <= nx + 1; i++) {
+    const double hf = d->hxxf[i - 1], jf = d->jdxf[i - 1];
+    t[0][i] = 1. / hf;
+    t[2][i] = 1. / jf;
+    t[4][i] = jf / hf;
+    t[6][i] = 0.5 * jf / hf;
+    t[9][i] = jf / hy;
+    t[10][i] = diffusivity / jf;
+    t[12][i] = hf;
+  }
+  for (size_t i = 1; i <= nx; i++) {
+    const double hc = d->hxxc[i - 1], jc = d->jdxc[i - 1];
+    t[1][i] = 1. / hc;
+    t[3][i] = 1. / jc;
+    t[5][i] = jc / hc;
+    t[7][i] = 0.5 * jc / hy;
+    t[8][i] = jc / hy;
+    t[11][i] = diffusivity / jc;
+    t[13][i] = hc;
+  }
+  (void)hipStreamSynchronize(stream());
+  (void)hipFree(g_coef.buf);
+  g_coef.buf = nullptr;
+  SBFS_HIP(hipMalloc(&g_coef.buf, sizeof(double) * h.size()));
+  SBFS_HIP(hipMemcpy(g_coef.buf, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+  const double* b = g_coef.buf;
+  Coef cf;
+  cf.rhf = b + 0 * w;
+  cf.rhc = b + 1 * w;
+  cf.rjf = b + 2 * w;
+  cf.rjc = b + 3 * w;
+  cf.qf = b + 4 * w;
+  cf.qc = b + 5 * w;
+  cf.eqf = b + 6 * w;
+  cf.ecy = b + 7 * w;
+  cf.cyc = b + 8 * w;
+  cf.cyf = b + 9 * w;
+  cf.dfx = b + 10 * w;
+  cf.dcx = b + 11 * w;
+  cf.hxxf1 = b + 12 * w;
+  cf.hxxc1 = b + 13 * w;
+  cf.rhy = 1. / hy;
+  g_coef.ctx = c;
+  g_coef.hash = c->hash;
+  g_coef.re = Re;
+  g_coef.cf = cf;
+  *out = cf;
+  return 0;
+}
+
+__global__ void k_march(MarchArgs a);  // march.hip
+
+int poisson_transform_fused(const domain_t* d, DomainCtx* c, fluid_t* f, double** rhs);  // poisson.hip
+
 int poisson_solve_fused(const domain_t* d, DomainCtx* c, const double* dt_dev, double g_old,
                         double g_new, const double* uxs, const double* uys, const double* vofn,
                         fluid_t* f);  // poisson.hip
@@ -510,7 +617,18 @@ int integrate_fused(const domain_t* domain, fluid_t* fluid, interface_t* interfa
   for (int n = 0; n < 2; n++)
     if (check_array(domain, &interface->src[n], NA_SRC, 8, "src")) return 1;
   if (scratch(domain, interface, fluid)) return 1;
+  Coef cf;
+  if (coef_tables(domain, c, fluid->Re, &cf)) return 1;
   const int nx = (int)c->nx, ny = (int)c->ny;
+  const bool use_tiles = getenv("SBFS_PRE_TILES") != nullptr;
+  double* rhs = nullptr;
+  if (!use_tiles && poisson_transform_fused(domain, c, fluid, &rhs)) return 1;
+  // rows per band: enough wavefronts to fill the device several times over
+  int band = 64;
+  {
+    const long strips = (nx + kStrip - 1) / kStrip;
+    while (band > 16 && strips * ((ny + band - 1) / band) < 4096) band /= 2;
+  }
   if (compute_dt_device(domain, c, fluid, interface)) return 1;
   const double* dtp = c->dt_dev;
   for (int k = 0; k < 3; k++) {
@@ -543,6 +661,7 @@ int integrate_fused(const domain_t* domain, fluid_t* fluid, interface_t* interfa
     a.jdxf = c->jdxf;
     a.jdxc = c->jdxc;
     a.hy = c->hy;
+    a.cf = cf;
     a.denr = fluid->denr;
     a.dlo = fmin(1., fluid->denr);
     a.dhi = fmax(1., fluid->denr);
@@ -560,18 +679,71 @@ int integrate_fused(const domain_t* domain, fluid_t* fluid, interface_t* interfa
     a.stage = k;
     a.nx = nx;
     a.ny = ny;
-    const dim3 grid(cdiv(nx, kTX), cdiv(ny, kTY)), block(kTX, kTY);
-    SBFS_TIMED("k_pre", hipLaunchKernelGGL((k_pre<kTX, kTY>), grid, block, 0, stream(), a));
-    if (check_launch("integrate: k_pre")) return 1;
-    // the new vof becomes the state; the old one is the next scratch
+    const double g_old = rk((k + 2) % 3).g, g_new = r.g;
+    if (use_tiles) {
+      const dim3 grid(cdiv(nx, kTX), cdiv(ny, kTY)), block(kTX, kTY);
+      SBFS_TIMED("k_pre", hipLaunchKernelGGL((k_pre<kTX, kTY>), grid, block, 0, stream(), a));
+      if (check_launch("integrate: k_pre")) return 1;
+    } else {
+      MarchArgs m;
+      m.vof = a.vof;
+      m.ux = a.ux;
+      m.uy = a.uy;
+      m.p = a.p;
+      m.psi0 = D(fluid->psi[0]);
+      m.srcb_vof = a.srcb_vof;
+      m.srcb_ux = a.srcb_ux;
+      m.srcb_uy = a.srcb_uy;
+      m.vofn = a.vofn;
+      m.uxs = a.uxs;
+      m.uys = a.uys;
+      m.rhs = rhs;
+      m.srca_vof = a.srca_vof;
+      m.srca_ux = a.srca_ux;
+      m.srca_uy = a.srca_uy;
+      m.den1 = a.den1;
+      m.txy = a.txy;
+      m.hxxf1 = cf.hxxf1;
+      m.hxxc1 = cf.hxxc1;
+      m.cf = cf;
+      m.hy = c->hy;
+      m.denr = a.denr;
+      m.dlo = a.dlo;
+      m.dhi = a.dhi;
+      m.visr = a.visr;
+      m.vlo = a.vlo;
+      m.vhi = a.vhi;
+      m.refdeninv = a.refdeninv;
+      m.tension = a.tension;
+      m.grav = a.grav;
+      m.refden = fluid->refden;
+      m.norm = 2. * (double)domain->glsizes[0];
+      m.dt = dtp;
+      m.ca = r.a;
+      m.cb = r.b;
+      m.cg = r.g;
+      m.g_old = g_old;
+      m.g_new = g_new;
+      m.stage = k;
+      m.nx = nx;
+      m.ny = ny;
+      m.band = band;
+      const dim3 grid(cdiv(nx, kStrip), cdiv(ny, band)), block(64);
+      SBFS_TIMED("k_march", hipLaunchKernelGGL(k_march, grid, block, 0, stream(), m));
+      if (check_launch("integrate: k_march")) return 1;
+    }
     {
+      // the new vof becomes the state; the old one is the next scratch
       void* t = interface->vof.data;
       interface->vof.data = g_scr.vof;
       g_scr.vof = (double*)t;
     }
-    const double g_old = rk((k + 2) % 3).g, g_new = r.g;
-    if (poisson_solve_fused(domain, c, dtp, g_old, g_new, g_scr.ux, g_scr.uy, D(interface->vof), fluid))
+    if (use_tiles) {
+      if (poisson_solve_fused(domain, c, dtp, g_old, g_new, g_scr.ux, g_scr.uy, D(interface->vof), fluid))
+        return 1;
+    } else if (poisson_transform_fused(domain, c, fluid, nullptr)) {
       return 1;
+    }
     PostArgs q;
     q.uxs = g_scr.ux;
     q.uys = g_scr.uy;

@@ -1,0 +1,64 @@
+// Arguments of the marching stage kernel (march.hip).
+#pragma once
+
+#include "sbfs_internal.hpp"
+
+namespace sbfs {
+
+// output columns per wavefront: 58 + 3 halo columns on each side = 64 lanes.
+// The right halo needs 3: the Poisson rhs of the last output cell reads the
+// predicted ux of the face to its right, whose mass flux upwinds into the
+// next cell, whose normal needs the corner gradient one column further.
+constexpr int kStrip = 58;
+constexpr int kWarm = 6;    // rows marched before the first output row of a band
+
+// per-column quotients of the metrics (see fused.hip)
+struct Coef {
+  const double* rhf;
+  const double* rhc;
+  const double* rjf;
+  const double* rjc;
+  const double* qf;
+  const double* qc;
+  const double* eqf;
+  const double* ecy;
+  const double* cyc;
+  const double* cyf;
+  const double* dfx;
+  const double* dcx;
+  const double* hxxf1;  // HXXF(i), 1-based
+  const double* hxxc1;  // HXXC(i), 1-based
+  double rhy;
+};
+
+struct MarchArgs {
+  const double* vof;
+  const double* ux;
+  const double* uy;
+  const double* p;
+  const double* psi0;
+  const double* srcb_vof;
+  const double* srcb_ux;
+  const double* srcb_uy;
+  double* vofn;
+  double* uxs;
+  double* uys;
+  double* rhs;
+  double* srca_vof;
+  double* srca_ux;
+  double* srca_uy;
+  double* den1;
+  double* txy;
+  const double* hxxf1;
+  const double* hxxc1;
+  Coef cf;
+  double hy;
+  double denr, dlo, dhi, visr, vlo, vhi;
+  double refdeninv, tension, grav, refden, norm;
+  const double* dt;
+  double ca, cb, cg, g_old, g_new;
+  int stage;
+  int nx, ny, band;
+};
+
+}  // namespace sbfs

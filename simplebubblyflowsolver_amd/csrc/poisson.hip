@@ -43,6 +43,7 @@ void poisson_destroy(PoissonPlan* p) {
   (void)hipFree(p->gfl);
   (void)hipFree(p->X);
   (void)hipFree(p->rtab);
+  (void)hipFree(p->rhs);
   delete p;
 }
 
@@ -442,8 +443,12 @@ static int poisson_tail(const domain_t* d, PoissonPlan* p, fluid_t* f) {
   // tridiagonal matrix l = u = 1/hy^2 (compute_potential.c:173-176)
   const double l = 1. / d->hy / d->hy;
   const int pin = options().gauge == SBFS_GAUGE_PINNED ? 1 : 0;
-  const int L = partition_chunk(ny, options().chunk);
-  if (options().tdm == SBFS_TDM_PARTITIONED && L != 0 && pin) {
+  // chunk: the requested length, doubled while the reduced system would
+  // exceed the separator kernel's LDS column (128 separators)
+  int req = options().chunk;
+  while (req < 64 && ny / req > 128) req *= 2;
+  const int L = partition_chunk(ny, req);
+  if (options().tdm == SBFS_TDM_PARTITIONED && L != 0 && pin && ny / L <= 128) {
     if (p->L != L && partition_setup(p, d, L)) return 1;
     if (partition_solve(p, pin)) return 1;
   } else {
@@ -457,6 +462,26 @@ static int poisson_tail(const domain_t* d, PoissonPlan* p, fluid_t* f) {
                                 (const double*)p->buf, (double*)f->psi[1].data, nx, ny, m, p->tw,
                                 p->t1, p->t2));
   return check_launch("poisson: DCT-III + psi");
+}
+
+// marching path: `rhs` (filled by k_march) -> DCT-II rows -> tridiagonal ->
+// DCT-III into psi[1].  Called with rhs_out non-null it only returns the
+// right-hand-side buffer (allocating the plan on first use).
+int poisson_transform_fused(const domain_t* d, DomainCtx* c, fluid_t* f, double** rhs_out) {
+  PoissonPlan* p = plan_get(d, c);
+  if (!p) return 1;
+  if (!p->rhs) SBFS_HIP(hipMalloc(&p->rhs, sizeof(double) * (size_t)p->nx * p->ny));
+  if (rhs_out) {
+    *rhs_out = p->rhs;
+    return 0;
+  }
+  const int m = p->m;
+  const size_t lds = sizeof(double2) * 2 * (size_t)(m > 0 ? m : 1);
+  SBFS_TIMED("k_dct2_rows",
+             hipLaunchKernelGGL(k_dct2_rows, dim3(p->ny), dim3(row_threads(m)), lds, stream(),
+                                (const double*)p->rhs, p->buf, p->nx, m, p->tw, p->t1, p->t2));
+  if (check_launch("poisson: DCT-II")) return 1;
+  return poisson_tail(d, p, f);
 }
 
 int poisson_solve_fused(const domain_t* d, DomainCtx* c, const double* dt_dev, double g_old,

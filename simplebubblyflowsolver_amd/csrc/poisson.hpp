@@ -11,6 +11,7 @@ struct PoissonPlan {
   int nx = 0, ny = 0, m = 0;
   double lx = 0.;
   double* buf = nullptr;     // nx * ny, modes r[j][m]
+  double* rhs = nullptr;     // nx * ny, right-hand side rows (marching path)
   double* v = nullptr;       // nx * ny, sequential-solver factors
   double* q1 = nullptr;      // nx * ny, sequential-solver spike
   double* evals = nullptr;   // nx

@@ -99,48 +99,73 @@ __global__ void k_part_reduce(const double* __restrict__ q, const double* __rest
 // Separators of every mode: assemble the reduced right-hand side and solve
 // the periodic reduced system with the reference algorithm.  Its pivots,
 // factors and spike depend only on the mode, so they come from tables
-// (rtab, pred) built with the same arithmetic; the only loop-carried
-// dependence left is the running value in a register.
+// (rtab, pred) built with the same arithmetic.  One wavefront handles 64
+// modes; the separators live in LDS ([P][64], P <= kMaxP) and the table
+// values are fetched in groups ahead of the dependent chain, whose only
+// carried value stays in a register.
+constexpr int kMaxP = 128;
+constexpr int kSepGroup = 16;
+
 template <int L>
-__global__ void k_part_separators(const double* __restrict__ q, const double* __restrict__ gfl,
-                                  const double* __restrict__ pred,
-                                  const double* __restrict__ rtab, double* __restrict__ X,
-                                  int nx, int P, double a, int pin_mode0) {
+__global__ __launch_bounds__(64) void k_part_separators(
+    const double* __restrict__ q, const double* __restrict__ gfl, const double* __restrict__ pred,
+    const double* __restrict__ rtab, double* __restrict__ X, int nx, int P, double a,
+    int pin_mode0) {
   constexpr int K = L - 1;
-  const int mo = blockIdx.x * blockDim.x + threadIdx.x;
-  if (mo >= nx) return;
+  __shared__ double sx[kMaxP * 64];
+  const int t = threadIdx.x;
+  const int mo = blockIdx.x * 64 + t;
+  const bool live = mo < nx;
+  const int mm = live ? mo : nx - 1;
   for (int c = 0; c < P; c++) {
-    const double gl = gfl[(size_t)(2 * c + 1) * nx + mo];
-    const double gf = gfl[(size_t)(2 * ((c + 1) % P)) * nx + mo];
-    X[(size_t)c * nx + mo] = (q[(size_t)(c * L + K) * nx + mo] - a * gl) - a * gf;
+    const double gl = gfl[(size_t)(2 * c + 1) * nx + mm];
+    const double gf = gfl[(size_t)(2 * ((c + 1) % P)) * nx + mm];
+    sx[c * 64 + t] = (q[(size_t)(c * L + K) * nx + mm] - a * gl) - a * gf;
   }
-  const double lo = pred[mo], di = pred[(size_t)nx + mo], up = pred[(size_t)2 * nx + mo];
-  const double ilast = pred[(size_t)3 * nx + mo], ok = pred[(size_t)4 * nx + mo];
-  const double den = pred[(size_t)5 * nx + mo];
+  const double lo = pred[mm], di = pred[(size_t)nx + mm], up = pred[(size_t)2 * nx + mm];
+  const double ilast = pred[(size_t)3 * nx + mm], ok = pred[(size_t)4 * nx + mm];
+  const double den = pred[(size_t)5 * nx + mm];
   const double* val = rtab;
   const double* vv = rtab + (size_t)P * nx;
   const double* q1 = rtab + (size_t)2 * P * nx;
-  const size_t st = nx;
-  double* x = X + mo;
   const int N = P - 1;
   // gtsv on the first N rows (src/tdm.c:26-48)
-  double prev = x[0] / di;
-  x[0] = prev;
-  for (int r = 1; r < N - 1; r++) {
-    prev = val[r * st + mo] * (x[r * st] - lo * prev);
-    x[r * st] = prev;
+  double prev = sx[t] / di;
+  sx[t] = prev;
+  for (int r0 = 1; r0 < N - 1; r0 += kSepGroup) {
+    double tv[kSepGroup];
+#pragma unroll
+    for (int u = 0; u < kSepGroup; u++) tv[u] = (r0 + u < N - 1) ? val[(size_t)(r0 + u) * nx + mm] : 0.;
+#pragma unroll
+    for (int u = 0; u < kSepGroup; u++) {
+      const int r = r0 + u;
+      if (r < N - 1) {
+        prev = tv[u] * (sx[r * 64 + t] - lo * prev);
+        sx[r * 64 + t] = prev;
+      }
+    }
   }
-  double nxt = ok != 0. ? ilast * (x[(N - 1) * st] - lo * prev) : 0.;
-  x[(N - 1) * st] = nxt;
-  for (int r = N - 2; r >= 0; r--) {
-    nxt = x[r * st] - vv[r * st + mo] * nxt;
-    x[r * st] = nxt;
+  double nxt = ok != 0. ? ilast * (sx[(N - 1) * 64 + t] - lo * prev) : 0.;
+  sx[(N - 1) * 64 + t] = nxt;
+  for (int r0 = N - 2; r0 >= 0; r0 -= kSepGroup) {
+    double tv[kSepGroup];
+#pragma unroll
+    for (int u = 0; u < kSepGroup; u++) tv[u] = (r0 - u >= 0) ? vv[(size_t)(r0 - u) * nx + mm] : 0.;
+#pragma unroll
+    for (int u = 0; u < kSepGroup; u++) {
+      const int r = r0 - u;
+      if (r >= 0) {
+        nxt = sx[r * 64 + t] - tv[u] * nxt;
+        sx[r * 64 + t] = nxt;
+      }
+    }
   }
   // closure (src/tdm.c:92-98); the singular mode pins the last separator
-  const double num = x[(P - 1) * st] - up * x[0] - lo * x[(P - 2) * st];
+  const double num = sx[(P - 1) * 64 + t] - up * sx[t] - lo * sx[(P - 2) * 64 + t];
   const double xl = ((pin_mode0 && mo == 0) || fabs(den) < DBL_EPSILON) ? 0. : num / den;
-  x[(P - 1) * st] = xl;
-  for (int r = 0; r < P - 1; r++) x[r * st] = x[r * st] + xl * q1[r * st + mo];
+  if (!live) return;
+  X[(size_t)(P - 1) * nx + mo] = xl;
+  for (int r = 0; r < P - 1; r++) X[(size_t)r * nx + mo] = sx[r * 64 + t] + xl * q1[(size_t)r * nx + mo];
 }
 
 template <int L>
