@@ -43,6 +43,7 @@ def lib() -> C.CDLL:
         _lib.oracle_tdm_periodic.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_int]
         _lib.oracle_tdm_partitioned.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_void_p]
         _lib.oracle_poisson_evals.argtypes = [C.c_void_p, C.c_void_p]
+        _lib.oracle_rkcoefs.argtypes = [C.c_void_p]
     return _lib
 
 
@@ -82,7 +83,25 @@ def fft(z: np.ndarray) -> np.ndarray:
     return out
 
 
-def tdm_periodic(q: np.ndarray, l: float, c: float, u: float, gauge: int = GAUGE_PINNED) -> np.ndarray:
+def rkcoefs() -> np.ndarray:
+    """(3, 3) table of {alpha, beta, gamma} per Runge-Kutta stage."""
+    out = np.zeros(9)
+    lib().oracle_rkcoefs(out.ctypes.data)
+    return out.reshape(3, 3)
+
+
+def poisson_evals(case) -> np.ndarray:
+    """Eigenvalues of the x second-difference operator per DCT mode."""
+    out = np.zeros(case.nx)
+    lib().oracle_poisson_evals(C.addressof(case.domain), out.ctypes.data)
+    return out
+
+
+def tdm_periodic(q: np.ndarray, l: float, c: float, u: float, gauge: int = GAUGE_REFERENCE) -> np.ndarray:
+    """Periodic tridiagonal solve.  GAUGE_REFERENCE runs the reference's
+    closure (its singular branch only when the denominator vanishes);
+    GAUGE_PINNED forces the singular branch, x[n-1] = 0, as the Poisson
+    solver does for mode 0."""
     q = np.ascontiguousarray(q, dtype=np.float64).copy()
     rc = lib().oracle_tdm_periodic(q.size, l, c, u, q.ctypes.data, gauge)
     if rc:

@@ -63,6 +63,13 @@ static const double k_rk[3][3] = {
 };
 enum { RKA = 0, RKB = 1, RKG = 2 };
 
+/* the table above, row by row: {alpha, beta, gamma} of stages 0, 1, 2 */
+int oracle_rkcoefs(double out[9]) {
+  for (int k = 0; k < 3; k++)
+    for (int c = 0; c < 3; c++) out[3 * k + c] = k_rk[k][c];
+  return 0;
+}
+
 /* ------------------------------------------------------------------ */
 /* options                                                             */
 /* ------------------------------------------------------------------ */

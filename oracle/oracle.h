@@ -74,6 +74,7 @@ int oracle_fft(size_t m, const double * in, double * out); /* interleaved comple
 int oracle_tdm_periodic(int n, double l, double c, double u, double * q,
                         int gauge);
 int oracle_tdm_partitioned(int n, int chunk, double a, double b, double * q);
+int oracle_rkcoefs(double out[9]); /* {alpha, beta, gamma} x 3 stages */
 int oracle_poisson_evals(const domain_t * domain, double * evals);
 int oracle_diagnostics(const domain_t *, const fluid_t *, const interface_t *,
                        double out[8]);
