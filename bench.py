@@ -36,18 +36,38 @@ from simplebubblyflowsolver_amd import configs, solver  # noqa: E402
 METRIC = "Mcell-steps/s, mixed+learned vs plain FP64 at <=1% rise-velocity error"
 HBM_PEAK_GBS = 8000.0
 
-# compulsory HBM bytes per interior cell of one launch (FP64 fields read once
-# and written once); the band-only normal is counted in full
+# Algorithmic HBM bytes per interior cell of one launch: every FP64 field
+# the kernel must read or write, counted once.  k_march differs per
+# Runge-Kutta stage (stage 0: reads vof, ux, uy, p, psi0 and writes vof',
+# ux*, uy*, rhs and the three alpha sources = 12 fields; stage 1: + the
+# three beta sources = 15; stage 2: 8 reads, writes vof', ux*, uy*, rhs,
+# den[1], txy = 14), so its entry is the mean over the three launches of a
+# step.  The staged kernels follow the reference's stage boundaries.
 BYTES_PER_CELL = {
     "k_density": 16, "k_visux": 16, "k_visuy": 16, "k_dvof": 24, "k_normal_curv": 56,
     "k_force": 40, "k_fluxx": 48, "k_fluxy": 48, "k_vof_advance": 48, "k_mass_flux": 48,
     "k_stress": 56, "k_rhs_ux": 80, "k_rhs_uy": 72, "k_predict_ux": 56, "k_predict_uy": 56,
-    "k_dct2_rhs": 40, "k_tdm_sequential": 48, "k_part_reduce": 8, "k_part_separators": 0,
+    "k_dct2_rhs": 40, "k_tdm_sequential": 16, "k_part_reduce": 8, "k_part_separators": 0,
     "k_part_expand": 16, "k_dct3_psi": 16, "k_correct_ux": 40, "k_correct_uy": 40,
     "k_update_p": 24, "k_dt_partial": 16, "k_dt_final": 0,
     # fused path
-    "k_pre": 0, "k_post": 0,
+    "k_march": (12 + 15 + 14) * 8 / 3, "k_pre": (10 + 13 + 12) * 8 / 3, "k_dct2_rows": 16,
+    "k_post": 72,
 }
+
+# HBM bytes per launch measured with rocprofv3 counters (FETCH_SIZE and
+# WRITE_SIZE, scaled by a copy kernel of known size), read from the
+# profile summaries; a line "traffic_bytes_per_launch: <kernel> <workload> <N>".
+PROFILES = ROOT / "profiles"
+
+
+def measured_traffic(kernel: str, workload: str):
+    for f in sorted(PROFILES.glob("*.md")):
+        for line in f.read_text().splitlines():
+            t = line.strip().strip("`").split()
+            if len(t) == 4 and t[0] == "traffic_bytes_per_launch:" and t[1] == kernel and t[2] == workload:
+                return float(t[3]), f.name
+    return None, None
 
 
 def workload(name: str) -> configs.Workload:
@@ -100,7 +120,9 @@ def main() -> int:
     ap.add_argument("--path", choices=["fused", "staged"], default="fused")
     ap.add_argument("--workload", default="swarm")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--copy-mib", type=int, default=1024,
+                    help="buffer size of the copy-bandwidth measurement (0: skip)")
     ap.add_argument("--no-profile", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -134,18 +156,37 @@ def main() -> int:
     g.close()
 
     value = w.cells * a.steps / t / 1e6
+    copy_gbs = None
+    if a.copy_mib > 0:
+        g = C.c_double(0.0)
+        lib.sbfs_copy_bandwidth.argtypes = [C.c_size_t, C.c_int, C.POINTER(C.c_double)]
+        be.check(lib.sbfs_copy_bandwidth(a.copy_mib << 20, 20, C.byref(g)), "copy_bandwidth")
+        copy_gbs = round(g.value, 1)
     roof = None
+    kernels = []
     if prof:
+        total = sum(e[2] for e in prof)
+        for n, cnt, ms in prof:
+            bpc = BYTES_PER_CELL.get(n)
+            avg_s = ms / cnt / 1e3
+            kernels.append({"name": n, "launches": cnt, "total_ms": round(ms, 3),
+                            "avg_us": round(avg_s * 1e6, 2),
+                            "alg_gbs": round(bpc * w.cells / avg_s / 1e9, 1) if bpc else None})
         name, cnt, ms = max(prof, key=lambda e: e[2])
         avg_s = ms / cnt / 1e3
-        bpc = BYTES_PER_CELL.get(name)
-        achieved = bpc * w.cells / avg_s / 1e9 if bpc else None
-        roof = {"bound": "hbm", "kernel": name, "avg_us": round(avg_s * 1e6, 2),
-                "bytes_per_cell": bpc,
-                "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                "traffic": None,
-                "share_of_gpu_time": round(ms / sum(e[2] for e in prof), 3)}
+        bpc = BYTES_PER_CELL[name]
+        alg_bytes = bpc * w.cells
+        achieved = alg_bytes / avg_s / 1e9
+        traffic, src = measured_traffic(name, w.name)
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic,
+                "kernel": name, "avg_us": round(avg_s * 1e6, 2), "launches": cnt,
+                "alg_bytes_per_launch": round(alg_bytes), "bytes_per_cell": round(bpc, 3),
+                "share_of_gpu_time": round(ms / total, 3),
+                "traffic_source": src,
+                "measured_copy_gbs": copy_gbs,
+                "frac_of_measured_copy": round(achieved / copy_gbs, 4) if copy_gbs else None}
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Mcell-steps/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(t / a.steps * 1e3, 4),
@@ -156,7 +197,7 @@ def main() -> int:
                    "volume_drift": abs(d1["volume"] - d0["volume"]) / d0["volume"],
                    "divmax": d1["divmax"]},
         "roofline": roof,
-        "kernels": [{"name": n, "launches": c, "total_ms": round(m, 3)} for n, c, m in prof],
+        "kernels": kernels,
     }
     if not a.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)

@@ -99,6 +99,11 @@ extern int sbfs_profile_entry(int k, char * name, size_t len, int * count,
 extern int sbfs_dct_rows(int inverse, size_t n, size_t howmany,
                          const double * in, double * out);
 
+/* device copy bandwidth: `reps` launches of a plain copy kernel (k_copy)
+ * over two buffers of `bytes` each (a multiple of 16); *gbs = bytes read +
+ * written per second / 1e9.  Allocates and frees its own buffers. */
+extern int sbfs_copy_bandwidth(size_t bytes, int reps, double * gbs);
+
 SBFS_EXTERN_C_END
 
 #endif /* SBFS_H */
