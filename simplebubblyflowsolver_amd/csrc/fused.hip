@@ -585,7 +585,6 @@ static int coef_tables(const domain_t* d, const DomainCtx* c, double Re, Coef* o
   return 0;
 }
 
-__global__ void k_march(MarchArgs a);  // march.hip
 
 int poisson_transform_fused(const domain_t* d, DomainCtx* c, fluid_t* f, double** rhs);  // poisson.hip
 
@@ -599,6 +598,36 @@ static void swap_data(array_t& x, array_t& y) {
   void* t = x.data;
   x.data = y.data;
   y.data = t;
+}
+
+__global__ void k_march(MarchArgs a);  // march.hip
+
+// Rows per band of k_march.  Every wavefront marches its band plus kWarm + 2
+// rows of ring fill, and a wavefront lives for the whole band, so the grid
+// is sized to one round of the device: as many bands as there are resident
+// wavefront slots per strip (occupancy x CUs), each as short as that
+// allows.  A partial second round would leave most SIMDs idle while it
+// runs.  SBFS_MARCH_BAND overrides the choice.
+static int march_band(int nx, int ny) {
+  static long slots = 0;
+  if (slots == 0) {
+    int dev = 0, per_cu = 0;
+    hipDeviceProp_t prop;
+    SBFS_HIP(hipGetDevice(&dev));
+    SBFS_HIP(hipGetDeviceProperties(&prop, dev));
+    SBFS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march, 64, 0));
+    slots = (long)(per_cu > 0 ? per_cu : 1) * prop.multiProcessorCount;
+  }
+  if (const char* e = getenv("SBFS_MARCH_BAND")) {
+    const int b = atoi(e);
+    if (b >= 1) return b < ny ? b : ny;
+  }
+  const long strips = (nx + kStrip - 1) / kStrip;
+  long bands = slots / strips;
+  if (bands < 1) bands = 1;
+  long band = (ny + bands - 1) / bands;
+  if (band < 4) band = 4;
+  return (int)(band < ny ? band : ny);
 }
 
 int integrate_fused(const domain_t* domain, fluid_t* fluid, interface_t* interface, double* dt) {
@@ -623,12 +652,8 @@ int integrate_fused(const domain_t* domain, fluid_t* fluid, interface_t* interfa
   const bool use_tiles = getenv("SBFS_PRE_TILES") != nullptr;
   double* rhs = nullptr;
   if (!use_tiles && poisson_transform_fused(domain, c, fluid, &rhs)) return 1;
-  // rows per band: enough wavefronts to fill the device several times over
-  int band = 64;
-  {
-    const long strips = (nx + kStrip - 1) / kStrip;
-    while (band > 16 && strips * ((ny + band - 1) / band) < 4096) band /= 2;
-  }
+  const int band = march_band(nx, ny);
+  if (band <= 0) return 1;
   if (compute_dt_device(domain, c, fluid, interface)) return 1;
   const double* dtp = c->dt_dev;
   for (int k = 0; k < 3; k++) {

@@ -18,8 +18,8 @@
 //   Poisson rhs          row s-1   (compute_potential.c:285-363)
 //
 // Older rows stay in register rings, x-neighbours come from DPP wavefront
-// shifts, so nothing is recomputed across rows and no LDS or barrier is
-// needed.  Every value is formed with the reference's operation order (the
+// shifts, so nothing is recomputed across rows and no LDS tile or barrier is
+// needed; a term shared by two neighbouring faces or rows is formed once.  Every value is formed with the reference's operation order (the
 // per-column quotients come from the coefficient tables), which keeps the
 // result bitwise equal to the staged kernels.
 #include <cfloat>
@@ -56,12 +56,13 @@ __device__ __forceinline__ int wrap_r(int j, int ny) {
 
 enum { T_RHF, T_RHC, T_RJF, T_RJC, T_QF, T_QC, T_EQF, T_ECY, T_CYC, T_CYF, T_DFX, T_DCX, T_HXXF, T_HXXC, T_N };
 
-__global__ __launch_bounds__(64) void k_march(MarchArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_march(MarchArgs a) {
   __shared__ double st[T_N][64];
   const int lane = threadIdx.x;
   const int nx = a.nx, ny = a.ny;
   const int i0 = 1 + blockIdx.x * kStrip;
-  const int c = i0 - 3 + lane;
+  const int cl = i0 - 3;  // column held by lane 0
+  const int c = cl + lane;
   const int jb = 1 + blockIdx.y * a.band;
   const int je = min(ny, jb + a.band - 1);
   const bool c_vof = c >= 0 && c <= nx + 1;   // cells incl. wall ghosts
@@ -88,9 +89,17 @@ __global__ __launch_bounds__(64) void k_march(MarchArgs a) {
   const double c1 = 1. / a.norm * -1. / dt_new * dt_old;
   const bool has_b = a.stage != 0;
 
-  auto ld_vof = [&](int j) { return c_vof ? a.vof[L::vof(c, wrap_r(j, ny), nx)] : 0.; };
-  auto ld_ux = [&](int j) { return c_fx ? a.ux[L::xf(c, wrap_r(j, ny), nx)] : 0.; };
-  auto ld_cc = [&](const double* f, int j) { return c_vof ? f[L::cc(c, wrap_r(j, ny), nx)] : 0.; };
+  // Row pointers are uniform and shifted to lane 0's column, so every access
+  // is a scalar base plus the lane's byte offset.
+  const ptrdiff_t wc = nx + 2, wf = nx + 1;
+  auto vof_row = [&](auto* f, int j) { return f + (ptrdiff_t)(j + 1) * wc + cl; };   // L::vof
+  auto cc_row = [&](auto* f, int j) { return f + (ptrdiff_t)j * wc + cl; };    // L::cc
+  auto xf_row = [&](auto* f, int j) { return f + (ptrdiff_t)j * wf + cl - 1; };  // L::xf
+  auto ci_row = [&](auto* f, int j) { return f + (ptrdiff_t)(j - 1) * nx + cl - 1; };  // L::ci
+  auto fx_row = [&](auto* f, int j) { return f + (ptrdiff_t)(j - 1) * (nx - 1) + cl - 2; };  // L::fx
+  auto ld_vof = [&](int j) { return c_vof ? vof_row(a.vof, wrap_r(j, ny))[lane] : 0.; };
+  auto ld_ux = [&](int j) { return c_fx ? xf_row(a.ux, wrap_r(j, ny))[lane] : 0.; };
+  auto ld_cc = [&](const double* f, int j) { return c_vof ? cc_row(f, wrap_r(j, ny))[lane] : 0.; };
   auto DEN = [&](double v) { return mix_clamp(a.denr, a.dlo, a.dhi, v); };
   // viscosity at x face c from the vof on its two sides (viscosity.c:37-53)
   auto VISUX = [&](int col, double vl, double vr) {
@@ -102,20 +111,28 @@ __global__ __launch_bounds__(64) void k_march(MarchArgs a) {
   };
   auto VISUY = [&](double vb, double vt) { return mix_clamp(a.visr, a.vlo, a.vhi, 0.5 * vb + 0.5 * vt); };
 
-  // rings (row offsets relative to the current step s)
+  // Rings (row offsets relative to the current step s).  Several terms of
+  // the reference's stencils are the same expression evaluated at the next
+  // column or row (the right face of cell c is the left face of cell c+1);
+  // they are formed once and taken from the neighbouring lane (DPP) or from
+  // the previous step, which gives the same bits.
   double vm1 = 0., v0 = 0., vp1 = 0., vp2 = 0.;             // vof s-1..s+2
   double g0x = 0., g0y = 0., g1x = 0., g1y = 0., g2x = 0., g2y = 0.;  // corners s..s+2
   double g0xr = 0., g0yr = 0., g1xr = 0., g1yr = 0.;         // corners of c+1, rows s, s+1
   double n0x = 0., n0y = 0., n0z = 0., n1x = 0., n1y = 0., n1z = 0.;  // normals s, s+1
   double km1 = 0., k0 = 0.;                                 // curvature s-1, s
-  double fx0 = 0., mfxm1 = 0., mfx0 = 0.;                   // x fluxes s-1, s
-  double fy0 = 0., fyp1 = 0., mfym1 = 0., mfy0 = 0., mfyp1 = 0.;  // y faces s-1..s+1
-  double txx0 = 0., tyym1 = 0., tyy0 = 0., txy0 = 0., txy1 = 0.;
-  double vnm2 = 0., vnm1 = 0., vn0 = 0.;                    // new vof s-2..s
-  double uxm1 = 0., ux0 = 0., uxp1 = 0., uym1 = 0., uy0 = 0., uyp1 = 0.;
+  double mfxm1 = 0., mfx0 = 0.;                             // x mass fluxes s-1, s
+  double fy0 = 0., fyp1 = 0., mfy0 = 0., mfyp1 = 0.;        // y faces s, s+1
+  double visx0 = 0., visx1 = 0., visy0 = 0., visy1 = 0.;    // viscosities rows / faces s, s+1
+  double txy0 = 0., cyf_txy0 = 0., cyc_tyym1 = 0.;          // stress terms of rows s, s-1
+  double d_0 = 0.;                                          // density of vof row s
+  double vn0 = 0., dnm1 = 0., dn0 = 0.;                     // new vof row s, its density s-1, s
+  double ux0 = 0., uxp1 = 0., uy0 = 0., uyp1 = 0.;
   double pm1 = 0., p0 = 0.;
-  double psm2 = 0., psm1 = 0., ps0 = 0.;                    // psi[0] s-2..s
-  double uxs_m1 = 0., uys_m1 = 0., uys0 = 0., uxs0 = 0.;    // predicted u rows s-1, s
+  double psm1 = 0., ps0 = 0.;                               // psi[0] s-1, s
+  double qy_ux = 0., qy_uy = 0.;                            // y momentum fluxes at face s
+  double q_uxs_m1 = 0., cyc_uys_m1 = 0.;                    // rhs terms of row s-1
+  double d1x_m1 = 0., d1y = 0., cg_ym = 0.;                 // rhs densities / y gradient
   const int s_first = jb - kWarm;
   // prefetched inputs of the next step
   double nv = ld_vof(s_first + 2), nux = ld_ux(s_first + 1), nuy = ld_cc(a.uy, s_first + 1);
@@ -123,18 +140,19 @@ __global__ __launch_bounds__(64) void k_march(MarchArgs a) {
   for (int s = s_first; s <= je + 1; s++) {
     // -- new input rows -----------------------------------------------------
     vm1 = v0; v0 = vp1; vp1 = vp2; vp2 = nv;
-    uxm1 = ux0; ux0 = uxp1; uxp1 = nux;
-    uym1 = uy0; uy0 = uyp1; uyp1 = nuy;
+    ux0 = uxp1; uxp1 = nux;
+    uy0 = uyp1; uyp1 = nuy;
     pm1 = p0; p0 = np;
-    psm2 = psm1; psm1 = ps0; ps0 = nps;
+    psm1 = ps0; ps0 = nps;
     const bool outrow = s >= jb && s <= je;
     double sbv = 0., sbx = 0., sby = 0.;
     if (has_b && c_in) {
       // the new vof of every marched row needs its beta source; the
       // predicted uy of row je+1 enters the Poisson rhs of row je
-      sbv = a.srcb_vof[L::ci(c, wrap_r(s, ny), nx)];
-      if (s >= jb) sby = a.srcb_uy[L::ci(c, wrap_r(s, ny), nx)];
-      if (outrow && c >= 2) sbx = a.srcb_ux[L::fx(c, s, nx)];
+      const int sw = wrap_r(s, ny);
+      sbv = ci_row(a.srcb_vof, sw)[lane];
+      if (s >= jb) sby = ci_row(a.srcb_uy, sw)[lane];
+      if (outrow && c >= 2) sbx = fx_row(a.srcb_ux, s)[lane];
     }
     if (s + 1 <= je + 1) {
       nv = ld_vof(s + 3);
@@ -143,7 +161,7 @@ __global__ __launch_bounds__(64) void k_march(MarchArgs a) {
       np = ld_cc(a.p, s + 1);
       nps = ld_cc(a.psi0, s + 1);
     }
-    // -- corner gradients, row s+2 ---------------------------------------------
+    // -- corner gradients, row s+2 (curvature_tensor.c:16-49) ----------------
     g0x = g1x; g0y = g1y; g1x = g2x; g1y = g2y;
     g0xr = g1xr; g0yr = g1yr;
     {
@@ -163,7 +181,7 @@ __global__ __launch_bounds__(64) void k_march(MarchArgs a) {
     const double g2xr = from_right(g2x), g2yr = from_right(g2y);
     g1xr = from_right(g1x);
     g1yr = from_right(g1y);
-    // -- normals, row s+1 -------------------------------------------------------
+    // -- normals, row s+1 (curvature_tensor.c:51-97) ---------------------------
     n0x = n1x; n0y = n1y; n0z = n1z;
     {
       double nxv = 0., nyv = 0., nz = 0.;
@@ -182,7 +200,7 @@ __global__ __launch_bounds__(64) void k_march(MarchArgs a) {
       n1y = nyv;
       n1z = nz;
     }
-    // -- curvature, row s -----------------------------------------------------
+    // -- curvature, row s (curvature_tensor.c:100-135) -------------------------
     km1 = k0;
     {
       double kk = 0.;
@@ -196,28 +214,28 @@ __global__ __launch_bounds__(64) void k_march(MarchArgs a) {
       }
       k0 = kk;
     }
-    // -- x fluxes at face c, row s ----------------------------------------------
+    // -- THINC and mass fluxes at x face c, row s (fluxx.c, mass_flux.c) -------
     mfxm1 = mfx0;
+    double fx0 = 0.;
     {
       const double vl = from_left(v0);
       const double nlx = from_left(n0x), nly = from_left(n0y), nlz = from_left(n0z);
-      double f = 0., mf = 0.;
+      double mf = 0.;
       if (c_fx) {
         const double vel = ux0;
         if (c != 1 && c != nx + 1) {
           const bool up = vel < 0.;
           const double v = up ? v0 : vl;
           const V3 n = up ? V3{n0x, n0y, n0z} : V3{nlx, nly, nlz};
-          f = thinc_flux_x(vel, v, in_band(v) ? n : V3{0., 0., 0.});
+          fx0 = thinc_flux_x(vel, v, in_band(v) ? n : V3{0., 0., 0.});
         }
-        mf = vel + (a.denr - 1.) * f;
+        mf = vel + (a.denr - 1.) * fx0;
       }
-      fx0 = f;
       mfx0 = mf;
     }
-    // -- y fluxes at face s+1 (between rows s and s+1) ----------------------------
+    // -- THINC and mass fluxes at y face s+1 (fluxy.c, mass_flux.c) ------------
     fy0 = fyp1;
-    mfym1 = mfy0; mfy0 = mfyp1;
+    mfy0 = mfyp1;
     {
       double f = 0., mf = 0.;
       if (c_vof) {
@@ -233,45 +251,39 @@ __global__ __launch_bounds__(64) void k_march(MarchArgs a) {
       fyp1 = f;
       mfyp1 = mf;
     }
-    // -- stress tensor ----------------------------------------------------------
+    // -- viscosities and stress tensor (viscosity.c, predict/txx.c, tyy.c, txy.c)
     // (every DPP shift runs in uniform control flow: an inactive source lane
     // would not deliver its value)
-    const double v0l = from_left(v0), v0r = from_right(v0), vp1l = from_left(vp1);
-    const double ux0l = from_left(ux0), ux0r = from_right(ux0);
+    const double v0l = from_left(v0), vp1l = from_left(vp1);
+    const double ux0r = from_right(ux0);
     const double uyp1l = from_left(uyp1);
-    {
-      double t = 0.;
-      if (c_in) {
-        const double vx = 0.5 * VISUX(c, v0l, v0) + 0.5 * VISUX(c + 1, v0, v0r);
-        const double l = TB(T_RHC) * (-ux0 + ux0r);
-        t = vx * l + vx * l;
-      }
-      txx0 = t;
+    visx0 = visx1;
+    visx1 = VISUX(c, vp1l, vp1);     // x face c, row s+1
+    visy0 = visy1;
+    visy1 = VISUY(v0, vp1);          // y face s+1
+    const double visx0r = from_right(visx0), visy1l = from_left(visy1);
+    double txx0 = 0.;
+    if (c_in) {
+      const double vx = 0.5 * visx0 + 0.5 * visx0r;
+      const double l = TB(T_RHC) * (-ux0 + ux0r);
+      txx0 = vx * l + vx * l;
     }
-    tyym1 = tyy0;
-    {
-      double t = 0.;
-      if (c_in) {
-        const double vy = 0.5 * VISUY(vm1, v0) + 0.5 * VISUY(v0, vp1);
-        const double l = rhy * (-uy0 + uyp1);
-        t = vy * l + vy * l;
-      }
-      tyy0 = t;
+    double tyy0 = 0.;
+    if (c_in) {
+      const double vy = 0.5 * visy0 + 0.5 * visy1;
+      const double l = rhy * (-uy0 + uyp1);
+      tyy0 = vy * l + vy * l;
     }
-    txy0 = txy1;
-    {
-      double t = 0.;
-      if (c_fx) {
-        const double vx = 0.5 * VISUX(c, v0l, v0) + 0.5 * VISUX(c, vp1l, vp1);
-        const double vy = 0.5 * VISUY(v0l, vp1l) + 0.5 * VISUY(v0, vp1);
-        const double lxy = rhy * (-ux0 + uxp1);
-        const double lyx = TB(T_RHF) * (-uyp1l + uyp1);
-        t = vx * lxy + vy * lyx;
-      }
-      txy1 = t;
+    const double txy_prev = txy0;   // corner row s
+    double txy1 = 0.;
+    if (c_fx) {
+      const double vx = 0.5 * visx0 + 0.5 * visx1;
+      const double vy = 0.5 * visy1l + 0.5 * visy1;
+      const double lxy = rhy * (-ux0 + uxp1);
+      const double lyx = TB(T_RHF) * (-uyp1l + uyp1);
+      txy1 = vx * lxy + vy * lyx;
     }
-    // -- new vof, row s ---------------------------------------------------------
-    vnm2 = vnm1; vnm1 = vn0;
+    // -- new vof, row s (update/main.c:54-120) ---------------------------------
     double src = 0.;
     const double fx0r = from_right(fx0);
     {
@@ -284,132 +296,142 @@ __global__ __launch_bounds__(64) void k_march(MarchArgs a) {
       }
       vn0 = v;
     }
-    // -- predicted velocities, row s ------------------------------------------
-    const double mfx0l = from_left(mfx0), mfx0r = from_right(mfx0);
-    const double mfy0l = from_left(mfy0), mfyp1l = from_left(mfyp1);
-    const double txx0l = from_left(txx0), p0l = from_left(p0), k0l = from_left(k0);
-    const double vn0l = from_left(vn0);
-    const double mfxm1r = from_right(mfxm1);
-    const double uy0l = from_left(uy0), uy0r = from_right(uy0), txy0r = from_right(txy0);
+    // -- densities (density.c) and their face averages -------------------------
+    const double d_m1 = d_0;
+    d_0 = DEN(v0);
+    dnm1 = dn0;
+    dn0 = DEN(vn0);
+    const double d_0l = from_left(d_0), dn0l = from_left(dn0);
+    const double dfx = 0.5 * d_0l + 0.5 * d_0;    // x face c, row s, old vof
+    const double dfy = 0.5 * d_m1 + 0.5 * d_0;    // y face s, old vof
+    const double d1x = 0.5 * dn0l + 0.5 * dn0;    // x face c, row s, new vof
+    d1y = 0.5 * dnm1 + 0.5 * dn0;                 // y face s, new vof
+    // -- momentum fluxes shared between neighbouring faces ---------------------
+    const double mfx0r = from_right(mfx0), mfyp1l = from_left(mfyp1);
+    const double uy0l = from_left(uy0);
+    // ux, x direction: cell-centre flux (mp * up) of the cell right of face c
+    const double qx = (TB(T_EQF) * mfx0 + TR(T_EQF) * mfx0r) * (0.5 * ux0 + 0.5 * ux0r);
+    // uy, x direction: corner flux (mm * um) at face c
+    const double qm = (TB(T_EQF) * mfxm1 + TB(T_EQF) * mfx0) * (0.5 * uy0l + 0.5 * uy0);
+    const double qxl = from_left(qx), qmr = from_right(qm);
+    const double qt = TB(T_QC) * txx0;
+    const double qtl = from_left(qt);
+    const double qf_txy0 = TB(T_QF) * txy_prev;
+    const double qf_txy0r = from_right(qf_txy0);
+    const double p0l = from_left(p0), k0l = from_left(k0);
+    // ux, y direction: corner flux (mp * up) at face s+1
+    const double qy_ux_new = (TL(T_ECY) * mfyp1l + TB(T_ECY) * mfyp1) * (0.5 * ux0 + 0.5 * uxp1);
+    // uy, y direction: cell-centre flux (mp * up) of row s
+    const double qy_uy_new = (TB(T_ECY) * mfy0 + TB(T_ECY) * mfyp1) * (0.5 * uy0 + 0.5 * uyp1);
+    const double cyf_txy1 = TB(T_CYF) * txy1;
+    const double cyc_tyy0 = TB(T_CYC) * tyy0;
+    // -- predicted ux, row s (predict/ux.c, force.c) ---------------------------
     double uxs = 0.;
     if (c >= 2 && c <= nx) {
       double s_ = 0.;
-      {
-        const double mm = TL(T_EQF) * mfx0l + TB(T_EQF) * mfx0;
-        const double mp = TB(T_EQF) * mfx0 + TR(T_EQF) * mfx0r;
-        const double um = 0.5 * ux0l + 0.5 * ux0;
-        const double up = 0.5 * ux0 + 0.5 * ux0r;
-        s_ -= TB(T_RJF) * (-mm * um + mp * up);
-      }
-      {
-        const double mm = TL(T_ECY) * mfy0l + TB(T_ECY) * mfy0;
-        const double mp = TL(T_ECY) * mfyp1l + TB(T_ECY) * mfyp1;
-        const double um = 0.5 * uxm1 + 0.5 * ux0;
-        const double up = 0.5 * ux0 + 0.5 * uxp1;
-        s_ -= TB(T_RJF) * (-mm * um + mp * up);
-      }
-      s_ += TB(T_DFX) * (-TL(T_QC) * txx0l + TB(T_QC) * txx0);
-      s_ += TB(T_DFX) * (-TB(T_CYF) * txy0 + TB(T_CYF) * txy1);
+      s_ -= TB(T_RJF) * (-qxl + qx);
+      s_ -= TB(T_RJF) * (-qy_ux + qy_ux_new);
+      s_ += TB(T_DFX) * (-qtl + qt);
+      s_ += TB(T_DFX) * (-cyf_txy0 + cyf_txy1);
       double g = 0.;
       g -= TB(T_RHF) * (-p0l + p0);
       {
-        const double d = 0.5 * DEN(v0l) + 0.5 * DEN(v0);
         const double k = 0.5 * k0l + 0.5 * k0;
         const double delta = TB(T_RHF) * (-v0l + v0);
-        s_ += d * a.refdeninv * a.tension * k * delta;
+        s_ += dfx * a.refdeninv * a.tension * k * delta;
       }
-      s_ += a.grav * (0.5 * DEN(v0l) + 0.5 * DEN(v0));
-      if (a.srca_ux && outrow && c_out) a.srca_ux[L::fx(c, s, nx)] = s_;
-      const double d0 = 0.5 * DEN(v0l) + 0.5 * DEN(v0);
-      uxs = d0 * ux0 + a.ca * dt * s_ + a.cb * dt * sbx + a.cg * dt * g;
-      const double d1 = 0.5 * DEN(vn0l) + 0.5 * DEN(vn0);
-      uxs /= d1;
+      s_ += a.grav * dfx;
+      if (a.srca_ux && outrow && c_out) fx_row(a.srca_ux, s)[lane] = s_;
+      uxs = dfx * ux0 + a.ca * dt * s_ + a.cb * dt * sbx + a.cg * dt * g;
+      uxs /= d1x;
     }
+    // -- predicted uy, row s (predict/uy.c, force.c) ---------------------------
     double uys = 0.;
     if (c_in) {
       double s_ = 0.;
-      {
-        const double mm = TB(T_EQF) * mfxm1 + TB(T_EQF) * mfx0;
-        const double mp = TR(T_EQF) * mfxm1r + TR(T_EQF) * mfx0r;
-        const double um = 0.5 * uy0l + 0.5 * uy0;
-        const double up = 0.5 * uy0 + 0.5 * uy0r;
-        s_ -= TB(T_RJC) * (-mm * um + mp * up);
-      }
-      {
-        const double mm = TB(T_ECY) * mfym1 + TB(T_ECY) * mfy0;
-        const double mp = TB(T_ECY) * mfy0 + TB(T_ECY) * mfyp1;
-        const double um = 0.5 * uym1 + 0.5 * uy0;
-        const double up = 0.5 * uy0 + 0.5 * uyp1;
-        s_ -= TB(T_RJC) * (-mm * um + mp * up);
-      }
-      s_ += TB(T_DCX) * (-TB(T_QF) * txy0 + TR(T_QF) * txy0r);
-      s_ += TB(T_DCX) * (-TB(T_CYC) * tyym1 + TB(T_CYC) * tyy0);
+      s_ -= TB(T_RJC) * (-qm + qmr);
+      s_ -= TB(T_RJC) * (-qy_uy + qy_uy_new);
+      s_ += TB(T_DCX) * (-qf_txy0 + qf_txy0r);
+      s_ += TB(T_DCX) * (-cyc_tyym1 + cyc_tyy0);
       double g = 0.;
       g -= rhy * (-pm1 + p0);
       {
-        const double d = 0.5 * DEN(vm1) + 0.5 * DEN(v0);
         const double k = 0.5 * km1 + 0.5 * k0;
         const double delta = rhy * (-vm1 + v0);
-        s_ += d * a.refdeninv * a.tension * k * delta;
+        s_ += dfy * a.refdeninv * a.tension * k * delta;
       }
-      if (a.srca_uy && outrow && c_out) a.srca_uy[L::ci(c, s, nx)] = s_;
-      const double d0 = 0.5 * DEN(vm1) + 0.5 * DEN(v0);
-      uys = d0 * uy0 + a.ca * dt * s_ + a.cb * dt * sby + a.cg * dt * g;
-      const double d1 = 0.5 * DEN(vnm1) + 0.5 * DEN(vn0);
-      uys /= d1;
+      if (a.srca_uy && outrow && c_out) ci_row(a.srca_uy, s)[lane] = s_;
+      uys = dfy * uy0 + a.ca * dt * s_ + a.cb * dt * sby + a.cg * dt * g;
+      uys /= d1y;
     }
-    uxs_m1 = uxs0; uxs0 = uxs;
-    uys_m1 = uys0; uys0 = uys;
+    qy_ux = qy_ux_new;
+    qy_uy = qy_uy_new;
+    cyf_txy0 = cyf_txy1;
+    cyc_tyym1 = cyc_tyy0;
+    txy0 = txy1;
     // -- outputs of row s -------------------------------------------------------
+    const double txy0r = from_right(txy_prev);
     if (outrow && c_out) {
-      auto ivof = [nx](int p, int q) { return L::vof(p, q, nx); };
-      auto icc = [nx](int p, int q) { return L::cc(p, q, nx); };
-      auto ixf = [nx](int p, int q) { return L::xf(p, q, nx); };
-      store_with_halo(a.vofn, c, s, ny, 2, vn0, ivof);
-      if (c == 1) store_with_halo(a.vofn, 0, s, ny, 2, 0., ivof);
-      if (c == nx) store_with_halo(a.vofn, nx + 1, s, ny, 2, 0., ivof);
-      if (a.srca_vof) a.srca_vof[L::ci(c, s, nx)] = src;
-      store_with_halo(a.uxs, c, s, ny, 1, c == 1 ? 0. : uxs, ixf);
-      if (c == nx) store_with_halo(a.uxs, nx + 1, s, ny, 1, 0., ixf);
-      store_with_halo(a.uys, c, s, ny, 1, uys, icc);
-      if (c == 1) store_with_halo(a.uys, 0, s, ny, 1, kUyXm, icc);
-      if (c == nx) store_with_halo(a.uys, nx + 1, s, ny, 1, kUyXp, icc);
+      auto put = [&](double* f, int nh, auto row, double v) {
+        row(f, s)[lane] = v;
+        if (s > ny - nh) row(f, s - ny)[lane] = v;
+        if (s <= nh) row(f, s + ny)[lane] = v;
+      };
+      auto put_at = [&](double* f, int nh, auto row, int off, double v) {
+        row(f, s)[lane + off] = v;
+        if (s > ny - nh) row(f, s - ny)[lane + off] = v;
+        if (s <= nh) row(f, s + ny)[lane + off] = v;
+      };
+      auto rv = [&](double* f, int j) { return vof_row(f, j); };
+      auto rc = [&](double* f, int j) { return cc_row(f, j); };
+      auto rx = [&](double* f, int j) { return xf_row(f, j); };
+      put(a.vofn, 2, rv, vn0);
+      if (c == 1) put_at(a.vofn, 2, rv, -1, 0.);
+      if (c == nx) put_at(a.vofn, 2, rv, 1, 0.);
+      if (a.srca_vof) ci_row(a.srca_vof, s)[lane] = src;
+      put(a.uxs, 1, rx, c == 1 ? 0. : uxs);
+      if (c == nx) put_at(a.uxs, 1, rx, 1, 0.);
+      put(a.uys, 1, rc, uys);
+      if (c == 1) put_at(a.uys, 1, rc, -1, kUyXm);
+      if (c == nx) put_at(a.uys, 1, rc, 1, kUyXp);
       if (a.den1) {
         const double dg = DEN(0.);
-        store_with_halo(a.den1, c, s, ny, 1, DEN(vn0), icc);
-        if (c == 1) store_with_halo(a.den1, 0, s, ny, 1, dg, icc);
-        if (c == nx) store_with_halo(a.den1, nx + 1, s, ny, 1, dg, icc);
+        put(a.den1, 1, rc, dn0);
+        if (c == 1) put_at(a.den1, 1, rc, -1, dg);
+        if (c == nx) put_at(a.den1, 1, rc, 1, dg);
       }
       if (a.txy) {
-        store_with_halo(a.txy, c, s, ny, 1, txy0, ixf);
-        if (c == nx) store_with_halo(a.txy, nx + 1, s, ny, 1, txy0r, ixf);
+        put(a.txy, 1, rx, txy_prev);
+        if (c == nx) put_at(a.txy, 1, rx, 1, txy0r);
       }
     }
     // -- Poisson right-hand side of row s-1 (compute_potential.c:315-360) ----
     {
-      const double uxsr = from_right(uxs_m1);
-      const double vnl = from_left(vnm1), vnr = from_right(vnm1);
-      const double psl = from_left(psm1), psr = from_right(psm1);
+      // u* divergence: the x-face terms of cell c+1 come from lane c+1
+      const double q_uxs = q_uxs_m1;                       // QF * ux* at face c, row s-1
+      const double q_uxsr = from_right(q_uxs);
+      const double cyc_uys0 = TB(T_CYC) * uys;
+      // explicit split-pressure term: the gradient at face c+1 is lane c+1's
+      // gradient at its left face; the top y face of row s-1 is the bottom
+      // face of row s, kept for the next step
+      const double psl = from_left(psm1);
+      const double dxm = d1x_m1;
+      const double gxm = (a.refden / dxm - 1.) * (-psl + psm1) / TB(T_HXXF);
+      const double qgx = TB(T_QF) * gxm;
+      const double qgxr = from_right(qgx);
+      const double gyp = (a.refden / d1y - 1.) * (-psm1 + ps0) / hy;
+      const double cg_yp = TB(T_CYC) * gyp;
       const int r = s - 1;
       if (r >= jb && r <= je && c_out) {
-        const double div = TB(T_RJC) * (-TB(T_QF) * uxs_m1 + TR(T_QF) * uxsr - TB(T_CYC) * uys_m1 + TB(T_CYC) * uys0);
-        const double d00 = DEN(vnm1);
-        const double dxm = 0.5 * DEN(vnl) + 0.5 * d00;
-        const double dxp = 0.5 * d00 + 0.5 * DEN(vnr);
-        const double dym = 0.5 * DEN(vnm2) + 0.5 * d00;
-        const double dyp = 0.5 * d00 + 0.5 * DEN(vn0);
-        const double pxm = -psl + psm1;
-        const double pxp = -psm1 + psr;
-        const double pym = -psm2 + psm1;
-        const double pyp = -psm1 + ps0;
-        const double gxm = (a.refden / dxm - 1.) * pxm / TB(T_HXXF);
-        const double gxp = (a.refden / dxp - 1.) * pxp / TR(T_HXXF);
-        const double gym = (a.refden / dym - 1.) * pym / hy;
-        const double gyp = (a.refden / dyp - 1.) * pyp / hy;
-        const double add = TB(T_RJC) * (-TB(T_QF) * gxm + TR(T_QF) * gxp - TB(T_CYC) * gym + TB(T_CYC) * gyp);
-        a.rhs[(size_t)(r - 1) * nx + (c - 1)] = c0 * div + c1 * add;
+        const double div = TB(T_RJC) * (-q_uxs + q_uxsr - cyc_uys_m1 + cyc_uys0);
+        const double add = TB(T_RJC) * (-qgx + qgxr - cg_ym + cg_yp);
+        ci_row(a.rhs, r)[lane] = c0 * div + c1 * add;
       }
+      cg_ym = cg_yp;
+      cyc_uys_m1 = cyc_uys0;
     }
+    q_uxs_m1 = TB(T_QF) * uxs;
+    d1x_m1 = d1x;
   }
 }
 
