@@ -85,6 +85,39 @@ extern int sbfs_diagnostics(const domain_t * domain, const fluid_t * fluid,
                             const interface_t * interface,
                             double out[SBFS_NDIAG]);
 
+/* every quantity the reference's logging writes (one file per quantity
+ * under src/logging), reduced on the device in FP64 in a fixed order, plus
+ * the bubble's volume, centroid and rise velocity (which the reference does
+ * not log).  Uses
+ * den[1] and txy as the last Runge-Kutta stage of integrate() leaves them. */
+#define SBFS_LOG_DIVMAX     0  /* max |div u|               divergence.c:41-60 */
+#define SBFS_LOG_MOMX       1  /* total x momentum          momentum.c:31-40   */
+#define SBFS_LOG_MOMY       2  /* total y momentum          momentum.c:42-50   */
+#define SBFS_LOG_EXTX       3  /* max |ux|                  extrema.c:25-31    */
+#define SBFS_LOG_EXTY       4  /* max |uy|                  extrema.c:33-38    */
+#define SBFS_LOG_ENGX       5  /* x kinetic energy          energy.c:31-40     */
+#define SBFS_LOG_ENGY       6  /* y kinetic energy          energy.c:42-50     */
+#define SBFS_LOG_STRESS_XM  7  /* wall shear stress, x = 0  stress.c:28-36     */
+#define SBFS_LOG_STRESS_XP  8  /* wall shear stress, x = lx stress.c:28-36     */
+#define SBFS_LOG_VOFMIN     9  /* min vof                   interface.c:104-117 */
+#define SBFS_LOG_VOFMAX    10  /* max vof                   interface.c:104-117 */
+#define SBFS_LOG_VOFMEAN   11  /* volume-weighted mean vof  interface.c:104-117 */
+#define SBFS_LOG_AREA      12  /* interface area            interface.c:119-137 */
+#define SBFS_LOG_VOLUME    13  /* gas volume sum vof J                          */
+#define SBFS_LOG_CENTROID  14  /* gas centroid in x                             */
+#define SBFS_LOG_RISE      15  /* gas-volume mean of ux (+x is up)              */
+#define SBFS_NLOG 16
+extern int sbfs_logging_values(const domain_t * domain, const fluid_t * fluid,
+                               const interface_t * interface,
+                               double out[SBFS_NLOG]);
+
+/* step counter and time of an NPY directory (step.npy '<u8', time.npy
+ * '<f8', both 0-d), as src/main.c:63-70 reads and :28-29 writes them */
+extern int sbfs_read_step_time(const char dirname[], size_t * step,
+                               double * time);
+extern int sbfs_write_step_time(const char dirname[], size_t step,
+                                double time);
+
 /* kernel timing: while on, every launch is bracketed by HIP events on the
  * library stream; stop() synchronises and aggregates per kernel name */
 extern int sbfs_profile_start(void);

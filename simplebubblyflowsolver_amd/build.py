@@ -3,7 +3,9 @@
     python -m simplebubblyflowsolver_amd.build [--jobs N] [--verbose]
 
 Every .hip file under csrc/ is compiled to an object under _build/ and
-linked into simplebubblyflowsolver_amd/libsbfs.so.  Flags:
+linked into simplebubblyflowsolver_amd/libsbfs.so; the command-line driver
+driver/main.c (C, gcc) is linked against it as simplebubblyflowsolver_amd/
+sbfs_run.  Flags:
   -ffp-contract=off   no a*b+c contraction: the FP64 arithmetic keeps the
                       reference's rounding (bitwise parity with the oracle)
   -Xclang -target-feature -Xclang -packed-fp32-ops
@@ -24,6 +26,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = ROOT / "_build"
 LIB = PKG / "libsbfs.so"
+DRIVER = PKG / "sbfs_run"
 ARCH = os.environ.get("SBFS_OFFLOAD_ARCH", "gfx950")
 
 
@@ -66,7 +69,22 @@ def build(jobs: int = 8, verbose: bool = False) -> Path:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_driver(verbose)
     return LIB
+
+
+def build_driver(verbose: bool = False) -> Path:
+    """gcc the C driver against libsbfs.so; it finds the library next to
+    itself ($ORIGIN), so the pair can move together."""
+    cc = os.environ.get("CC", "gcc")
+    tmp = DRIVER.with_name(DRIVER.name + ".tmp")
+    cmd = [cc, "-std=c99", "-O2", "-Wall", "-Wextra", f"-I{ROOT / 'include'}", str(PKG / "driver" / "main.c"),
+           f"-L{PKG}", "-lsbfs", "-Wl,-rpath,$ORIGIN", "-lm", "-o", str(tmp)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, DRIVER)
+    return DRIVER
 
 
 def main() -> None:

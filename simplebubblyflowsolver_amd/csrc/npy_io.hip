@@ -232,4 +232,16 @@ int interface_save(const char dirname[], const domain_t* d, const interface_t* s
   return dump_field(d, dirname, "vof", &s->vof);
 }
 
+int sbfs_read_step_time(const char dirname[], size_t* step, double* time) {
+  SBFS_REQUIRE(dirname && step && time, "sbfs_read_step_time: null argument");
+  if (npy_read(dirname, "step", "<u8", {}, sizeof(size_t), step)) return 1;
+  return npy_read(dirname, "time", "<f8", {}, sizeof(double), time);
+}
+
+int sbfs_write_step_time(const char dirname[], size_t step, double time) {
+  SBFS_REQUIRE(dirname, "sbfs_write_step_time: null argument");
+  if (npy_write(dirname, "step", "<u8", {}, sizeof(size_t), &step)) return 1;
+  return npy_write(dirname, "time", "<f8", {}, sizeof(double), &time);
+}
+
 }  // extern "C"

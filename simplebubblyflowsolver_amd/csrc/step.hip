@@ -8,6 +8,7 @@
 #include "integrate.h"
 #include "interface_solver.h"
 #include "sbfs.h"
+#include "reduce.hpp"
 #include "sbfs_internal.hpp"
 
 namespace sbfs {
@@ -15,36 +16,6 @@ namespace sbfs {
 static const double kPiDt = 3.1415926535897932384626433832;  // decide_dt.c:16
 static const double kDtMax = 1.;                              // decide_dt.c:25
 
-constexpr int kRedThreads = 256;
-
-__device__ __forceinline__ double wave_min(double v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-  return v;
-}
-
-__device__ __forceinline__ double wave_max(double v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  return v;
-}
-
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-// block-wide reduction of kRedThreads values (4 wavefronts), fixed order
-template <int OP>
-__device__ double block_reduce(double v, double* sh) {
-  v = OP == 0 ? wave_min(v) : OP == 1 ? wave_max(v) : wave_sum(v);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) sh[w] = v;
-  __syncthreads();
-  double r = sh[0];
-  for (int k = 1; k < kRedThreads / 64; k++)
-    r = OP == 0 ? fmin(r, sh[k]) : OP == 1 ? fmax(r, sh[k]) : r + sh[k];
-  return r;
-}
 
 // advective limit min over faces of h / (|u| + small) (decide_dt.c:49-64)
 __global__ void k_dt_partial(const double* __restrict__ ux, const double* __restrict__ uy,

@@ -87,3 +87,56 @@ def poisson_disk_bubbles(n: int, nx: int, ny: int, lx: float, ly: float,
         if ok:
             out.append((cx, cy, r))
     return out
+
+
+def write_ic_dir(dest, nx: int, ny: int, lx: float, ly: float, vof: np.ndarray,
+                 ux: np.ndarray | None = None, uy: np.ndarray | None = None,
+                 p: np.ndarray | None = None, psi: np.ndarray | None = None,
+                 step: int = 0, time: float = 0.0) -> None:
+    """Write an initial-condition directory in the layout the reference's
+    generator writes (initial_condition/main.py:6-61): step.npy ('<u8', 0-d),
+    time.npy, glsizes.npy ('<u8'), lengths.npy, xf.npy, xc.npy, and the
+    fields ux (Ny, Nx+1), uy / p / psi / vof (Ny, Nx+2); absent velocity,
+    pressure and potential fields are zero."""
+    import os
+    os.makedirs(dest, exist_ok=True)
+    xf, xc, _ = grid(nx, ny, lx, ly)
+    np.save(f"{dest}/step.npy", np.array(step, dtype=np.uint64))
+    np.save(f"{dest}/time.npy", np.array(time, dtype=np.float64))
+    np.save(f"{dest}/glsizes.npy", np.array([nx, ny], dtype=np.uint64))
+    np.save(f"{dest}/lengths.npy", np.array([lx, ly], dtype=np.float64))
+    np.save(f"{dest}/xf.npy", np.asarray(xf, dtype=np.float64))
+    np.save(f"{dest}/xc.npy", np.asarray(xc, dtype=np.float64))
+    s0, s1 = (ny, nx + 1), (ny, nx + 2)
+    for name, arr, shape in (("ux", ux, s0), ("uy", uy, s1), ("p", p, s1), ("psi", psi, s1),
+                             ("vof", vof, s1)):
+        a = np.zeros(shape) if arr is None else np.ascontiguousarray(arr, dtype=np.float64)
+        if a.shape != shape:
+            raise ValueError(f"{name}: shape {a.shape} != {shape}")
+        np.save(f"{dest}/{name}.npy", a)
+
+
+def main(argv=None) -> int:
+    """python -m simplebubblyflowsolver_amd.ic DEST [--workload rising|swarm|large]
+    [--nx N --ny N]: write the IC directory of a named workload."""
+    import argparse
+    from . import configs
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("dest")
+    ap.add_argument("--workload", default="rising", choices=["rising", "swarm", "large"])
+    ap.add_argument("--nx", type=int)
+    ap.add_argument("--ny", type=int)
+    a = ap.parse_args(argv)
+    if a.workload == "rising":
+        w = configs.rising_bubble(a.nx or 128, a.ny or 256)
+    elif a.workload == "swarm":
+        w = configs.bubble_swarm(a.nx or 2048, a.ny or 4096)
+    else:
+        w = configs.large_swarm()
+    write_ic_dir(a.dest, w.nx, w.ny, w.lx, w.ly, w.vof())
+    print(f"{a.dest}: {w.name}, Re={w.Re} We={w.We} Fr={w.Fr} denr={w.denr} visr={w.visr}")
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -1,0 +1,139 @@
+"""The command-line driver (NPY IC / save directories, logs) and the device
+logging reductions, on the GPU."""
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from simplebubblyflowsolver_amd import abi, build, configs, ic
+
+pytestmark = pytest.mark.gpu
+
+ENV = dict(timemax="1e9", wtimemax="1e9", log_rate="1e-12", save_rate="1e9", save_after="0",
+           coef_dt_adv="0.20", coef_dt_dif="0.50", coef_dt_int="0.95",
+           Re="35", We="10", Fr="1", denr="1e-3", visr="1e-2")
+
+
+def run_driver(ic_dir, out, steps, cwd):
+    exe = build.DRIVER
+    assert exe.exists(), "build the library first (it also builds sbfs_run)"
+    env = dict(os.environ)
+    env.update(ENV)
+    print(f"driver: {ic_dir} -> {out}, {steps} steps", flush=True)
+    r = subprocess.run([str(exe), str(ic_dir), "--steps", str(steps), "--output", str(out)],
+                       cwd=cwd, env=env, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def load_dir(d):
+    d = Path(d)
+    return {n: np.load(d / f"{n}.npy") for n in ("step", "time", "ux", "uy", "p", "psi", "vof")}
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def test_driver_matches_oracle_and_resumes_bitwise(oracle_mod, tmp_path):
+    """A 32 x 64 rising bubble from an IC directory: 6 steps of sbfs_run
+    agree with 6 oracle steps within 1e-10 (p, psi up to their mean); 3
+    steps, save, resume for 3 more gives the same save directory bit for
+    bit; the logs hold one line per logged step in the reference's formats."""
+    w = configs.rising_bubble(32, 64)
+    icd = tmp_path / "ic"
+    ic.write_ic_dir(icd, w.nx, w.ny, w.lx, w.ly, w.vof())
+    run_driver(icd, tmp_path / "a", 6, tmp_path)
+    run_driver(icd, tmp_path / "b", 3, tmp_path)
+    run_driver(tmp_path / "b/save/step0000000003", tmp_path / "c", 3, tmp_path)
+    a = load_dir(tmp_path / "a/save/step0000000006")
+    c = load_dir(tmp_path / "c/save/step0000000006")
+    for n in a:
+        assert np.array_equal(a[n], c[n]), n
+    assert int(a["step"]) == 6
+    # oracle from the same IC
+    oracle_mod.set_poisson(dct=oracle_mod.DCT_FFT, tdm=oracle_mod.TDM_SEQUENTIAL)
+    o = oracle_mod.new_case(w.nx, w.ny, w.lx, w.ly, w.Re, w.Fr, w.denr, w.visr, w.We)
+    try:
+        o.load_initial_condition(np.load(icd / "vof.npy"))
+        t = 0.0
+        for _ in range(6):
+            t += o.integrate()
+        assert float(a["time"]) == pytest.approx(t, rel=1e-12)
+        for n, arr in (("ux", "ux"), ("uy", "uy"), ("vof", "vof")):
+            r0, _ = abi.field_origin(arr)
+            want = o.get(arr)[r0:r0 + w.ny]
+            assert rel(a[n], want) <= 1e-10, n
+        for n, arr in (("p", "p"), ("psi", "psi0")):
+            r0, _ = abi.field_origin(arr)
+            want = o.get(arr)[r0:r0 + w.ny][:, 1:-1]
+            got = a[n][:, 1:-1]
+            assert rel(got - got.mean(), want - want.mean()) <= 1e-10, n
+        vol = o.diagnostics()["volume"]
+    finally:
+        o.close()
+    logs = tmp_path / "a/log"
+    for name, ncol in (("divergence", 2), ("momentum", 3), ("extrema", 3), ("energy", 3),
+                       ("stress", 3), ("vof", 5), ("bubble", 5)):
+        rows = [l.split() for l in (logs / f"{name}.dat").read_text().splitlines()]
+        assert len(rows) == 5 and all(len(r) == ncol for r in rows), name
+    assert all(float(r[1]) < 1e-10 for r in (l.split() for l in (logs / "divergence.dat").read_text().splitlines()))
+    last = (logs / "bubble.dat").read_text().splitlines()[-1].split()
+    assert float(last[2]) == pytest.approx(vol, rel=1e-9)   # volume is conserved step to step
+    prog = (logs / "progress.dat").read_text().splitlines()
+    assert prog[0].startswith("step          1, time ")
+
+
+def test_logging_values_match_host_sums(product):
+    """sbfs_logging_values against the reference's logging formulas
+    evaluated with numpy on the downloaded fields (after 3 steps, so den[1]
+    and txy are those of the last stage)."""
+    from harness import initial_state
+    from simplebubblyflowsolver_amd import solver
+    nx, ny = 64, 32
+    g = solver.Case(product, nx, ny, 1.0, 0.5, 35.0, 1.0, 1e-3, 1e-2, 10.0)
+    try:
+        initial_state(g, seed=3, velocity=0.2, ly=0.5, cy=0.25, r=0.15)
+        for _ in range(3):
+            g.integrate()
+        out = (C.c_double * 16)()
+        product.lib.sbfs_logging_values.argtypes = [C.POINTER(abi.DomainT), C.POINTER(abi.FluidT),
+                                                    C.POINTER(abi.InterfaceT), C.POINTER(C.c_double)]
+        assert product.lib.sbfs_logging_values(C.byref(g.domain), C.byref(g.fluid), C.byref(g.interface), out) == 0
+        q = list(out)
+        ux, uy, den, txy, vof = (g.get(n) for n in ("ux", "uy", "den1", "txy", "vof"))
+        hf, jf, jc, xc = g.metric("hxxf"), g.metric("jdxf"), g.metric("jdxc"), g.metric("xc")
+        hy = g.domain.hy
+        U = lambda i, j: ux[j, i - 1]            # noqa: E731  ux rows j = 0..ny+1, cols i = 1..nx+1
+        V = lambda i, j: uy[j, i]                 # noqa: E731
+        D = lambda i, j: den[j, i]                # noqa: E731
+        F = lambda i, j: vof[j + 1, i]            # noqa: E731
+        J = np.arange(1, ny + 1)[:, None]
+        I2 = np.arange(2, nx + 1)[None, :]
+        I1 = np.arange(1, nx + 1)[None, :]
+        momx = ((0.5 * D(I2 - 1, J) + 0.5 * D(I2, J)) * U(I2, J) * jf[I2 - 1]).sum()
+        momy = ((0.5 * D(I1, J - 1) + 0.5 * D(I1, J)) * V(I1, J) * jc[I1 - 1]).sum()
+        engx = (0.5 * (0.5 * D(I2 - 1, J) + 0.5 * D(I2, J)) * U(I2, J) ** 2 * jf[I2 - 1]).sum()
+        engy = (0.5 * (0.5 * D(I1, J - 1) + 0.5 * D(I1, J)) * V(I1, J) ** 2 * jc[I1 - 1]).sum()
+        strm = (1 / 35.0 * jf[0] / hf[0] * txy[1:ny + 1, 0]).sum()
+        strp = (1 / 35.0 * jf[nx] / hf[nx] * txy[1:ny + 1, nx]).sum()
+        v = F(I1, J)
+        area = (np.abs(F(np.arange(1, nx + 2)[None, :], J) - F(np.arange(0, nx + 1)[None, :], J))
+                * (jf / hf)[None, :]).sum() + (np.abs(v - F(I1, J - 1)) * (jc / hy)[None, :]).sum()
+        div = (1 / jc[I1 - 1]) * (-(jf[I1 - 1] / hf[I1 - 1]) * U(I1, J) + (jf[I1] / hf[I1]) * U(I1 + 1, J)
+                                  - (jc[I1 - 1] / hy) * V(I1, J) + (jc[I1 - 1] / hy) * V(I1, J + 1))
+        vol = (v * jc[None, :]).sum()
+        want = [np.abs(div).max(), momx, momy, np.abs(U(I2, J)).max(), np.abs(V(I1, J)).max(), engx, engy,
+                strm, strp, v.min(), v.max(), vol / (jc.sum() * ny), area, vol,
+                (v * xc[1:nx + 1][None, :] * jc[None, :]).sum() / vol,
+                (v * (0.5 * U(I1, J) + 0.5 * U(I1 + 1, J)) * jc[None, :]).sum() / vol]
+        for k, (got, ref) in enumerate(zip(q, want)):
+            if k == 0:
+                assert got <= 1e-10 and ref <= 1e-10
+            else:
+                assert got == pytest.approx(ref, rel=1e-12, abs=1e-15), k
+    finally:
+        g.close()
