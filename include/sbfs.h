@@ -118,6 +118,16 @@ extern int sbfs_read_step_time(const char dirname[], size_t * step,
 extern int sbfs_write_step_time(const char dirname[], size_t step,
                                 double time);
 
+/* temporal statistics (src/statistics.c): collect() adds ux, uy and vof to
+ * device-resident running sums ux1 ([1:Nx+1] x [1:Ny]), uy1 and vof1
+ * ([0:Nx+1] x [1:Ny]) and counts the samples; save() writes num.npy
+ * ('<u8'), ux1.npy, uy1.npy and vof1.npy into an existing directory, as
+ * statistics.output does (:170-214).  One set of sums per process. */
+extern int sbfs_statistics_collect(const domain_t * domain, const fluid_t * fluid,
+                                   const interface_t * interface);
+extern int sbfs_statistics_num(size_t * num);
+extern int sbfs_statistics_save(const char dirname[], const domain_t * domain);
+
 /* kernel timing: while on, every launch is bracketed by HIP events on the
  * library stream; stop() synchronises and aggregates per kernel name */
 extern int sbfs_profile_start(void);

@@ -94,7 +94,7 @@ static int npy_read(const char* dir, const char* name, const char* descr,
   return 0;
 }
 
-static int npy_write(const char* dir, const char* name, const char* descr,
+int npy_write(const char* dir, const char* name, const char* descr,
                      const std::vector<size_t>& shape, size_t elem, const void* data) {
   std::string sh = "(";
   for (size_t k = 0; k < shape.size(); k++) {

@@ -140,6 +140,11 @@ hipStream_t stream();
 int init_runtime();
 // device context of a host domain (created / refreshed on demand)
 DomainCtx* domain_ctx(const domain_t* d);
+
+// NPY v1 file `dir/name.npy` of `shape` (C order) with dtype `descr`
+// ("<f8", "<u8"); shape {} writes a 0-d array (npy_io.hip)
+int npy_write(const char* dir, const char* name, const char* descr,
+              const std::vector<size_t>& shape, size_t elem, const void* data);
 int load_dt_coefs();
 
 // kernel timing (profiler.hip): events around launches while profiling

@@ -8,8 +8,8 @@
  * lengths.npy, xf.npy, xc.npy, ux.npy, uy.npy, p.npy, psi.npy and vof.npy
  * (initial_condition/main.py:6-61).  Parameters come from the environment
  * exactly as in exec/main.sh: timemax, wtimemax, log_rate, save_rate,
- * save_after, coef_dt_adv, coef_dt_dif, coef_dt_int, Re, We, Fr, denr,
- * visr.  The logs are appended to DIR/log/{progress, divergence, momentum,
+ * save_after, stat_rate, stat_after, coef_dt_adv, coef_dt_dif,
+ * coef_dt_int, Re, We, Fr, denr, visr.  The logs are appended to DIR/log/{progress, divergence, momentum,
  * extrema, energy, stress, vof}.dat in the reference's formats (one
  * file per quantity under src/logging); bubble.dat adds the gas volume,
  * centroid and rise velocity.  Flow fields are saved to DIR/save/stepNNNNNNNNNN
@@ -131,6 +131,19 @@ static int save_entrypoint(const char * outdir, const domain_t * domain, size_t 
   return 0;
 }
 
+/* statistics.output (src/statistics.c:165-214) */
+static int stat_output(const char * outdir, const domain_t * domain, size_t step) {
+  size_t num = 0;
+  if (sbfs_statistics_num(&num) || 0 == num) return 0;
+  char dirname[4096];
+  snprintf(dirname, sizeof(dirname), "%s/stat", outdir);
+  if (make_dir(dirname)) return 1;
+  snprintf(dirname, sizeof(dirname), "%s/stat/step%0*zu", outdir, 10, step);
+  if (make_dir(dirname)) return 1;
+  if (sbfs_statistics_save(dirname, domain)) return 1;
+  return domain_save(dirname, domain);
+}
+
 int main(int argc, char * argv[]) {
   const double tic = wall_time();
   const char * dirname_ic = NULL;
@@ -162,12 +175,14 @@ int main(int argc, char * argv[]) {
   memset(&fluid, 0, sizeof(fluid));
   memset(&interface, 0, sizeof(interface));
   double timemax = 0., wtimemax = 0., log_rate = 0., save_rate = 0., save_after = 0.;
+  double stat_rate = 0., stat_after = 0.;
   if (sbfs_read_step_time(dirname_ic, &step, &time)) goto abort;
   if (domain_init(dirname_ic, &domain)) goto abort;
   if (fluid_init(dirname_ic, &domain, &fluid)) goto abort;
   if (interface_init(dirname_ic, &domain, &interface)) goto abort;
   if (get_double("log_rate", &log_rate) || get_double("save_rate", &save_rate) ||
-      get_double("save_after", &save_after) || get_double("timemax", &timemax) ||
+      get_double("save_after", &save_after) || get_double("stat_rate", &stat_rate) ||
+      get_double("stat_after", &stat_after) || get_double("timemax", &timemax) ||
       get_double("wtimemax", &wtimemax))
     return 1;
   if (make_dir(outdir)) return 1;
@@ -181,12 +196,15 @@ int main(int argc, char * argv[]) {
   /* schedulers (src/logging/main.c:23-25, src/save.c:43-45) */
   double log_next = log_rate * ceil(fmax(DBL_EPSILON, time) / log_rate);
   double save_next = save_rate * ceil(fmax(DBL_EPSILON, fmax(time, save_after)) / save_rate);
+  double stat_next = stat_rate * ceil(fmax(DBL_EPSILON, fmax(time, stat_after)) / stat_rate);
   {
     char name[256] = "";
     sbfs_device_name(name, sizeof(name));
     printf("DEVICE\n\t%s\n", name);
     printf("LOGGING\n\tnext: % .3e\n\trate: % .3e\n", log_next, log_rate);
     printf("SAVE\n\tdest: %s/save/step\n\tnext: % .3e\n\trate: % .3e\n", outdir, save_next, save_rate);
+    printf("STATISTICS\n\tdest: %s/stat/step\n\tnext: % .3e\n\trate: % .3e\n", outdir, stat_next,
+           stat_rate);
     printf("step: %zu, time: % .7e\n", step, time);
     printf("timemax: % .7e, wtimemax: % .7e\n", timemax, wtimemax);
     fflush(stdout);
@@ -210,8 +228,13 @@ int main(int argc, char * argv[]) {
       if (save_entrypoint(outdir, &domain, step, time, &fluid, &interface)) goto abort;
       save_next += save_rate;
     }
+    if (stat_next < time) {
+      if (sbfs_statistics_collect(&domain, &fluid, &interface)) goto abort;
+      stat_next += stat_rate;
+    }
   }
   if (save_entrypoint(outdir, &domain, step, time, &fluid, &interface)) goto abort;
+  if (stat_output(outdir, &domain, step)) goto abort;
   printf("finished: step %zu, time % .7e, elapsed %.2f s\n", step, time, wall_time() - tic);
   return 0;
 abort:

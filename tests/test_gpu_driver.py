@@ -13,6 +13,7 @@ from simplebubblyflowsolver_amd import abi, build, configs, ic
 pytestmark = pytest.mark.gpu
 
 ENV = dict(timemax="1e9", wtimemax="1e9", log_rate="1e-12", save_rate="1e9", save_after="0",
+           stat_rate="1e-12", stat_after="0",
            coef_dt_adv="0.20", coef_dt_dif="0.50", coef_dt_int="0.95",
            Re="35", We="10", Fr="1", denr="1e-3", visr="1e-2")
 
@@ -42,7 +43,8 @@ def test_driver_matches_oracle_and_resumes_bitwise(oracle_mod, tmp_path):
     """A 32 x 64 rising bubble from an IC directory: 6 steps of sbfs_run
     agree with 6 oracle steps within 1e-10 (p, psi up to their mean); 3
     steps, save, resume for 3 more gives the same save directory bit for
-    bit; the logs hold one line per logged step in the reference's formats."""
+    bit; the logs hold one line per logged step in the reference's formats;
+    the statistics hold the sums of ux, uy and vof over the collected steps."""
     w = configs.rising_bubble(32, 64)
     icd = tmp_path / "ic"
     ic.write_ic_dir(icd, w.nx, w.ny, w.lx, w.ly, w.vof())
@@ -60,8 +62,13 @@ def test_driver_matches_oracle_and_resumes_bitwise(oracle_mod, tmp_path):
     try:
         o.load_initial_condition(np.load(icd / "vof.npy"))
         t = 0.0
-        for _ in range(6):
+        sums = {"ux": 0.0, "uy": 0.0, "vof": 0.0}
+        for k in range(6):
             t += o.integrate()
+            if k < 5:  # the last step ends the run before its collection
+                for n in sums:
+                    r0, _ = abi.field_origin(n)
+                    sums[n] = sums[n] + o.get(n)[r0:r0 + w.ny]
         assert float(a["time"]) == pytest.approx(t, rel=1e-12)
         for n, arr in (("ux", "ux"), ("uy", "uy"), ("vof", "vof")):
             r0, _ = abi.field_origin(arr)
@@ -73,6 +80,12 @@ def test_driver_matches_oracle_and_resumes_bitwise(oracle_mod, tmp_path):
             got = a[n][:, 1:-1]
             assert rel(got - got.mean(), want - want.mean()) <= 1e-10, n
         vol = o.diagnostics()["volume"]
+        st = tmp_path / "a/stat/step0000000006"
+        assert int(np.load(st / "num.npy")) == 5
+        assert rel(np.load(st / "ux1.npy"), sums["ux"]) <= 1e-10
+        assert rel(np.load(st / "uy1.npy"), sums["uy"]) <= 1e-10
+        assert rel(np.load(st / "vof1.npy"), sums["vof"]) <= 1e-10
+        assert np.array_equal(np.load(st / "xf.npy"), np.load(icd / "xf.npy"))
     finally:
         o.close()
     logs = tmp_path / "a/log"
