@@ -1197,9 +1197,11 @@ int oracle_tdm_periodic(int n, double l, double c, double u, double * q, int gau
  * L-1 interior rows are solved with zero ends (g), plus the two unit spikes
  * w (driven by the lower separator) and z (upper separator); the separators
  * then satisfy a periodic tridiagonal system of size P = n / L with
- * constant coefficients, solved by the reference algorithm (src/tdm.c).
- * The singular mode pins the last separator (x_{n-1} = 0), the same gauge
- * as the reference's singular branch.
+ * constant coefficients (a circulant), solved as the circular convolution
+ * of its right-hand side with the first column of the inverse, which the
+ * reference algorithm (src/tdm.c) finds from e_0.  The singular mode pins
+ * the last separator (x_{n-1} = 0), the same gauge as the reference's
+ * singular branch.
  */
 typedef struct {
   int L, K, P;
@@ -1264,8 +1266,23 @@ static void part_solve(const part_tables_t * t, double * q, int pin, double * g,
     gf[c] = g[0];
     gl[c] = g[K - 1];
   }
-  for (int c = 0; c < P; c++) X[c] = (q[(size_t)c * L + K] - t->a * gl[c]) - t->a * gf[(c + 1) % P];
-  tdm_periodic_work(P, t->lo, t->di, t->up, X, pin, v, q1);
+  /* reduced right-hand side in gl (its chunk values are no longer needed) */
+  for (int c = 0; c < P; c++) gl[c] = (q[(size_t)c * L + K] - t->a * gl[c]) - t->a * gf[(c + 1) % P];
+  /* The reduced matrix is circulant (every chunk is the same), so its
+   * inverse is the circulant of h = C^-1 e_0, found with the reference
+   * algorithm; the singular mode uses h = C^+ (e_0 - 1/P) with h[P-1] = 0
+   * and then pins X[P-1] = 0.  X = h (*) R, summed in ascending k. */
+  for (int c = 0; c < P; c++) gf[c] = pin ? (c == 0 ? 1. - 1. / (double)P : -1. / (double)P) : (c == 0 ? 1. : 0.);
+  tdm_periodic_work(P, t->lo, t->di, t->up, gf, pin, v, q1);
+  for (int i = 0; i < P; i++) {
+    double acc = 0.;
+    for (int k = 0; k < P; k++) acc += gf[(i - k + P) % P] * gl[k];
+    X[i] = acc;
+  }
+  if (pin) {
+    const double xl = X[P - 1];
+    for (int i = 0; i < P; i++) X[i] = X[i] - xl;
+  }
   for (int c = 0; c < P; c++) {
     part_thomas(t, q + (size_t)c * L, g);
     const double xl = X[(c + P - 1) % P], xu = X[c];

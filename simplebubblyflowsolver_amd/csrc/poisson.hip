@@ -39,10 +39,9 @@ void poisson_destroy(PoissonPlan* p) {
   (void)hipFree(p->t1);
   (void)hipFree(p->t2);
   (void)hipFree(p->ptab);
-  (void)hipFree(p->pred);
   (void)hipFree(p->gfl);
   (void)hipFree(p->X);
-  (void)hipFree(p->rtab);
+  (void)hipFree(p->htab);
   (void)hipFree(p->rhs);
   delete p;
 }

@@ -22,9 +22,7 @@ struct PoissonPlan {
   int L = 0, P = 0;
   double a = 0.;             // off-diagonal 1/hy^2
   double* ptab = nullptr;    // [4][K][nx]: inv, v, w, z per (row-in-chunk, mode)
-  double* pred = nullptr;    // [6][nx]: reduced lower / diagonal / upper,
-                             //   1/(last pivot), last-pivot flag, closure denominator
-  double* rtab = nullptr;    // [3][P][nx]: reduced-solver pivots, factors, spike
+  double* htab = nullptr;    // [P][nx]: first column of the reduced circulant's inverse
   double* gfl = nullptr;     // [2P][nx]: first / last interior of each chunk
   double* X = nullptr;       // [P][nx]: separators
 };

@@ -6,10 +6,11 @@
 // whose last row is a separator:
 //   T1  thread (mode, chunk): Thomas on the L-1 interior rows with zero ends
 //       -> first / last interior value                       (reads r once)
-//   T2  thread per mode: the separators satisfy a periodic tridiagonal
-//       system of size P with constant coefficients, solved by the
-//       reference algorithm; the singular mode 0 pins the last separator,
-//       i.e. x_{Ny-1} = 0, the reference's singular branch (src/tdm.c:94)
+//   T2  the separators satisfy a periodic tridiagonal system of size P
+//       with constant coefficients (a circulant), solved in parallel as a
+//       circular convolution with the first column of its inverse; the
+//       singular mode 0 pins the last separator, i.e. x_{Ny-1} = 0, the
+//       reference's singular branch (src/tdm.c:94)
 //   T3  thread (mode, chunk): recompute the interior solution and add the
 //       two unit spikes weighted by the bounding separators (writes x)
 // The chunk matrices are identical for all chunks of a mode, so the Thomas
@@ -96,76 +97,70 @@ __global__ void k_part_reduce(const double* __restrict__ q, const double* __rest
   gfl[(size_t)(2 * ch + 1) * nx + mo] = g[K - 1];
 }
 
-// Separators of every mode: assemble the reduced right-hand side and solve
-// the periodic reduced system with the reference algorithm.  Its pivots,
-// factors and spike depend only on the mode, so they come from tables
-// (rtab, pred) built with the same arithmetic.  One wavefront handles 64
-// modes; the separators live in LDS ([P][64], P <= kMaxP) and the table
-// values are fetched in groups ahead of the dependent chain, whose only
-// carried value stays in a register.
-constexpr int kMaxP = 128;
-constexpr int kSepGroup = 16;
+// Separators of every mode.  The reduced system is periodic tridiagonal with
+// the same coefficients in every row (all chunks are alike), i.e. circulant,
+// so its solution is the circular convolution X_i = sum_k h[(i-k) mod P] R_k
+// of the reduced right-hand side R with h = C^-1 e_0 (one column per mode,
+// built on the host with the reference algorithm).  For the singular mode
+// h = C^+ (e_0 - 1/P), and the result is shifted so that X_{P-1} = 0, the
+// reference's singular branch.  A workgroup holds R and h of 16 modes in LDS;
+// each thread forms kConvS consecutive X_i of one mode, reading each R_k once
+// and sliding a register window over h.  The sum runs over ascending k.
+constexpr int kConvModes = 16;
+constexpr int kConvS = 8;
 
 template <int L>
-__global__ __launch_bounds__(64) void k_part_separators(
-    const double* __restrict__ q, const double* __restrict__ gfl, const double* __restrict__ pred,
-    const double* __restrict__ rtab, double* __restrict__ X, int nx, int P, double a,
-    int pin_mode0) {
+__global__ __launch_bounds__(256) void k_part_conv(const double* __restrict__ q,
+                                                   const double* __restrict__ gfl,
+                                                   const double* __restrict__ htab,
+                                                   double* __restrict__ X, int nx, int P, double a,
+                                                   int pin_mode0) {
   constexpr int K = L - 1;
-  __shared__ double sx[kMaxP * 64];
-  const int t = threadIdx.x;
-  const int mo = blockIdx.x * 64 + t;
+  constexpr int G = 256 / kConvModes;
+  extern __shared__ double lds[];
+  double* R = lds;                               // [P][16] reduced right-hand side
+  double* H = lds + (size_t)P * kConvModes;      // [P][16] inverse columns
+  double* Y = lds + (size_t)2 * P * kConvModes;  // [P][16] result
+  const int tm = threadIdx.x % kConvModes, g = threadIdx.x / kConvModes;
+  const int mo = blockIdx.x * kConvModes + tm;
   const bool live = mo < nx;
   const int mm = live ? mo : nx - 1;
-  for (int c = 0; c < P; c++) {
+  for (int c = g; c < P; c += G) {
     const double gl = gfl[(size_t)(2 * c + 1) * nx + mm];
     const double gf = gfl[(size_t)(2 * ((c + 1) % P)) * nx + mm];
-    sx[c * 64 + t] = (q[(size_t)(c * L + K) * nx + mm] - a * gl) - a * gf;
+    R[c * kConvModes + tm] = (q[(size_t)(c * L + K) * nx + mm] - a * gl) - a * gf;
+    H[c * kConvModes + tm] = htab[(size_t)c * nx + mm];
   }
-  const double lo = pred[mm], di = pred[(size_t)nx + mm], up = pred[(size_t)2 * nx + mm];
-  const double ilast = pred[(size_t)3 * nx + mm], ok = pred[(size_t)4 * nx + mm];
-  const double den = pred[(size_t)5 * nx + mm];
-  const double* val = rtab;
-  const double* vv = rtab + (size_t)P * nx;
-  const double* q1 = rtab + (size_t)2 * P * nx;
-  const int N = P - 1;
-  // gtsv on the first N rows (src/tdm.c:26-48)
-  double prev = sx[t] / di;
-  sx[t] = prev;
-  for (int r0 = 1; r0 < N - 1; r0 += kSepGroup) {
-    double tv[kSepGroup];
+  __syncthreads();
+  for (int i0 = g * kConvS; i0 < P; i0 += G * kConvS) {
+    double acc[kConvS], w[kConvS];
 #pragma unroll
-    for (int u = 0; u < kSepGroup; u++) tv[u] = (r0 + u < N - 1) ? val[(size_t)(r0 + u) * nx + mm] : 0.;
-#pragma unroll
-    for (int u = 0; u < kSepGroup; u++) {
-      const int r = r0 + u;
-      if (r < N - 1) {
-        prev = tv[u] * (sx[r * 64 + t] - lo * prev);
-        sx[r * 64 + t] = prev;
-      }
+    for (int r = 0; r < kConvS; r++) {
+      acc[r] = 0.;
+      w[r] = H[((i0 + r) % P) * kConvModes + tm];  // h[(i0 + r - k) mod P] at k = 0
     }
-  }
-  double nxt = ok != 0. ? ilast * (sx[(N - 1) * 64 + t] - lo * prev) : 0.;
-  sx[(N - 1) * 64 + t] = nxt;
-  for (int r0 = N - 2; r0 >= 0; r0 -= kSepGroup) {
-    double tv[kSepGroup];
+    int next = i0 > 0 ? i0 - 1 : P - 1;  // (i0 - k - 1) mod P
+    for (int k = 0; k < P; k++) {
+      const double rk = R[k * kConvModes + tm];
 #pragma unroll
-    for (int u = 0; u < kSepGroup; u++) tv[u] = (r0 - u >= 0) ? vv[(size_t)(r0 - u) * nx + mm] : 0.;
+      for (int r = 0; r < kConvS; r++) acc[r] += w[r] * rk;
 #pragma unroll
-    for (int u = 0; u < kSepGroup; u++) {
-      const int r = r0 - u;
-      if (r >= 0) {
-        nxt = sx[r * 64 + t] - tv[u] * nxt;
-        sx[r * 64 + t] = nxt;
-      }
+      for (int r = kConvS - 1; r > 0; r--) w[r] = w[r - 1];
+      w[0] = H[next * kConvModes + tm];
+      next = next > 0 ? next - 1 : P - 1;
     }
+#pragma unroll
+    for (int r = 0; r < kConvS; r++)
+      if (i0 + r < P) Y[(i0 + r) * kConvModes + tm] = acc[r];
   }
-  // closure (src/tdm.c:92-98); the singular mode pins the last separator
-  const double num = sx[(P - 1) * 64 + t] - up * sx[t] - lo * sx[(P - 2) * 64 + t];
-  const double xl = ((pin_mode0 && mo == 0) || fabs(den) < DBL_EPSILON) ? 0. : num / den;
+  __syncthreads();
+  const bool pin = pin_mode0 && mo == 0;
+  const double xl = Y[(P - 1) * kConvModes + tm];
   if (!live) return;
-  X[(size_t)(P - 1) * nx + mo] = xl;
-  for (int r = 0; r < P - 1; r++) X[(size_t)r * nx + mo] = sx[r * 64 + t] + xl * q1[(size_t)r * nx + mo];
+  for (int i = g; i < P; i += G) {
+    const double x = Y[i * kConvModes + tm];
+    X[(size_t)i * nx + mo] = pin ? x - xl : x;
+  }
 }
 
 template <int L>
@@ -200,20 +195,48 @@ static void host_thomas(int K, const double* inv, const double* v, double a, con
   for (int k = K - 2; k >= 0; k--) g[k] = g[k] - v[k] * g[k + 1];
 }
 
+// periodic tridiagonal with constant coefficients by the reference
+// algorithm (src/tdm.c:17-99): Thomas on the first n-1 rows for the
+// right-hand side and for the corner spike, then the closure for the last
+// unknown; `pin` takes the singular branch (last unknown 0)
+static void host_gtsv(int n, double l, double c, double u, double* v, double* q) {
+  v[0] = u / c;
+  q[0] = q[0] / c;
+  for (int i = 1; i < n - 1; i++) {
+    const double val = 1. / (c - l * v[i - 1]);
+    v[i] = val * u;
+    q[i] = val * (q[i] - l * q[i - 1]);
+  }
+  const double val = c - l * v[n - 2];
+  if (fabs(val) > DBL_EPSILON) q[n - 1] = 1. / val * (q[n - 1] - l * q[n - 2]);
+  else q[n - 1] = 0.;
+  for (int i = n - 2; i >= 0; i--) q[i] -= v[i] * q[i + 1];
+}
+
+static void host_periodic_tdm(int n, double l, double c, double u, bool pin, double* q, double* v,
+                              double* q1) {
+  for (int i = 0; i < n - 1; i++) q1[i] = (i == 0) ? -l : (i == n - 2) ? -u : 0.;
+  host_gtsv(n - 1, l, c, u, v, q1);
+  host_gtsv(n - 1, l, c, u, v, q);
+  const double num = q[n - 1] - u * q[0] - l * q[n - 2];
+  const double den = c + u * q1[0] + l * q1[n - 2];
+  q[n - 1] = (pin || fabs(den) < DBL_EPSILON) ? 0. : num / den;
+  for (int i = 0; i < n - 1; i++) q[i] = q[i] + q[n - 1] * q1[i];
+}
+
 int partition_setup(PoissonPlan* p, const domain_t* d, int L) {
   const int nx = p->nx, ny = p->ny, K = L - 1, P = ny / L;
   (void)hipFree(p->ptab);
-  (void)hipFree(p->pred);
   (void)hipFree(p->gfl);
   (void)hipFree(p->X);
-  (void)hipFree(p->rtab);
-  p->ptab = p->pred = p->gfl = p->X = p->rtab = nullptr;
+  (void)hipFree(p->htab);
+  p->ptab = p->gfl = p->X = p->htab = nullptr;
   std::vector<double> ev(nx);
   SBFS_HIP(hipMemcpy(ev.data(), p->evals, sizeof(double) * nx, hipMemcpyDeviceToHost));
   const double l = 1. / d->hy / d->hy;
   const double u = l;
-  std::vector<double> tab((size_t)4 * K * nx), red((size_t)6 * nx), rtab((size_t)3 * P * nx, 0.);
-  std::vector<double> rv(P), rq(P), rval(P);
+  std::vector<double> tab((size_t)4 * K * nx), htab((size_t)P * nx);
+  std::vector<double> hcol(P), hv(P), hq1(P);
   std::vector<double> inv(K), vv(K), w(K), z(K), f(K);
   for (int mo = 0; mo < nx; mo++) {
     const double a = l;
@@ -239,51 +262,20 @@ int partition_setup(PoissonPlan* p, const domain_t* d, int L) {
     const double lo = a * w[K - 1];
     const double di = (b + a * z[K - 1]) + a * w[0];
     const double up = a * z[0];
-    // data-independent part of the reduced periodic solve (src/tdm.c:78-94
-    // with l = lo, c = di, u = up): pivots, factors, spike q1, denominator
-    const int N = P - 1;
-    std::fill(rv.begin(), rv.end(), 0.);
-    std::fill(rval.begin(), rval.end(), 0.);
-    for (int r = 0; r < N; r++) rq[r] = (r == 0) ? -lo : (r == P - 2) ? -up : 0.;
-    rq[N] = 0.;
-    rv[0] = up / di;
-    rq[0] = rq[0] / di;
-    for (int r = 1; r < N - 1; r++) {
-      rval[r] = 1. / (di - lo * rv[r - 1]);
-      rv[r] = rval[r] * up;
-      rq[r] = rval[r] * (rq[r] - lo * rq[r - 1]);
-    }
-    const double lval = di - lo * rv[N - 2];
-    double ilast = 0., ok = 0.;
-    if (fabs(lval) > DBL_EPSILON) {
-      ilast = 1. / lval;
-      ok = 1.;
-      rq[N - 1] = ilast * (rq[N - 1] - lo * rq[N - 2]);
-    } else {
-      rq[N - 1] = 0.;
-    }
-    for (int r = N - 2; r >= 0; r--) rq[r] -= rv[r] * rq[r + 1];
-    const double den = di + up * rq[0] + lo * rq[P - 2];
-    for (int r = 0; r < P; r++) {
-      rtab[(size_t)(0 * P + r) * nx + mo] = rval[r];
-      rtab[(size_t)(1 * P + r) * nx + mo] = rv[r];
-      rtab[(size_t)(2 * P + r) * nx + mo] = rq[r];
-    }
-    red[mo] = lo;
-    red[nx + mo] = di;
-    red[2 * nx + mo] = up;
-    red[3 * nx + mo] = ilast;
-    red[4 * nx + mo] = ok;
-    red[5 * nx + mo] = den;
+    // first column of the reduced circulant's inverse (the singular mode 0:
+    // of its pseudo-inverse, from e_0 - 1/P with the last entry pinned)
+    const bool pin = mo == 0;
+    for (int c = 0; c < P; c++)
+      hcol[c] = pin ? (c == 0 ? 1. - 1. / (double)P : -1. / (double)P) : (c == 0 ? 1. : 0.);
+    host_periodic_tdm(P, lo, di, up, pin, hcol.data(), hv.data(), hq1.data());
+    for (int c = 0; c < P; c++) htab[(size_t)c * nx + mo] = hcol[c];
   }
   SBFS_HIP(hipMalloc(&p->ptab, sizeof(double) * tab.size()));
-  SBFS_HIP(hipMalloc(&p->pred, sizeof(double) * red.size()));
   SBFS_HIP(hipMalloc(&p->gfl, sizeof(double) * 2 * (size_t)P * nx));
   SBFS_HIP(hipMalloc(&p->X, sizeof(double) * (size_t)P * nx));
-  SBFS_HIP(hipMalloc(&p->rtab, sizeof(double) * rtab.size()));
-  SBFS_HIP(hipMemcpy(p->rtab, rtab.data(), sizeof(double) * rtab.size(), hipMemcpyHostToDevice));
+  SBFS_HIP(hipMalloc(&p->htab, sizeof(double) * htab.size()));
+  SBFS_HIP(hipMemcpy(p->htab, htab.data(), sizeof(double) * htab.size(), hipMemcpyHostToDevice));
   SBFS_HIP(hipMemcpy(p->ptab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
-  SBFS_HIP(hipMemcpy(p->pred, red.data(), sizeof(double) * red.size(), hipMemcpyHostToDevice));
   p->L = L;
   p->P = P;
   p->a = l;
@@ -298,8 +290,9 @@ static void part_launch(PoissonPlan* p, double a, int pin) {
              hipLaunchKernelGGL(k_part_reduce<L>, grd, blk, 0, stream(), p->buf, p->ptab, p->gfl,
                                 nx, P, a));
   SBFS_TIMED("k_part_separators",
-             hipLaunchKernelGGL(k_part_separators<L>, dim3(cdiv(nx, 64)), dim3(64), 0, stream(),
-                                p->buf, p->gfl, p->pred, p->rtab, p->X, nx, P, a, pin));
+             hipLaunchKernelGGL(k_part_conv<L>, dim3(cdiv(nx, kConvModes)), dim3(256),
+                                sizeof(double) * 3 * (size_t)P * kConvModes, stream(), p->buf,
+                                p->gfl, p->htab, p->X, nx, P, a, pin));
   SBFS_TIMED("k_part_expand",
              hipLaunchKernelGGL(k_part_expand<L>, grd, blk, 0, stream(), p->buf, p->ptab, p->X,
                                 nx, P, a));

@@ -62,12 +62,13 @@ def test_stages_isolated(oracle_mod, product, nx, ny, seed):
         set_tdm(product, 1)
 
 
-def test_partitioned_tdm_matches_oracle(oracle_mod, product):
+@pytest.mark.parametrize("nx,ny,chunk", [(32, 128, 16), (16, 48, 16), (64, 96, 8), (8, 2048, 16)])
+def test_partitioned_tdm_matches_oracle(oracle_mod, product, nx, ny, chunk):
     """GPU partitioned solver vs the oracle's restatement of the same
-    algorithm: bitwise; and vs the reference-order solver: 1e-11 relative."""
-    nx, ny = 32, 128
-    set_tdm(product, 1, 16)
-    oracle_mod.set_poisson(dct=oracle_mod.DCT_FFT, tdm=oracle_mod.TDM_PARTITIONED, chunk=16)
+    algorithm: bitwise; and vs the reference-order solver: 1e-11 relative
+    (reduced systems of 8, 3, 12 and 128 separators)."""
+    set_tdm(product, 1, chunk)
+    oracle_mod.set_poisson(dct=oracle_mod.DCT_FFT, tdm=oracle_mod.TDM_PARTITIONED, chunk=chunk)
     o, g = new_pair(oracle_mod, product, nx, ny)
     try:
         initial_state(o, 3)
