@@ -93,7 +93,7 @@ def test_partitioned_tdm_matches_oracle(oracle_mod, product, nx, ny, chunk):
         set_tdm(product, 1, 32)
 
 
-@pytest.mark.parametrize("n", [2, 4, 8, 16, 64, 128, 1024, 2048])
+@pytest.mark.parametrize("n", [2, 4, 8, 16, 64, 128, 1024, 2048, 4096])
 def test_dct_rows_bitwise(oracle_mod, product, n):
     rng = np.random.default_rng(n)
     x = rng.normal(size=(5, n))

@@ -73,8 +73,32 @@ def test_swarm_invariants(product):
         assert abs(d["volume"] - d0["volume"]) <= 1e-12 * d0["volume"]
         assert d["divmax"] < 1e-9
         assert d["vofmin"] > -1e-6 and d["vofmax"] < 1 + 1e-6
-        assert d["rise_velocity"] > 0.0
     finally:
+        g.close()
+
+
+def test_large_swarm_matches_oracle(oracle_mod, product):
+    """4096 x 8192 with 1024 bubbles (4096-point DCT rows, 64-row chunks of
+    the partitioned solve), 2 steps on the GPU against the oracle on the
+    host: velocity and vof within 1e-10 relative, p and psi after removing
+    their means."""
+    w = configs.large_swarm()
+    print(f"{w.name}: oracle and GPU, 2 steps", flush=True)
+    oracle_mod.set_threads(16)
+    oracle_mod.set_poisson(dct=oracle_mod.DCT_FFT, tdm=oracle_mod.TDM_SEQUENTIAL)
+    o, g = new_pair(oracle_mod, product, w.nx, w.ny, phys=dict(Re=w.Re, We=w.We))
+    try:
+        vof = w.vof()
+        o.load_initial_condition(vof)
+        g.load_initial_condition(vof)
+        for _ in range(2):
+            assert g.integrate() == pytest.approx(o.integrate(), rel=1e-12)
+        for n in ("ux", "uy", "vof"):
+            assert rel(g.get(n), o.get(n)) <= 1e-10, n
+        for n in ("p", "psi0"):
+            assert rel(demean(g.get(n)), demean(o.get(n))) <= 1e-10, n
+    finally:
+        o.close()
         g.close()
 
 
@@ -106,3 +130,78 @@ def test_fused_path_matches_staged_bitwise(oracle_mod, product, nx, ny):
         a.close()
         b.close()
         set_path(product, 1)
+
+
+def test_repo_default_rising_bubble_100_steps(oracle_mod, product):
+    """The 128 x 256 rising bubble (the reference's CPU-runnable case), 100
+    steps on the default (fused) path against the oracle: velocity and vof
+    within 1e-10 relative, p and psi after removing their means, the
+    rise-velocity history overlaid to 1e-9 and the volume to 1e-13."""
+    w = configs.rising_bubble(128, 256)
+    oracle_mod.set_poisson(dct=oracle_mod.DCT_FFT, tdm=oracle_mod.TDM_SEQUENTIAL)
+    o, g = new_pair(oracle_mod, product, w.nx, w.ny)
+    try:
+        o.load_initial_condition(w.vof())
+        copy_state(o, g)
+        v0 = o.diagnostics()["volume"]
+        for s in range(100):
+            dto, dtg = o.integrate(), g.integrate()
+            assert dtg == pytest.approx(dto, rel=1e-12)
+            if s % 20 == 19:
+                do, dg = o.diagnostics(), g.diagnostics()
+                assert dg["rise_velocity"] == pytest.approx(do["rise_velocity"], rel=1e-9)
+                assert abs(dg["volume"] - v0) <= 1e-13 * v0
+        for n in ("ux", "uy", "vof"):
+            assert rel(g.get(n), o.get(n)) <= 1e-10, n
+        for n in ("p", "psi0"):
+            assert rel(demean(g.get(n)), demean(o.get(n))) <= 1e-10, n
+    finally:
+        o.close()
+        g.close()
+
+
+@pytest.mark.parametrize("which", ["swarm", "large"])
+def test_full_size_swarm_invariants(product, which):
+    """The benchmark swarm (2048 x 4096, 64 bubbles) and the large swarm
+    (4096 x 8192, 1024 bubbles), 5 steps at full size: gas volume conserved
+    to 1e-12 relative, max |div u| at round-off, vof within [0, 1] to the
+    band tolerance."""
+    w = configs.bubble_swarm(2048, 4096, 64, seed=1) if which == "swarm" else configs.large_swarm()
+    print(f"{w.name}: {w.nx} x {w.ny}", flush=True)
+    g = solver.Case(product, w.nx, w.ny, w.lx, w.ly, w.Re, w.Fr, w.denr, w.visr, w.We)
+    try:
+        g.load_initial_condition(w.vof())
+        d0 = g.diagnostics()
+        for _ in range(5):
+            g.integrate()
+        d = g.diagnostics()
+        assert abs(d["volume"] - d0["volume"]) <= 1e-12 * d0["volume"]
+        assert d["divmax"] < 1e-9
+        assert d["vofmin"] > -1e-6 and d["vofmax"] < 1 + 1e-6
+    finally:
+        g.close()
+
+
+def test_large_swarm_matches_oracle(oracle_mod, product):
+    """4096 x 8192 with 1024 bubbles (4096-point DCT rows, 64-row chunks of
+    the partitioned solve), 2 steps on the GPU against the oracle on the
+    host: velocity and vof within 1e-10 relative, p and psi after removing
+    their means."""
+    w = configs.large_swarm()
+    print(f"{w.name}: oracle and GPU, 2 steps", flush=True)
+    oracle_mod.set_threads(16)
+    oracle_mod.set_poisson(dct=oracle_mod.DCT_FFT, tdm=oracle_mod.TDM_SEQUENTIAL)
+    o, g = new_pair(oracle_mod, product, w.nx, w.ny, phys=dict(Re=w.Re, We=w.We))
+    try:
+        vof = w.vof()
+        o.load_initial_condition(vof)
+        g.load_initial_condition(vof)
+        for _ in range(2):
+            assert g.integrate() == pytest.approx(o.integrate(), rel=1e-12)
+        for n in ("ux", "uy", "vof"):
+            assert rel(g.get(n), o.get(n)) <= 1e-10, n
+        for n in ("p", "psi0"):
+            assert rel(demean(g.get(n)), demean(o.get(n))) <= 1e-10, n
+    finally:
+        o.close()
+        g.close()
