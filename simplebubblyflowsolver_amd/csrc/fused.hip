@@ -19,7 +19,9 @@
 // term at stage 0).  dt stays in device memory for the whole step.
 #include <cfloat>
 #include <cmath>
+#include <cstdint>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "decide_dt.h"
@@ -630,30 +632,13 @@ static int march_band(int nx, int ny) {
   return (int)(band < ny ? band : ny);
 }
 
-int integrate_fused(const domain_t* domain, fluid_t* fluid, interface_t* interface, double* dt) {
-  DomainCtx* c = domain_ctx(domain);
-  if (!c) return 1;
-  const Nadds* nas[] = {&NA_UX, &NA_UY, &NA_P, &NA_P, &NA_P, &NA_DEN, &NA_DEN, &NA_TXY};
-  const array_t* arrs[] = {&fluid->ux, &fluid->uy, &fluid->p, &fluid->psi[0], &fluid->psi[1],
-                           &fluid->den[0], &fluid->den[1], &fluid->txy};
-  for (int k = 0; k < 8; k++)
-    if (check_array(domain, arrs[k], *nas[k], 8, "fluid array")) return 1;
-  for (int n = 0; n < 3; n++) {
-    if (check_array(domain, &fluid->srcux[n], NA_SRCUX, 8, "srcux")) return 1;
-    if (check_array(domain, &fluid->srcuy[n], NA_SRCUY, 8, "srcuy")) return 1;
-  }
-  if (check_array(domain, &interface->vof, NA_VOF, 8, "vof")) return 1;
-  for (int n = 0; n < 2; n++)
-    if (check_array(domain, &interface->src[n], NA_SRC, 8, "src")) return 1;
-  if (scratch(domain, interface, fluid)) return 1;
-  Coef cf;
-  if (coef_tables(domain, c, fluid->Re, &cf)) return 1;
+// Every launch of one fused time step (decide_dt, then per Runge-Kutta stage
+// the stage kernel, the Poisson solve and the projection) plus the host-side
+// pointer swaps the reference performs.  No allocation and no host
+// synchronisation, so the sequence can be captured into a graph.
+static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, interface_t* interface,
+                      const Coef& cf, double* rhs, int band, bool use_tiles) {
   const int nx = (int)c->nx, ny = (int)c->ny;
-  const bool use_tiles = getenv("SBFS_PRE_TILES") != nullptr;
-  double* rhs = nullptr;
-  if (!use_tiles && poisson_transform_fused(domain, c, fluid, &rhs)) return 1;
-  const int band = march_band(nx, ny);
-  if (band <= 0) return 1;
   if (compute_dt_device(domain, c, fluid, interface)) return 1;
   const double* dtp = c->dt_dev;
   for (int k = 0; k < 3; k++) {
@@ -794,6 +779,149 @@ int integrate_fused(const domain_t* domain, fluid_t* fluid, interface_t* interfa
     if (check_launch("integrate: k_post")) return 1;
     swap_data(fluid->psi[0], fluid->psi[1]);
   }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Graph replay of the fused step.  The step's launches depend only on the
+// array pointers, the case parameters and the options, and the pointers
+// return to one of two states (psi and vof swap an odd number of times per
+// step), so up to two captured graphs serve a run.  A configuration is
+// captured the second time it is seen: the first, eager, pass builds every
+// plan and table the step needs, so the captured sequence allocates and
+// synchronises nothing.  Profiling (per-launch events) and SBFS_GRAPH=0
+// keep the eager path.
+// ---------------------------------------------------------------------------
+struct StepKey {
+  std::vector<uint64_t> v;
+  bool operator==(const StepKey& o) const { return v == o.v; }
+};
+
+static void key_add(StepKey& k, const void* p) { k.v.push_back((uint64_t)(uintptr_t)p); }
+static void key_add(StepKey& k, double x) {
+  uint64_t u;
+  memcpy(&u, &x, sizeof(u));
+  k.v.push_back(u);
+}
+
+static array_t* state_arrays(fluid_t* f, interface_t* s, int k) {
+  array_t* a[] = {&f->ux, &f->uy, &f->p, &f->psi[0], &f->psi[1], &f->den[0], &f->den[1],
+                  &f->txy, &f->srcux[0], &f->srcux[1], &f->srcux[2], &f->srcuy[0],
+                  &f->srcuy[1], &f->srcuy[2], &s->vof, &s->src[0], &s->src[1]};
+  return k < 17 ? a[k] : nullptr;
+}
+constexpr int kStateArrays = 17;
+
+static StepKey step_key(const DomainCtx* c, fluid_t* f, interface_t* s, const Coef& cf, double* rhs,
+                        int band, bool use_tiles) {
+  StepKey k;
+  key_add(k, c);
+  k.v.push_back(c->hash);
+  for (int n = 0; n < kStateArrays; n++) key_add(k, state_arrays(f, s, n)->data);
+  key_add(k, g_scr.vof);
+  key_add(k, g_scr.ux);
+  key_add(k, g_scr.uy);
+  key_add(k, cf.rhf);
+  key_add(k, rhs);
+  for (double x : {f->Re, f->Fr, f->denr, f->visr, f->refden, s->We, options().cadv, options().cdif,
+                   options().cint})
+    key_add(k, x);
+  k.v.push_back((uint64_t)band | ((uint64_t)use_tiles << 32));
+  k.v.push_back((uint64_t)options().gauge | ((uint64_t)options().tdm << 8) |
+                ((uint64_t)options().chunk << 16));
+  return k;
+}
+
+struct StepGraph {
+  StepKey key;
+  hipGraphExec_t exec = nullptr;
+  void* after[kStateArrays + 1] = {};  // state pointers after the step, + scratch vof
+  uint64_t last_use = 0;
+};
+static StepGraph g_graph[2];
+static StepKey g_seen[2];
+static uint64_t g_clock = 0;
+
+static bool graphs_enabled() {
+  static int on = -1;
+  if (on < 0) {
+    const char* e = getenv("SBFS_GRAPH");
+    on = (e && e[0] == '0') ? 0 : 1;
+  }
+  return on == 1 && !prof_active();
+}
+
+static int step_graph(const domain_t* domain, DomainCtx* c, fluid_t* fluid, interface_t* interface,
+                      const Coef& cf, double* rhs, int band, bool use_tiles) {
+  if (!graphs_enabled()) return fused_body(domain, c, fluid, interface, cf, rhs, band, use_tiles);
+  const StepKey key = step_key(c, fluid, interface, cf, rhs, band, use_tiles);
+  g_clock++;
+  for (StepGraph& g : g_graph) {
+    if (g.exec && g.key == key) {
+      SBFS_HIP(hipGraphLaunch(g.exec, stream()));
+      for (int n = 0; n < kStateArrays; n++) state_arrays(fluid, interface, n)->data = g.after[n];
+      g_scr.vof = (double*)g.after[kStateArrays];
+      g.last_use = g_clock;
+      return 0;
+    }
+  }
+  const bool seen = g_seen[0] == key || g_seen[1] == key;
+  if (!seen) {
+    // first sight of this configuration: run eagerly (builds plans / tables)
+    g_seen[g_clock & 1] = key;
+    return fused_body(domain, c, fluid, interface, cf, rhs, band, use_tiles);
+  }
+  StepGraph& g = g_graph[0].last_use <= g_graph[1].last_use ? g_graph[0] : g_graph[1];
+  if (g.exec) {
+    (void)hipGraphExecDestroy(g.exec);
+    g.exec = nullptr;
+  }
+  hipGraph_t graph = nullptr;
+  SBFS_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeRelaxed));
+  const int rc = fused_body(domain, c, fluid, interface, cf, rhs, band, use_tiles);
+  const hipError_t ce = hipStreamEndCapture(stream(), &graph);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return 1;
+  }
+  SBFS_HIP(ce);
+  const hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  SBFS_HIP(ie);
+  g.key = key;
+  for (int n = 0; n < kStateArrays; n++) g.after[n] = state_arrays(fluid, interface, n)->data;
+  g.after[kStateArrays] = g_scr.vof;
+  g.last_use = g_clock;
+  SBFS_HIP(hipGraphLaunch(g.exec, stream()));
+  return 0;
+}
+
+int integrate_fused(const domain_t* domain, fluid_t* fluid, interface_t* interface, double* dt) {
+  DomainCtx* c = domain_ctx(domain);
+  if (!c) return 1;
+  const Nadds* nas[] = {&NA_UX, &NA_UY, &NA_P, &NA_P, &NA_P, &NA_DEN, &NA_DEN, &NA_TXY};
+  const array_t* arrs[] = {&fluid->ux, &fluid->uy, &fluid->p, &fluid->psi[0], &fluid->psi[1],
+                           &fluid->den[0], &fluid->den[1], &fluid->txy};
+  for (int k = 0; k < 8; k++)
+    if (check_array(domain, arrs[k], *nas[k], 8, "fluid array")) return 1;
+  for (int n = 0; n < 3; n++) {
+    if (check_array(domain, &fluid->srcux[n], NA_SRCUX, 8, "srcux")) return 1;
+    if (check_array(domain, &fluid->srcuy[n], NA_SRCUY, 8, "srcuy")) return 1;
+  }
+  if (check_array(domain, &interface->vof, NA_VOF, 8, "vof")) return 1;
+  for (int n = 0; n < 2; n++)
+    if (check_array(domain, &interface->src[n], NA_SRC, 8, "src")) return 1;
+  if (scratch(domain, interface, fluid)) return 1;
+  Coef cf;
+  if (coef_tables(domain, c, fluid->Re, &cf)) return 1;
+  const int nx = (int)c->nx, ny = (int)c->ny;
+  const bool use_tiles = getenv("SBFS_PRE_TILES") != nullptr;
+  double* rhs = nullptr;
+  if (!use_tiles && poisson_transform_fused(domain, c, fluid, &rhs)) return 1;
+  const int band = march_band(nx, ny);
+  if (band <= 0) return 1;
+  if (load_dt_coefs()) return 1;
+  if (step_graph(domain, c, fluid, interface, cf, rhs, band, use_tiles)) return 1;
   SBFS_HIP(hipMemcpyAsync(c->host_pin, c->dt_dev, sizeof(double), hipMemcpyDeviceToHost, stream()));
   SBFS_HIP(hipStreamSynchronize(stream()));
   *dt = c->host_pin[0];

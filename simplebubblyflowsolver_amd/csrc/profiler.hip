@@ -31,6 +31,8 @@ static hipEvent_t take_event() {
   return g_pool[g_used++];
 }
 
+bool prof_active() { return g_on; }
+
 void prof_begin(const char* name) {
   if (!g_on) return;
   ProfRec r{name, take_event(), take_event()};

@@ -148,6 +148,7 @@ int npy_write(const char* dir, const char* name, const char* descr,
 int load_dt_coefs();
 
 // kernel timing (profiler.hip): events around launches while profiling
+bool prof_active();  // kernel timing on (sbfs_profile_start)
 void prof_begin(const char* name);
 void prof_end();
 #define SBFS_TIMED(name, ...) \

@@ -105,9 +105,12 @@ def test_large_swarm_matches_oracle(oracle_mod, product):
 @pytest.mark.parametrize("nx,ny", [(64, 128), (32, 66)])
 def test_fused_path_matches_staged_bitwise(oracle_mod, product, nx, ny):
     """The fused kernels recompute every intermediate with the staged
-    kernels' device functions and operation order: 4 steps from a state
+    kernels' device functions and operation order: 8 steps from a state
     with random velocities agree bitwise in every state field (ny / nx not
-    multiples of the tile size exercise the ragged tiles)."""
+    multiples of the tile size exercise the ragged tiles).  The fused case
+    runs its first two steps eagerly, captures the two pointer states into
+    graphs on steps 3-4 and replays them on steps 5-8, with the staged
+    case's eager work and synchronisations in between."""
     from harness import initial_state
     o, _ = new_pair(oracle_mod, product, nx, ny)
     a = solver.Case(product, nx, ny, 1.0, 2.0, 35.0, 1.0, 1e-3, 1e-2, 10.0)
@@ -116,7 +119,7 @@ def test_fused_path_matches_staged_bitwise(oracle_mod, product, nx, ny):
         initial_state(o, 7, velocity=0.05)
         copy_state(o, a)
         copy_state(o, b)
-        for _ in range(4):
+        for _ in range(8):
             set_path(product, 0)
             dta = a.integrate()
             set_path(product, 1)
