@@ -18,21 +18,34 @@ static const double kDtMax = 1.;                              // decide_dt.c:25
 
 
 // advective limit min over faces of h / (|u| + small) (decide_dt.c:49-64)
-__global__ void k_dt_partial(const double* __restrict__ ux, const double* __restrict__ uy,
-                             const double* __restrict__ hxxf, double hy, int nx, int ny,
-                             double* __restrict__ partial) {
+// advective limit min over faces of h / (|u| + small) (decide_dt.c:49-64):
+// a workgroup takes kRedThreads consecutive columns of `rows` rows (16, or
+// more when the partial minima would not fit the reduction buffer), each
+// thread one column, so the loads of a thread are independent and in flight
+// together; a min does not depend on the combination order
+
+__global__ __launch_bounds__(kRedThreads) void k_dt_partial(const double* __restrict__ ux,
+                                                           const double* __restrict__ uy,
+                                                           const double* __restrict__ hxxf,
+                                                           double hy, int nx, int ny, int rows,
+                                                           double* __restrict__ partial) {
   __shared__ double sh[kRedThreads / 64];
   const double small = 1.e-8;
   double m = kDtMax;
-  const long total = (long)nx * ny;
-  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
-       idx += (long)gridDim.x * blockDim.x) {
-    const int i = (int)(idx % nx) + 1, j = (int)(idx / nx) + 1;
-    if (i >= 2) m = fmin(m, hxxf[i - 1] / (fabs(ux[L::xf(i, j, nx)]) + small));
-    m = fmin(m, hy / (fabs(uy[L::cc(i, j, nx)]) + small));
+  const int i = 1 + (int)(blockIdx.x * kRedThreads + threadIdx.x);
+  const int j0 = 1 + (int)blockIdx.y * rows;
+  if (i <= nx) {
+    const double hx = hxxf[i - 1];
+#pragma unroll 16
+    for (int r = 0; r < rows; r++) {
+      const int j = j0 + r;
+      if (j > ny) break;
+      if (i >= 2) m = fmin(m, hx / (fabs(ux[L::xf(i, j, nx)]) + small));
+      m = fmin(m, hy / (fabs(uy[L::cc(i, j, nx)]) + small));
+    }
   }
   m = block_reduce<0>(m, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = m;
+  if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = m;
 }
 
 // combine the limits (decide_dt.c:157-167)
@@ -80,12 +93,14 @@ int compute_dt_device(const domain_t* d, DomainCtx* c, const fluid_t* f,
   const int nx = (int)c->nx, ny = (int)c->ny;
   double dif0, dif1, dint;
   dt_limits(d, f, s, &dif0, &dif1, &dint);
-  const long total = (long)nx * ny;
-  const int nblk = (int)std::min<long>(kRedBlocks, (total + kRedThreads - 1) / kRedThreads);
+  int rows = 16;
+  while ((long)cdiv(nx, kRedThreads) * cdiv(ny, rows) > kRedSlots - 64) rows *= 2;
+  const dim3 grd(cdiv(nx, kRedThreads), cdiv(ny, rows));
+  const int nblk = (int)(grd.x * grd.y);
   SBFS_TIMED("k_dt_partial",
-             hipLaunchKernelGGL(k_dt_partial, dim3(nblk), dim3(kRedThreads), 0, stream(),
+             hipLaunchKernelGGL(k_dt_partial, grd, dim3(kRedThreads), 0, stream(),
                                 (const double*)f->ux.data, (const double*)f->uy.data, c->hxxf,
-                                c->hy, nx, ny, c->red));
+                                c->hy, nx, ny, rows, c->red));
   SBFS_TIMED("k_dt_final",
              hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(kRedThreads), 0, stream(),
                                 (const double*)c->red, nblk, options().cadv, dif0, dif1, dint,
