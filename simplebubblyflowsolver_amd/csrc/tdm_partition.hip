@@ -289,7 +289,7 @@ static void part_launch(PoissonPlan* p, double a, int pin) {
   SBFS_TIMED("k_part_reduce",
              hipLaunchKernelGGL(k_part_reduce<L>, grd, blk, 0, stream(), p->buf, p->ptab, p->gfl,
                                 nx, P, a));
-  SBFS_TIMED("k_part_separators",
+  SBFS_TIMED("k_part_conv",
              hipLaunchKernelGGL(k_part_conv<L>, dim3(cdiv(nx, kConvModes)), dim3(256),
                                 sizeof(double) * 3 * (size_t)P * kConvModes, stream(), p->buf,
                                 p->gfl, p->htab, p->X, nx, P, a, pin));
