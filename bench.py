@@ -53,7 +53,7 @@ BYTES_PER_CELL = {
     "k_part_expand": 16, "k_dct3_psi": 16, "k_correct_ux": 40, "k_correct_uy": 40,
     "k_update_p": 24, "k_dt_partial": 16, "k_dt_final": 0,
     # fused path
-    "k_march": (12 + 15 + 14) * 8 / 3, "k_pre": (10 + 13 + 12) * 8 / 3, "k_dct2_rows": 16,
+    "k_march": (12 + 15 + 14) * 8 / 3, "k_march_mixed": (12 + 15 + 14) * 8 / 3, "k_pre": (10 + 13 + 12) * 8 / 3, "k_dct2_rows": 16,
     "k_post": 72,
 }
 
@@ -120,6 +120,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--path", choices=["fused", "staged"], default="fused")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="mixed: FP32 interface and momentum stencils in the stage kernel")
     ap.add_argument("--workload", default="swarm")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=40)
@@ -139,6 +141,7 @@ def main() -> int:
     lib = be.lib
     be.set_dt_coefs(*configs.DT_COEFS)
     lib.sbfs_set_path(1 if a.path == "fused" else 0)
+    be.check(lib.sbfs_set_precision(1 if a.precision == "mixed" else 0), "set_precision")
     g = solver.Case(be, w.nx, w.ny, w.lx, w.ly, w.Re, w.Fr, w.denr, w.visr, w.We)
     g.load_initial_condition(w.vof())
     d0 = g.diagnostics()
@@ -197,10 +200,13 @@ def main() -> int:
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Mcell-steps/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(t / a.steps * 1e3, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if a.precision == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": w.name, "grid": [w.nx, w.ny], "bubbles": len(w.bubbles),
-                   "Re": w.Re, "We": w.We, "precision": "plain FP64", "path": a.path,
+                   "Re": w.Re, "We": w.We,
+                   "precision": "plain FP64" if a.precision == "fp64" else "mixed (FP32 stencils, FP64 rest)",
+                   "path": a.path,
                    "volume_drift": abs(d1["volume"] - d0["volume"]) / d0["volume"],
                    "divmax": d1["divmax"]},
         "roofline": roof,

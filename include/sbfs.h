@@ -48,6 +48,17 @@ extern int sbfs_set_gauge(int gauge);
                                   (default for Ny >= 64) */
 extern int sbfs_set_tdm(int kind, int chunk);
 
+/* arithmetic of the fused step's stage kernel (the staged path is always
+ * FP64).  MIXED evaluates the interface stencils (gradients, normals,
+ * curvature, THINC and mass fluxes) and the momentum sources (viscous
+ * stresses, advection, surface tension, buoyancy) in FP32; the vof update,
+ * the density-weighted velocity update, the pressure gradient, the Poisson
+ * right-hand side and solve, the projection and every field in memory stay
+ * FP64, so the gas volume is conserved to FP64 round-off. */
+#define SBFS_PRECISION_FP64  0
+#define SBFS_PRECISION_MIXED 1
+extern int sbfs_set_precision(int precision);
+
 /* uniform-x domain built as the reference's IC generator does
  * (initial_condition/main.py:15-24): xf = linspace(0, lx, Nx+1),
  * xc = [0, face midpoints, lx]; metrics as src/domain.c:98-146 */

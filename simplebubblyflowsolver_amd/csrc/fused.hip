@@ -602,7 +602,8 @@ static void swap_data(array_t& x, array_t& y) {
   y.data = t;
 }
 
-__global__ void k_march(MarchArgs a);  // march.hip
+__global__ void k_march(MarchArgs a);        // march.hip, plain FP64
+__global__ void k_march_mixed(MarchArgs a);  // march.hip, FP32 stencils
 
 // Rows per band of k_march.  Every wavefront marches its band plus kWarm + 2
 // rows of ring fill, and a wavefront lives for the whole band, so the grid
@@ -610,22 +611,25 @@ __global__ void k_march(MarchArgs a);  // march.hip
 // wavefront slots per strip (occupancy x CUs), each as short as that
 // allows.  A partial second round would leave most SIMDs idle while it
 // runs.  SBFS_MARCH_BAND overrides the choice.
-static int march_band(int nx, int ny) {
-  static long slots = 0;
-  if (slots == 0) {
+static int march_band(int nx, int ny, bool mixed) {
+  static long slots[2] = {0, 0};
+  long& sl = slots[mixed ? 1 : 0];
+  if (sl == 0) {
     int dev = 0, per_cu = 0;
     hipDeviceProp_t prop;
     SBFS_HIP(hipGetDevice(&dev));
     SBFS_HIP(hipGetDeviceProperties(&prop, dev));
-    SBFS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march, 64, 0));
-    slots = (long)(per_cu > 0 ? per_cu : 1) * prop.multiProcessorCount;
+    if (mixed) SBFS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march_mixed, 64, 0));
+    else SBFS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march, 64, 0));
+    sl = (long)(per_cu > 0 ? per_cu : 1) * prop.multiProcessorCount;
   }
+  const long slots_now = sl;
   if (const char* e = getenv("SBFS_MARCH_BAND")) {
     const int b = atoi(e);
     if (b >= 1) return b < ny ? b : ny;
   }
   const long strips = (nx + kStrip - 1) / kStrip;
-  long bands = slots / strips;
+  long bands = slots_now / strips;
   if (bands < 1) bands = 1;
   long band = (ny + bands - 1) / bands;
   if (band < 4) band = 4;
@@ -739,7 +743,10 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
       m.ny = ny;
       m.band = band;
       const dim3 grid(cdiv(nx, kStrip), cdiv(ny, band)), block(64);
-      SBFS_TIMED("k_march", hipLaunchKernelGGL(k_march, grid, block, 0, stream(), m));
+      if (options().precision == SBFS_PRECISION_MIXED)
+        SBFS_TIMED("k_march_mixed", hipLaunchKernelGGL(k_march_mixed, grid, block, 0, stream(), m));
+      else
+        SBFS_TIMED("k_march", hipLaunchKernelGGL(k_march, grid, block, 0, stream(), m));
       if (check_launch("integrate: k_march")) return 1;
     }
     {
@@ -828,7 +835,7 @@ static StepKey step_key(const DomainCtx* c, fluid_t* f, interface_t* s, const Co
     key_add(k, x);
   k.v.push_back((uint64_t)band | ((uint64_t)use_tiles << 32));
   k.v.push_back((uint64_t)options().gauge | ((uint64_t)options().tdm << 8) |
-                ((uint64_t)options().chunk << 16));
+                ((uint64_t)options().chunk << 16) | ((uint64_t)options().precision << 48));
   return k;
 }
 
@@ -918,7 +925,7 @@ int integrate_fused(const domain_t* domain, fluid_t* fluid, interface_t* interfa
   const bool use_tiles = getenv("SBFS_PRE_TILES") != nullptr;
   double* rhs = nullptr;
   if (!use_tiles && poisson_transform_fused(domain, c, fluid, &rhs)) return 1;
-  const int band = march_band(nx, ny);
+  const int band = march_band(nx, ny, options().precision == SBFS_PRECISION_MIXED);
   if (band <= 0) return 1;
   if (load_dt_coefs()) return 1;
   if (step_graph(domain, c, fluid, interface, cf, rhs, band, use_tiles)) return 1;

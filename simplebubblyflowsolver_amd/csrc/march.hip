@@ -29,6 +29,10 @@
 #include "sbfs_internal.hpp"
 #include "stencils.hpp"
 
+#ifndef SBFS_MIXED_WAVES
+#define SBFS_MIXED_WAVES 2
+#endif
+
 namespace sbfs {
 
 // value of the neighbouring lane (l-1 / l+1) through DPP wave shifts
@@ -56,7 +60,90 @@ __device__ __forceinline__ int wrap_r(int j, int ny) {
 
 enum { T_RHF, T_RHC, T_RJF, T_RJC, T_QF, T_QC, T_EQF, T_ECY, T_CYC, T_CYF, T_DFX, T_DCX, T_HXXF, T_HXXC, T_N };
 
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_march(MarchArgs a) {
+// Arithmetic of the interface and momentum stencils.  FP64 is the reference's
+// arithmetic, expression for expression (stencils.hpp); FP32 is the mixed-
+// precision step's, with the device's fast reciprocal, square root,
+// exponential and logarithm.
+template <typename R>
+struct Arith;
+
+template <>
+struct Arith<double> {
+  static __device__ __forceinline__ double left(double v) { return from_left(v); }
+  static __device__ __forceinline__ double right(double v) { return from_right(v); }
+  static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+  static __device__ __forceinline__ double sqr(double x) { return sqrt(x); }
+  static __device__ __forceinline__ double eps() { return DBL_EPSILON; }
+  // 0.5 (1 + tanh(beta sf)) (update/main.c:18-35)
+  static __device__ __forceinline__ double indicator(double sf) { return 0.5 * (1. + tanh(kVofBeta * sf)); }
+  // -0.5 / beta ln(1 / vof - 1) (curvature_tensor.c:92)
+  static __device__ __forceinline__ double intercept(double v) { return -0.5 / kVofBeta * log(1. / v - 1.); }
+};
+
+template <>
+struct Arith<float> {
+  static __device__ __forceinline__ float left(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+  }
+  static __device__ __forceinline__ float right(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+  }
+  static __device__ __forceinline__ float div(float a, float b) { return __fdividef(a, b); }
+  static __device__ __forceinline__ float sqr(float x) { return __fsqrt_rn(x); }
+  static __device__ __forceinline__ float eps() { return (float)DBL_EPSILON; }
+  // tanh from one exponential: 1 - 2 / (e^{2|x|} + 1), sign restored
+  static __device__ __forceinline__ float indicator(float sf) {
+    const float x = (float)kVofBeta * sf;
+    const float t = 1.f - __fdividef(2.f, __expf(2.f * fabsf(x)) + 1.f);
+    return 0.5f * (1.f + copysignf(t, x));
+  }
+  // 0.5 / beta (ln vof - ln(1 - vof)); 1 - vof in FP64 keeps the band edge
+  static __device__ __forceinline__ float intercept(double v) {
+    return 0.5f / (float)kVofBeta * (__logf((float)v) - __logf((float)(1. - v)));
+  }
+};
+
+template <typename R>
+struct V3R {
+  R x, y, z;
+};
+
+// THINC face value from the upwind cell, averaged over the 2-point Gauss
+// rule (update/fluxx.c:53-68, fluxy.c:53-68); the band test reads the FP64 vof
+template <typename R, bool XFACE>
+__device__ __forceinline__ R thinc_flux(R vel, double v, V3R<R> n) {
+  using A = Arith<R>;
+  const R h = vel < R(0) ? R(-0.5) : R(+0.5);
+  if (!in_band(v)) return vel * R(v);
+  const R gp = R(kGaussP), gw = R(kGaussW);
+  R fl = R(0);
+  if (XFACE) {
+    fl += gw * A::indicator(n.x * h + n.y * -gp + n.z);
+    fl += gw * A::indicator(n.x * h + n.y * +gp + n.z);
+  } else {
+    fl += gw * A::indicator(n.x * -gp + n.y * h + n.z);
+    fl += gw * A::indicator(n.x * +gp + n.y * h + n.z);
+  }
+  return vel * fl;
+}
+
+template <typename R>
+__device__ __forceinline__ R mixr(double ratio, double lo, double hi, R v) {
+  R r = R(1) + (R(ratio) - R(1)) * v;
+  r = fmax(R(lo), r);
+  r = fmin(R(hi), r);
+  return r;
+}
+
+// R = double: the plain FP64 step.  R = float: the stencils of the
+// interface (gradients, normals, curvature, THINC and mass fluxes) and of the
+// momentum sources (viscosities, stresses, advection, surface tension,
+// buoyancy) in FP32; the vof update, the density-weighted velocity update,
+// the pressure gradient and the Poisson right-hand side stay FP64.
+template <typename R>
+__device__ __forceinline__ void march_body(const MarchArgs& a) {
+  using A = Arith<R>;
+  using V3 = V3R<R>;
   __shared__ double st[T_N][64];
   const int lane = threadIdx.x;
   const int nx = a.nx, ny = a.ny;
@@ -78,16 +165,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
   }
   __syncthreads();
   const int ll = lane > 0 ? lane - 1 : 0, lr = lane < 63 ? lane + 1 : 63;
+  // tables in FP64 (the FP64 parts) and in the stencil type
   auto TB = [&](int t) { return st[t][lane]; };
-  auto TL = [&](int t) { return st[t][ll]; };
   auto TR = [&](int t) { return st[t][lr]; };
+  auto SB = [&](int t) { return R(st[t][lane]); };
+  auto SL = [&](int t) { return R(st[t][ll]); };
+  auto SR = [&](int t) { return R(st[t][lr]); };
   const double hy = a.hy, rhy = a.cf.rhy;
+  const R hyr = R(hy), rhyr = R(rhy);
   const double dt = *a.dt;
   const double dtca = dt * a.ca, dtcb = dt * a.cb;
   const double dt_old = a.g_old * dt, dt_new = a.g_new * dt;
   const double c0 = 1. / a.norm / dt_new * a.refden;
   const double c1 = 1. / a.norm * -1. / dt_new * dt_old;
   const bool has_b = a.stage != 0;
+  const R denr1 = R(a.denr) - R(1);
+  const R refdeninv = R(a.refdeninv), tension = R(a.tension), grav = R(a.grav);
 
   // Row pointers are uniform and shifted to lane 0's column, so every access
   // is a scalar base plus the lane's byte offset.
@@ -102,14 +195,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
   auto ld_cc = [&](const double* f, int j) { return c_vof ? cc_row(f, wrap_r(j, ny))[lane] : 0.; };
   auto DEN = [&](double v) { return mix_clamp(a.denr, a.dlo, a.dhi, v); };
   // viscosity at x face c from the vof on its two sides (viscosity.c:37-53)
-  auto VISUX = [&](int col, double vl, double vr) {
-    double v;
+  auto VISUX = [&](int col, R vl, R vr) {
+    R v;
     if (col == 1) v = vl;
     else if (col == nx + 1) v = vr;
-    else v = 0.5 * vl + 0.5 * vr;
-    return mix_clamp(a.visr, a.vlo, a.vhi, v);
+    else v = R(0.5) * vl + R(0.5) * vr;
+    return mixr<R>(a.visr, a.vlo, a.vhi, v);
   };
-  auto VISUY = [&](double vb, double vt) { return mix_clamp(a.visr, a.vlo, a.vhi, 0.5 * vb + 0.5 * vt); };
+  auto VISUY = [&](R vb, R vt) { return mixr<R>(a.visr, a.vlo, a.vhi, R(0.5) * vb + R(0.5) * vt); };
 
   // Rings (row offsets relative to the current step s).  Several terms of
   // the reference's stencils are the same expression evaluated at the next
@@ -117,20 +210,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
   // they are formed once and taken from the neighbouring lane (DPP) or from
   // the previous step, which gives the same bits.
   double vm1 = 0., v0 = 0., vp1 = 0., vp2 = 0.;             // vof s-1..s+2
-  double g0x = 0., g0y = 0., g1x = 0., g1y = 0., g2x = 0., g2y = 0.;  // corners s..s+2
-  double g0xr = 0., g0yr = 0., g1xr = 0., g1yr = 0.;         // corners of c+1, rows s, s+1
-  double n0x = 0., n0y = 0., n0z = 0., n1x = 0., n1y = 0., n1z = 0.;  // normals s, s+1
-  double km1 = 0., k0 = 0.;                                 // curvature s-1, s
-  double mfxm1 = 0., mfx0 = 0.;                             // x mass fluxes s-1, s
-  double fy0 = 0., fyp1 = 0., mfy0 = 0., mfyp1 = 0.;        // y faces s, s+1
-  double visx0 = 0., visx1 = 0., visy0 = 0., visy1 = 0.;    // viscosities rows / faces s, s+1
-  double txy0 = 0., cyf_txy0 = 0., cyc_tyym1 = 0.;          // stress terms of rows s, s-1
+  R g0x = 0, g0y = 0, g1x = 0, g1y = 0, g2x = 0, g2y = 0;   // corners s..s+2
+  R g0xr = 0, g0yr = 0, g1xr = 0, g1yr = 0;                 // corners of c+1, rows s, s+1
+  R n0x = 0, n0y = 0, n0z = 0, n1x = 0, n1y = 0, n1z = 0;   // normals s, s+1
+  R km1 = 0, k0 = 0;                                        // curvature s-1, s
+  R mfxm1 = 0, mfx0 = 0;                                    // x mass fluxes s-1, s
+  R fy0 = 0, fyp1 = 0, mfy0 = 0, mfyp1 = 0;                 // y faces s, s+1
+  R visx0 = 0, visx1 = 0, visy0 = 0, visy1 = 0;             // viscosities rows / faces s, s+1
+  R txy0 = 0, cyf_txy0 = 0, cyc_tyym1 = 0;                  // stress terms of rows s, s-1
   double d_0 = 0.;                                          // density of vof row s
   double vn0 = 0., dnm1 = 0., dn0 = 0.;                     // new vof row s, its density s-1, s
   double ux0 = 0., uxp1 = 0., uy0 = 0., uyp1 = 0.;
   double pm1 = 0., p0 = 0.;
   double psm1 = 0., ps0 = 0.;                               // psi[0] s-1, s
-  double qy_ux = 0., qy_uy = 0.;                            // y momentum fluxes at face s
+  R qy_ux = 0, qy_uy = 0;                                   // y momentum fluxes at face s
   double q_uxs_m1 = 0., cyc_uys_m1 = 0.;                    // rhs terms of row s-1
   double d1x_m1 = 0., d1y = 0., cg_ym = 0.;                 // rhs densities / y gradient
   const int s_first = jb - kWarm;
@@ -161,40 +254,43 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
       np = ld_cc(a.p, s + 1);
       nps = ld_cc(a.psi0, s + 1);
     }
+    const R rux0 = R(ux0), ruxp1 = R(uxp1), ruy0 = R(uy0), ruyp1 = R(uyp1);
+    const R rv0 = R(v0), rvp1 = R(vp1), rvm1 = R(vm1);
     // -- corner gradients, row s+2 (curvature_tensor.c:16-49) ----------------
     g0x = g1x; g0y = g1y; g1x = g2x; g1y = g2y;
     g0xr = g1xr; g0yr = g1yr;
     {
       const double mm = from_left(vp1), pm = vp1, mp = from_left(vp2), pp = vp2;
-      double gx = 0., gy = 0.;
+      R gx = 0, gy = 0;
       if (c_fx && !(mm == pm && mm == mp && mm == pp)) {
-        gx = TB(T_RHF) * (-mm + pm - mp + pp);
-        gy = rhy * (-mm - pm + mp + pp);
-        const double norm = sqrt(gx * gx + gy * gy);
-        const double ninv = 1. / fmax(norm, DBL_EPSILON);
+        const R rmm = R(mm), rpm = R(pm), rmp = R(mp), rpp = R(pp);
+        gx = SB(T_RHF) * (-rmm + rpm - rmp + rpp);
+        gy = rhyr * (-rmm - rpm + rmp + rpp);
+        const R norm = A::sqr(gx * gx + gy * gy);
+        const R ninv = A::div(R(1), fmax(norm, A::eps()));
         gx = gx * ninv;
         gy = gy * ninv;
       }
       g2x = gx;
       g2y = gy;
     }
-    const double g2xr = from_right(g2x), g2yr = from_right(g2y);
-    g1xr = from_right(g1x);
-    g1yr = from_right(g1y);
+    const R g2xr = A::right(g2x), g2yr = A::right(g2y);
+    g1xr = A::right(g1x);
+    g1yr = A::right(g1y);
     // -- normals, row s+1 (curvature_tensor.c:51-97) ---------------------------
     n0x = n1x; n0y = n1y; n0z = n1z;
     {
-      double nxv = 0., nyv = 0., nz = 0.;
+      R nxv = 0, nyv = 0, nz = 0;
       if (c_in && in_band(vp1)) {
         nxv = g1x + g1xr + g2x + g2xr;
         nyv = g1y + g1yr + g2y + g2yr;
-        nxv /= TB(T_HXXC);
-        nyv /= hy;
-        const double norm = sqrt(nxv * nxv + nyv * nyv);
-        const double ninv = 1. / fmax(norm, DBL_EPSILON);
+        nxv = A::div(nxv, SB(T_HXXC));
+        nyv = A::div(nyv, hyr);
+        const R norm = A::sqr(nxv * nxv + nyv * nyv);
+        const R ninv = A::div(R(1), fmax(norm, A::eps()));
         nxv *= ninv;
         nyv *= ninv;
-        nz = -0.5 / kVofBeta * log(1. / vp1 - 1.);
+        nz = A::intercept(vp1);
       }
       n1x = nxv;
       n1y = nyv;
@@ -203,33 +299,33 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     // -- curvature, row s (curvature_tensor.c:100-135) -------------------------
     km1 = k0;
     {
-      double kk = 0.;
+      R kk = 0;
       if (c_in) {
-        const double nxm = 0.5 * g0x + 0.5 * g1x;
-        const double nxp = 0.5 * g0xr + 0.5 * g1xr;
-        const double nym = 0.5 * g0y + 0.5 * g0yr;
-        const double nyp = 0.5 * g1y + 0.5 * g1yr;
-        const double div = TB(T_RJC) * (-TB(T_QF) * nxm + TR(T_QF) * nxp - TB(T_CYC) * nym + TB(T_CYC) * nyp);
-        kk = -1. * div;
+        const R nxm = R(0.5) * g0x + R(0.5) * g1x;
+        const R nxp = R(0.5) * g0xr + R(0.5) * g1xr;
+        const R nym = R(0.5) * g0y + R(0.5) * g0yr;
+        const R nyp = R(0.5) * g1y + R(0.5) * g1yr;
+        const R div = SB(T_RJC) * (-SB(T_QF) * nxm + SR(T_QF) * nxp - SB(T_CYC) * nym + SB(T_CYC) * nyp);
+        kk = R(-1) * div;
       }
       k0 = kk;
     }
     // -- THINC and mass fluxes at x face c, row s (fluxx.c, mass_flux.c) -------
     mfxm1 = mfx0;
-    double fx0 = 0.;
+    R fx0 = 0;
     {
       const double vl = from_left(v0);
-      const double nlx = from_left(n0x), nly = from_left(n0y), nlz = from_left(n0z);
-      double mf = 0.;
+      const R nlx = A::left(n0x), nly = A::left(n0y), nlz = A::left(n0z);
+      R mf = 0;
       if (c_fx) {
-        const double vel = ux0;
+        const R vel = rux0;
         if (c != 1 && c != nx + 1) {
-          const bool up = vel < 0.;
+          const bool up = vel < R(0);
           const double v = up ? v0 : vl;
           const V3 n = up ? V3{n0x, n0y, n0z} : V3{nlx, nly, nlz};
-          fx0 = thinc_flux_x(vel, v, in_band(v) ? n : V3{0., 0., 0.});
+          fx0 = thinc_flux<R, true>(vel, v, in_band(v) ? n : V3{0, 0, 0});
         }
-        mf = vel + (a.denr - 1.) * fx0;
+        mf = vel + denr1 * fx0;
       }
       mfx0 = mf;
     }
@@ -237,16 +333,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     fy0 = fyp1;
     mfy0 = mfyp1;
     {
-      double f = 0., mf = 0.;
+      R f = 0, mf = 0;
       if (c_vof) {
-        const double vel = uyp1;
+        const R vel = ruyp1;
         if (c != 0 && c != nx + 1) {
-          const bool up = vel < 0.;
+          const bool up = vel < R(0);
           const double v = up ? vp1 : v0;
           const V3 n = up ? V3{n1x, n1y, n1z} : V3{n0x, n0y, n0z};
-          f = thinc_flux_y(vel, v, in_band(v) ? n : V3{0., 0., 0.});
+          f = thinc_flux<R, false>(vel, v, in_band(v) ? n : V3{0, 0, 0});
         }
-        mf = vel + (a.denr - 1.) * f;
+        mf = vel + denr1 * f;
       }
       fyp1 = f;
       mfyp1 = mf;
@@ -254,42 +350,44 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     // -- viscosities and stress tensor (viscosity.c, predict/txx.c, tyy.c, txy.c)
     // (every DPP shift runs in uniform control flow: an inactive source lane
     // would not deliver its value)
-    const double v0l = from_left(v0), vp1l = from_left(vp1);
-    const double ux0r = from_right(ux0);
-    const double uyp1l = from_left(uyp1);
+    const double v0l = from_left(v0);
+    const R rv0l = R(v0l), rvp1l = A::left(rvp1);
+    const R rux0r = A::right(rux0);
+    const R ruyp1l = A::left(ruyp1);
     visx0 = visx1;
-    visx1 = VISUX(c, vp1l, vp1);     // x face c, row s+1
+    visx1 = VISUX(c, rvp1l, rvp1);   // x face c, row s+1
     visy0 = visy1;
-    visy1 = VISUY(v0, vp1);          // y face s+1
-    const double visx0r = from_right(visx0), visy1l = from_left(visy1);
-    double txx0 = 0.;
+    visy1 = VISUY(rv0, rvp1);        // y face s+1
+    const R visx0r = A::right(visx0), visy1l = A::left(visy1);
+    R txx0 = 0;
     if (c_in) {
-      const double vx = 0.5 * visx0 + 0.5 * visx0r;
-      const double l = TB(T_RHC) * (-ux0 + ux0r);
+      const R vx = R(0.5) * visx0 + R(0.5) * visx0r;
+      const R l = SB(T_RHC) * (-rux0 + rux0r);
       txx0 = vx * l + vx * l;
     }
-    double tyy0 = 0.;
+    R tyy0 = 0;
     if (c_in) {
-      const double vy = 0.5 * visy0 + 0.5 * visy1;
-      const double l = rhy * (-uy0 + uyp1);
+      const R vy = R(0.5) * visy0 + R(0.5) * visy1;
+      const R l = rhyr * (-ruy0 + ruyp1);
       tyy0 = vy * l + vy * l;
     }
-    const double txy_prev = txy0;   // corner row s
-    double txy1 = 0.;
+    const R txy_prev = txy0;   // corner row s
+    R txy1 = 0;
     if (c_fx) {
-      const double vx = 0.5 * visx0 + 0.5 * visx1;
-      const double vy = 0.5 * visy1l + 0.5 * visy1;
-      const double lxy = rhy * (-ux0 + uxp1);
-      const double lyx = TB(T_RHF) * (-uyp1l + uyp1);
+      const R vx = R(0.5) * visx0 + R(0.5) * visx1;
+      const R vy = R(0.5) * visy1l + R(0.5) * visy1;
+      const R lxy = rhyr * (-rux0 + ruxp1);
+      const R lyx = SB(T_RHF) * (-ruyp1l + ruyp1);
       txy1 = vx * lxy + vy * lyx;
     }
-    // -- new vof, row s (update/main.c:54-120) ---------------------------------
+    // -- new vof, row s (update/main.c:54-120), FP64 ---------------------------
     double src = 0.;
-    const double fx0r = from_right(fx0);
+    const double fx0d = double(fx0);
+    const double fx0r = A::right(fx0);
     {
       double v = 0.;
       if (c_in) {
-        src = TB(T_RJC) * (+TB(T_QF) * fx0 - TR(T_QF) * fx0r + TB(T_CYC) * fy0 - TB(T_CYC) * fyp1);
+        src = TB(T_RJC) * (+TB(T_QF) * fx0d - TR(T_QF) * fx0r + TB(T_CYC) * double(fy0) - TB(T_CYC) * double(fyp1));
         v = v0;
         v += dtca * src;
         if (has_b) v += dtcb * sbv;
@@ -307,61 +405,65 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     const double d1x = 0.5 * dn0l + 0.5 * dn0;    // x face c, row s, new vof
     d1y = 0.5 * dnm1 + 0.5 * dn0;                 // y face s, new vof
     // -- momentum fluxes shared between neighbouring faces ---------------------
-    const double mfx0r = from_right(mfx0), mfyp1l = from_left(mfyp1);
-    const double uy0l = from_left(uy0);
+    const R mfx0r = A::right(mfx0), mfyp1l = A::left(mfyp1);
+    const R ruy0l = A::left(ruy0);
     // ux, x direction: cell-centre flux (mp * up) of the cell right of face c
-    const double qx = (TB(T_EQF) * mfx0 + TR(T_EQF) * mfx0r) * (0.5 * ux0 + 0.5 * ux0r);
+    const R qx = (SB(T_EQF) * mfx0 + SR(T_EQF) * mfx0r) * (R(0.5) * rux0 + R(0.5) * rux0r);
     // uy, x direction: corner flux (mm * um) at face c
-    const double qm = (TB(T_EQF) * mfxm1 + TB(T_EQF) * mfx0) * (0.5 * uy0l + 0.5 * uy0);
-    const double qxl = from_left(qx), qmr = from_right(qm);
-    const double qt = TB(T_QC) * txx0;
-    const double qtl = from_left(qt);
-    const double qf_txy0 = TB(T_QF) * txy_prev;
-    const double qf_txy0r = from_right(qf_txy0);
-    const double p0l = from_left(p0), k0l = from_left(k0);
+    const R qm = (SB(T_EQF) * mfxm1 + SB(T_EQF) * mfx0) * (R(0.5) * ruy0l + R(0.5) * ruy0);
+    const R qxl = A::left(qx), qmr = A::right(qm);
+    const R qt = SB(T_QC) * txx0;
+    const R qtl = A::left(qt);
+    const R qf_txy0 = SB(T_QF) * txy_prev;
+    const R qf_txy0r = A::right(qf_txy0);
+    const double p0l = from_left(p0);
+    const R k0l = A::left(k0);
     // ux, y direction: corner flux (mp * up) at face s+1
-    const double qy_ux_new = (TL(T_ECY) * mfyp1l + TB(T_ECY) * mfyp1) * (0.5 * ux0 + 0.5 * uxp1);
+    const R qy_ux_new = (SL(T_ECY) * mfyp1l + SB(T_ECY) * mfyp1) * (R(0.5) * rux0 + R(0.5) * ruxp1);
     // uy, y direction: cell-centre flux (mp * up) of row s
-    const double qy_uy_new = (TB(T_ECY) * mfy0 + TB(T_ECY) * mfyp1) * (0.5 * uy0 + 0.5 * uyp1);
-    const double cyf_txy1 = TB(T_CYF) * txy1;
-    const double cyc_tyy0 = TB(T_CYC) * tyy0;
+    const R qy_uy_new = (SB(T_ECY) * mfy0 + SB(T_ECY) * mfyp1) * (R(0.5) * ruy0 + R(0.5) * ruyp1);
+    const R cyf_txy1 = SB(T_CYF) * txy1;
+    const R cyc_tyy0 = SB(T_CYC) * tyy0;
     // -- predicted ux, row s (predict/ux.c, force.c) ---------------------------
     double uxs = 0.;
     if (c >= 2 && c <= nx) {
-      double s_ = 0.;
-      s_ -= TB(T_RJF) * (-qxl + qx);
-      s_ -= TB(T_RJF) * (-qy_ux + qy_ux_new);
-      s_ += TB(T_DFX) * (-qtl + qt);
-      s_ += TB(T_DFX) * (-cyf_txy0 + cyf_txy1);
+      R s_ = 0;
+      s_ -= SB(T_RJF) * (-qxl + qx);
+      s_ -= SB(T_RJF) * (-qy_ux + qy_ux_new);
+      s_ += SB(T_DFX) * (-qtl + qt);
+      s_ += SB(T_DFX) * (-cyf_txy0 + cyf_txy1);
       double g = 0.;
       g -= TB(T_RHF) * (-p0l + p0);
+      const R rdfx = R(dfx);
       {
-        const double k = 0.5 * k0l + 0.5 * k0;
-        const double delta = TB(T_RHF) * (-v0l + v0);
-        s_ += dfx * a.refdeninv * a.tension * k * delta;
+        const R k = R(0.5) * k0l + R(0.5) * k0;
+        const R delta = SB(T_RHF) * (-rv0l + rv0);
+        s_ += rdfx * refdeninv * tension * k * delta;
       }
-      s_ += a.grav * dfx;
-      if (a.srca_ux && outrow && c_out) fx_row(a.srca_ux, s)[lane] = s_;
-      uxs = dfx * ux0 + a.ca * dt * s_ + a.cb * dt * sbx + a.cg * dt * g;
+      s_ += grav * rdfx;
+      const double sd = double(s_);
+      if (a.srca_ux && outrow && c_out) fx_row(a.srca_ux, s)[lane] = sd;
+      uxs = dfx * ux0 + a.ca * dt * sd + a.cb * dt * sbx + a.cg * dt * g;
       uxs /= d1x;
     }
     // -- predicted uy, row s (predict/uy.c, force.c) ---------------------------
     double uys = 0.;
     if (c_in) {
-      double s_ = 0.;
-      s_ -= TB(T_RJC) * (-qm + qmr);
-      s_ -= TB(T_RJC) * (-qy_uy + qy_uy_new);
-      s_ += TB(T_DCX) * (-qf_txy0 + qf_txy0r);
-      s_ += TB(T_DCX) * (-cyc_tyym1 + cyc_tyy0);
+      R s_ = 0;
+      s_ -= SB(T_RJC) * (-qm + qmr);
+      s_ -= SB(T_RJC) * (-qy_uy + qy_uy_new);
+      s_ += SB(T_DCX) * (-qf_txy0 + qf_txy0r);
+      s_ += SB(T_DCX) * (-cyc_tyym1 + cyc_tyy0);
       double g = 0.;
       g -= rhy * (-pm1 + p0);
       {
-        const double k = 0.5 * km1 + 0.5 * k0;
-        const double delta = rhy * (-vm1 + v0);
-        s_ += dfy * a.refdeninv * a.tension * k * delta;
+        const R k = R(0.5) * km1 + R(0.5) * k0;
+        const R delta = rhyr * (-rvm1 + rv0);
+        s_ += R(dfy) * refdeninv * tension * k * delta;
       }
-      if (a.srca_uy && outrow && c_out) ci_row(a.srca_uy, s)[lane] = s_;
-      uys = dfy * uy0 + a.ca * dt * s_ + a.cb * dt * sby + a.cg * dt * g;
+      const double sd = double(s_);
+      if (a.srca_uy && outrow && c_out) ci_row(a.srca_uy, s)[lane] = sd;
+      uys = dfy * uy0 + a.ca * dt * sd + a.cb * dt * sby + a.cg * dt * g;
       uys /= d1y;
     }
     qy_ux = qy_ux_new;
@@ -370,7 +472,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     cyc_tyym1 = cyc_tyy0;
     txy0 = txy1;
     // -- outputs of row s -------------------------------------------------------
-    const double txy0r = from_right(txy_prev);
+    const double txy0r = double(A::right(txy_prev));
     if (outrow && c_out) {
       auto put = [&](double* f, int nh, auto row, double v) {
         row(f, s)[lane] = v;
@@ -401,11 +503,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
         if (c == nx) put_at(a.den1, 1, rc, 1, dg);
       }
       if (a.txy) {
-        put(a.txy, 1, rx, txy_prev);
+        put(a.txy, 1, rx, double(txy_prev));
         if (c == nx) put_at(a.txy, 1, rx, 1, txy0r);
       }
     }
-    // -- Poisson right-hand side of row s-1 (compute_potential.c:315-360) ----
+    // -- Poisson right-hand side of row s-1 (compute_potential.c:315-360), FP64
     {
       // u* divergence: the x-face terms of cell c+1 come from lane c+1
       const double q_uxs = q_uxs_m1;                       // QF * ux* at face c, row s-1
@@ -433,6 +535,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     q_uxs_m1 = TB(T_QF) * uxs;
     d1x_m1 = d1x;
   }
+}
+
+// the FP64 body needs 256 VGPRs (2 wavefronts per SIMD); the FP32 body
+// needs about 200, and forcing 3 or 4 wavefronts per SIMD spills to scratch,
+// which measured 1.7x and 3x slower, so both run 2 per SIMD
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_march(MarchArgs a) {
+  march_body<double>(a);
+}
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SBFS_MIXED_WAVES))) void k_march_mixed(
+    MarchArgs a) {
+  march_body<float>(a);
 }
 
 }  // namespace sbfs

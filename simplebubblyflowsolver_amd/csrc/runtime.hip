@@ -405,6 +405,13 @@ int sbfs_set_gauge(int gauge) {
   return 0;
 }
 
+int sbfs_set_precision(int precision) {
+  SBFS_REQUIRE(precision == SBFS_PRECISION_FP64 || precision == SBFS_PRECISION_MIXED,
+               "unknown precision %d", precision);
+  options().precision = precision;
+  return 0;
+}
+
 int sbfs_set_tdm(int kind, int chunk) {
   SBFS_REQUIRE(kind == SBFS_TDM_SEQUENTIAL || kind == SBFS_TDM_PARTITIONED, "unknown tdm kind %d", kind);
   SBFS_REQUIRE(chunk >= 2 && chunk <= 64 && (chunk & (chunk - 1)) == 0,

@@ -131,6 +131,7 @@ struct Options {
   int gauge = 0;       // SBFS_GAUGE_PINNED
   int tdm = 1;         // SBFS_TDM_PARTITIONED
   int chunk = 32;
+  int precision = 0;   // SBFS_PRECISION_FP64
   bool coefs_set = false;
   double cadv = 0., cdif = 0., cint = 0.;
 };
