@@ -114,6 +114,37 @@ def cpu_baseline(w: configs.Workload, steps: int, threads: int) -> dict:
                       f"CPU oracle (FP64 C restatement, OpenMP, {threads} threads), {t:.1f} s"}
 
 
+def mixed_run(be, w, warmup, steps, t_fp64, d0, d1_fp64) -> dict:
+    """The same workload from the same initial state with the mixed-precision
+    stage kernel: throughput, speedup over the FP64 loop just timed, and the
+    rise-velocity and gas-volume errors after the same number of steps."""
+    lib = be.lib
+    be.check(lib.sbfs_set_precision(1), "set_precision")
+    try:
+        g = solver.Case(be, w.nx, w.ny, w.lx, w.ly, w.Re, w.Fr, w.denr, w.visr, w.We)
+        g.load_initial_condition(w.vof())
+        for _ in range(warmup):
+            g.integrate()
+        be.check(lib.sbfs_synchronize(), "sync")
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            g.integrate()
+        be.check(lib.sbfs_synchronize(), "sync")
+        t = time.perf_counter() - t0
+        d = g.diagnostics()
+        g.close()
+    finally:
+        be.check(lib.sbfs_set_precision(0), "set_precision")
+    ur = d1_fp64["rise_velocity"]
+    return {"value": round(w.cells * steps / t / 1e6, 3), "ms_per_step": round(t / steps * 1e3, 4),
+            "speedup_vs_fp64": round(t_fp64 / t, 3),
+            "rise_velocity_rel_error": abs(d["rise_velocity"] - ur) / max(abs(ur), 1e-300),
+            "volume_drift": abs(d["volume"] - d0["volume"]) / d0["volume"], "divmax": d["divmax"],
+            "precision": "FP32 interface and momentum stencils; FP64 vof / velocity updates, "
+                         "Poisson solve, projection and all fields in memory",
+            "learned_curvature": None}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +159,8 @@ def main() -> int:
     ap.add_argument("--copy-mib", type=int, default=1024,
                     help="buffer size of the copy-bandwidth measurement (0: skip)")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-mixed", dest="mixed", action="store_false",
+                    help="skip the mixed-precision comparison run")
     ap.add_argument("--profile-steps", type=int, default=20,
                     help="steps of the separate per-kernel timing pass")
     a = ap.parse_args()
@@ -154,6 +187,9 @@ def main() -> int:
     be.check(lib.sbfs_synchronize(), "sync")
     t = time.perf_counter() - t0
     d1 = g.diagnostics()
+    mixed = None
+    if a.mixed and a.precision == "fp64" and a.path == "fused":
+        mixed = mixed_run(be, w, a.warmup, a.steps, t, d0, d1)
     # per-kernel times: a separate pass with HIP events around every launch
     # (the events keep the step out of graph replay, so it is not the timed loop)
     prof = []
@@ -211,6 +247,7 @@ def main() -> int:
                    "divmax": d1["divmax"]},
         "roofline": roof,
         "kernels": kernels,
+        "mixed": mixed,
     }
     if not a.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
