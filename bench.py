@@ -44,7 +44,11 @@ HBM_PEAK_GBS = 8000.0
 # ux*, uy*, rhs and the three alpha sources = 12 fields; stage 1: + the
 # three beta sources = 15; stage 2: 8 reads, writes vof', ux*, uy*, rhs,
 # den[1], txy = 14), so its entry is the mean over the three launches of a
-# step.  The staged kernels follow the reference's stage boundaries.
+# step.  With the previous stage's projection folded into stages 1 and 2
+# (k_march_corr), k_march is stage 0 alone (12 fields) and k_march_corr
+# also reads the previous potential and writes the pressure (stage 1: 9
+# reads + 8 writes = 17; stage 2: 9 + 7 = 16; mean 16.5).  The staged
+# kernels follow the reference's stage boundaries.
 BYTES_PER_CELL = {
     "k_density": 16, "k_visux": 16, "k_visuy": 16, "k_dvof": 24, "k_normal_curv": 56,
     "k_force": 40, "k_fluxx": 48, "k_fluxy": 48, "k_vof_advance": 48, "k_mass_flux": 48,
@@ -54,8 +58,16 @@ BYTES_PER_CELL = {
     "k_update_p": 24, "k_dt_partial": 16, "k_dt_final": 0,
     # fused path
     "k_march": (12 + 15 + 14) * 8 / 3, "k_march_mixed": (12 + 15 + 14) * 8 / 3, "k_pre": (10 + 13 + 12) * 8 / 3, "k_dct2_rows": 16,
-    "k_post": 72,
+    "k_post": 72, "k_march_corr": (17 + 16) * 8 / 2, "k_march_mixed_corr": (17 + 16) * 8 / 2,
 }
+
+
+def bytes_per_cell(name: str, names) -> float | None:
+    """BYTES_PER_CELL, with k_march counted as stage 0 alone when the
+    folded-projection kernels ran beside it."""
+    if name in ("k_march", "k_march_mixed") and (name + "_corr") in names:
+        return 12 * 8
+    return BYTES_PER_CELL.get(name)
 
 # HBM bytes per launch measured with rocprofv3 counters (FETCH_SIZE and
 # WRITE_SIZE, scaled by a copy kernel of known size), read from the
@@ -212,15 +224,16 @@ def main() -> int:
     kernels = []
     if prof:
         total = sum(e[2] for e in prof)
+        names = {e[0] for e in prof}
         for n, cnt, ms in prof:
-            bpc = BYTES_PER_CELL.get(n)
+            bpc = bytes_per_cell(n, names)
             avg_s = ms / cnt / 1e3
             kernels.append({"name": n, "launches": cnt, "total_ms": round(ms, 3),
                             "avg_us": round(avg_s * 1e6, 2),
                             "alg_gbs": round(bpc * w.cells / avg_s / 1e9, 1) if bpc else None})
         name, cnt, ms = max(prof, key=lambda e: e[2])
         avg_s = ms / cnt / 1e3
-        bpc = BYTES_PER_CELL[name]
+        bpc = bytes_per_cell(name, names)
         alg_bytes = bpc * w.cells
         achieved = alg_bytes / avg_s / 1e9
         traffic, src = measured_traffic(name, w.name)

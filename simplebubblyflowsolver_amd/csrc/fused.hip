@@ -494,24 +494,26 @@ constexpr int kTX = 64, kTY = 4;
 struct FusedScratch {
   size_t nx = 0, ny = 0;
   double* vof = nullptr;
-  double* ux = nullptr;
+  double* ux = nullptr;   // predicted velocity of stages 0 and 2
   double* uy = nullptr;
+  double* ux2 = nullptr;  // predicted velocity of stage 1 (folded projection)
+  double* uy2 = nullptr;
+  double* p = nullptr;    // pressure written by a stage kernel with the folded projection
 };
 static FusedScratch g_scr;
 
 static int scratch(const domain_t* d, const interface_t* s, const fluid_t* f) {
   if (g_scr.nx == d->mysizes[0] && g_scr.ny == d->mysizes[1]) return 0;
   (void)hipStreamSynchronize(stream());
-  (void)hipFree(g_scr.vof);
-  (void)hipFree(g_scr.ux);
-  (void)hipFree(g_scr.uy);
+  for (double* q : {g_scr.vof, g_scr.ux, g_scr.uy, g_scr.ux2, g_scr.uy2, g_scr.p}) (void)hipFree(q);
   g_scr = FusedScratch();
-  SBFS_HIP(hipMalloc(&g_scr.vof, s->vof.datasize));
-  SBFS_HIP(hipMalloc(&g_scr.ux, f->ux.datasize));
-  SBFS_HIP(hipMalloc(&g_scr.uy, f->uy.datasize));
-  SBFS_HIP(hipMemset(g_scr.vof, 0, s->vof.datasize));
-  SBFS_HIP(hipMemset(g_scr.ux, 0, f->ux.datasize));
-  SBFS_HIP(hipMemset(g_scr.uy, 0, f->uy.datasize));
+  const std::pair<double**, size_t> bufs[] = {{&g_scr.vof, s->vof.datasize}, {&g_scr.ux, f->ux.datasize},
+                                              {&g_scr.uy, f->uy.datasize},   {&g_scr.ux2, f->ux.datasize},
+                                              {&g_scr.uy2, f->uy.datasize},  {&g_scr.p, f->p.datasize}};
+  for (const auto& b : bufs) {
+    SBFS_HIP(hipMalloc(b.first, b.second));
+    SBFS_HIP(hipMemset(*b.first, 0, b.second));
+  }
   g_scr.nx = d->mysizes[0];
   g_scr.ny = d->mysizes[1];
   return 0;
@@ -602,8 +604,24 @@ static void swap_data(array_t& x, array_t& y) {
   y.data = t;
 }
 
-__global__ void k_march(MarchArgs a);        // march.hip, plain FP64
-__global__ void k_march_mixed(MarchArgs a);  // march.hip, FP32 stencils
+__global__ void k_march(MarchArgs a);             // march.hip, plain FP64
+__global__ void k_march_corr(MarchArgs a);        // + the previous stage's projection
+__global__ void k_march_mixed(MarchArgs a);       // FP32 stencils
+__global__ void k_march_mixed_corr(MarchArgs a);  // FP32 stencils + projection
+
+// Stages 1 and 2 fold the previous stage's projection and pressure update
+// into their stage kernel (k_march_corr), which reads the predicted velocity,
+// both potentials and the pressure, corrects each row as it enters its rings
+// and writes the new pressure; only the last stage keeps its k_post.
+// SBFS_MARCH_FOLD=0 runs k_post after every stage instead.
+static bool march_fold() {
+  static int on = -1;
+  if (on < 0) {
+    const char* e = getenv("SBFS_MARCH_FOLD");
+    on = (e && e[0] == '0') ? 0 : 1;
+  }
+  return on == 1;
+}
 
 // Rows per band of k_march.  Every wavefront marches its band plus kWarm + 2
 // rows of ring fill, and a wavefront lives for the whole band, so the grid
@@ -645,7 +663,13 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
   const int nx = (int)c->nx, ny = (int)c->ny;
   if (compute_dt_device(domain, c, fluid, interface)) return 1;
   const double* dtp = c->dt_dev;
+  const bool fold = !use_tiles && march_fold();
+  // predicted velocity of each stage: with the fold, stage 1 writes the
+  // second pair while it reads the first
+  double* const us[3] = {g_scr.ux, fold ? g_scr.ux2 : g_scr.ux, g_scr.ux};
+  double* const vs[3] = {g_scr.uy, fold ? g_scr.uy2 : g_scr.uy, g_scr.uy};
   for (int k = 0; k < 3; k++) {
+    const bool corr = fold && k > 0;  // this stage applies stage k-1's projection
     if (k != 0) {
       swap_data(interface->src[0], interface->src[1]);
       swap_data(fluid->srcux[0], fluid->srcux[1]);
@@ -661,8 +685,8 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
     a.srcb_ux = D(fluid->srcux[1]);
     a.srcb_uy = D(fluid->srcuy[1]);
     a.vofn = g_scr.vof;
-    a.uxs = g_scr.ux;
-    a.uys = g_scr.uy;
+    a.uxs = us[k];
+    a.uys = vs[k];
     // the alpha sources of the last stage are never read again (beta of
     // stage 0 is zero), so they are not written
     a.srca_vof = k < 2 ? D(interface->src[0]) : nullptr;
@@ -701,10 +725,14 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
     } else {
       MarchArgs m;
       m.vof = a.vof;
-      m.ux = a.ux;
-      m.uy = a.uy;
+      m.ux = corr ? us[k - 1] : a.ux;
+      m.uy = corr ? vs[k - 1] : a.uy;
       m.p = a.p;
       m.psi0 = D(fluid->psi[0]);
+      m.psi_old = corr ? D(fluid->psi[1]) : nullptr;
+      m.p_out = corr ? g_scr.p : nullptr;
+      m.pg_old = rk((k + 1) % 3).g;  // gammas of stage k-1's dt_old / dt_new
+      m.pg_new = rk((k + 2) % 3).g;
       m.srcb_vof = a.srcb_vof;
       m.srcb_ux = a.srcb_ux;
       m.srcb_uy = a.srcb_uy;
@@ -743,11 +771,22 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
       m.ny = ny;
       m.band = band;
       const dim3 grid(cdiv(nx, kStrip), cdiv(ny, band)), block(64);
-      if (options().precision == SBFS_PRECISION_MIXED)
+      const bool mixed = options().precision == SBFS_PRECISION_MIXED;
+      if (mixed && corr)
+        SBFS_TIMED("k_march_mixed_corr", hipLaunchKernelGGL(k_march_mixed_corr, grid, block, 0, stream(), m));
+      else if (mixed)
         SBFS_TIMED("k_march_mixed", hipLaunchKernelGGL(k_march_mixed, grid, block, 0, stream(), m));
+      else if (corr)
+        SBFS_TIMED("k_march_corr", hipLaunchKernelGGL(k_march_corr, grid, block, 0, stream(), m));
       else
         SBFS_TIMED("k_march", hipLaunchKernelGGL(k_march, grid, block, 0, stream(), m));
       if (check_launch("integrate: k_march")) return 1;
+      if (corr) {
+        // the stage kernel wrote the updated pressure of stage k-1
+        double* t = (double*)fluid->p.data;
+        fluid->p.data = g_scr.p;
+        g_scr.p = t;
+      }
     }
     {
       // the new vof becomes the state; the old one is the next scratch
@@ -761,9 +800,14 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
     } else if (poisson_transform_fused(domain, c, fluid, nullptr)) {
       return 1;
     }
+    if (fold && k < 2) {
+      // projected by the next stage's kernel
+      swap_data(fluid->psi[0], fluid->psi[1]);
+      continue;
+    }
     PostArgs q;
-    q.uxs = g_scr.ux;
-    q.uys = g_scr.uy;
+    q.uxs = us[k];
+    q.uys = vs[k];
     q.psi1 = D(fluid->psi[1]);
     q.psi0 = D(fluid->psi[0]);
     q.vofn = D(interface->vof);
@@ -828,6 +872,10 @@ static StepKey step_key(const DomainCtx* c, fluid_t* f, interface_t* s, const Co
   key_add(k, g_scr.vof);
   key_add(k, g_scr.ux);
   key_add(k, g_scr.uy);
+  key_add(k, g_scr.ux2);
+  key_add(k, g_scr.uy2);
+  key_add(k, g_scr.p);
+  k.v.push_back(march_fold() ? 1 : 0);
   key_add(k, cf.rhf);
   key_add(k, rhs);
   for (double x : {f->Re, f->Fr, f->denr, f->visr, f->refden, s->We, options().cadv, options().cdif,

@@ -23,14 +23,18 @@
 // per-column quotients come from the coefficient tables), which keeps the
 // result bitwise equal to the staged kernels.
 #include <cfloat>
+#include <type_traits>
 #include <cmath>
 
 #include "march.hpp"
 #include "sbfs_internal.hpp"
 #include "stencils.hpp"
 
+#ifndef SBFS_FP64_WAVES
+#define SBFS_FP64_WAVES 2
+#endif
 #ifndef SBFS_MIXED_WAVES
-#define SBFS_MIXED_WAVES 2
+#define SBFS_MIXED_WAVES 3
 #endif
 
 namespace sbfs {
@@ -50,6 +54,38 @@ __device__ __forceinline__ double from_right(double v) {
   lo = __builtin_amdgcn_update_dpp(0, lo, 0x130, 0xf, 0xf, false);  // wave_shl:1
   hi = __builtin_amdgcn_update_dpp(0, hi, 0x130, 0xf, 0xf, false);
   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// The element of a wave-uniform row at the lane's byte offset.  The row
+// address is pinned to SGPRs (readfirstlane of a uniform value), so every
+// access takes the scalar-base + 32-bit vector-offset form and all arrays
+// share one offset VGPR; left to itself the compiler hoists a 64-bit per-lane
+// pointer per array out of the row loop, two VGPRs each.
+template <typename T>
+__device__ __forceinline__ T& lane_at(T* row, unsigned loff) {
+  const unsigned long long u = (unsigned long long)row;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+  // a global-address-space pointer, so the accesses stay global_load /
+  // global_store (a generic pointer made from an integer would compile to
+  // flat accesses, which also count against the LDS wait counter)
+  using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+  using GB = __attribute__((address_space(1))) B;
+  GB* base = (GB*)(((unsigned long long)hi << 32) | lo);
+  return *(T*)(base + loff);
+}
+
+// A wave-uniform value pinned to SGPRs.  FP64 arithmetic has no scalar
+// form, so a constant such as dt * gamma is formed on the vector unit and
+// would otherwise hold a VGPR pair for the whole kernel.
+__device__ __forceinline__ double uni(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readfirstlane((int)b);
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ float uni(float x) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
 }
 
 __device__ __forceinline__ int wrap_r(int j, int ny) {
@@ -140,7 +176,7 @@ __device__ __forceinline__ R mixr(double ratio, double lo, double hi, R v) {
 // momentum sources (viscosities, stresses, advection, surface tension,
 // buoyancy) in FP32; the vof update, the density-weighted velocity update,
 // the pressure gradient and the Poisson right-hand side stay FP64.
-template <typename R>
+template <typename R, bool CORR>
 __device__ __forceinline__ void march_body(const MarchArgs& a) {
   using A = Arith<R>;
   using V3 = V3R<R>;
@@ -163,6 +199,15 @@ __device__ __forceinline__ void march_body(const MarchArgs& a) {
 #pragma unroll
     for (int t = 0; t < T_N; t++) st[t][lane] = tabs[t][cc];
   }
+  // The folded projection needs more VGPRs than the plain body has left, so
+  // that variant keeps its older rows of the Poisson right-hand side, the
+  // potential, the pressure and the new density in LDS instead (each lane
+  // reads back only what it wrote, so no barrier is needed).
+  constexpr bool LR = CORR;
+  enum { RG_QUXS, RG_CYCUYS, RG_D1X, RG_CGY, RG_PSM1, RG_PM1, RG_DNM1, RG_N };
+  __shared__ double rings[LR ? RG_N : 1][64];
+  if (LR)
+    for (int k = 0; k < RG_N; k++) rings[k][lane] = 0.;
   __syncthreads();
   const int ll = lane > 0 ? lane - 1 : 0, lr = lane < 63 ? lane + 1 : 63;
   // tables in FP64 (the FP64 parts) and in the stencil type
@@ -172,15 +217,21 @@ __device__ __forceinline__ void march_body(const MarchArgs& a) {
   auto SL = [&](int t) { return R(st[t][ll]); };
   auto SR = [&](int t) { return R(st[t][lr]); };
   const double hy = a.hy, rhy = a.cf.rhy;
-  const R hyr = R(hy), rhyr = R(rhy);
-  const double dt = *a.dt;
-  const double dtca = dt * a.ca, dtcb = dt * a.cb;
+  const R hyr = uni(R(hy)), rhyr = uni(R(rhy));
+  const double dt = uni(*a.dt);
+  // (dt * a.ca equals a.ca * dt bit for bit)
+  const double dtca = uni(dt * a.ca), dtcb = uni(dt * a.cb), dtcg = uni(dt * a.cg);
   const double dt_old = a.g_old * dt, dt_new = a.g_new * dt;
-  const double c0 = 1. / a.norm / dt_new * a.refden;
-  const double c1 = 1. / a.norm * -1. / dt_new * dt_old;
+  const double c0 = uni(1. / a.norm / dt_new * a.refden);
+  const double c1 = uni(1. / a.norm * -1. / dt_new * dt_old);
   const bool has_b = a.stage != 0;
-  const R denr1 = R(a.denr) - R(1);
-  const R refdeninv = R(a.refdeninv), tension = R(a.tension), grav = R(a.grav);
+  const R denr1 = uni(R(a.denr) - R(1));
+  // the previous stage's projection (correct_velocity/ux.c, uy.c): the same
+  // expressions as k_post, with the lane's face distance in the divisions
+  const double pdtn = a.pg_new * dt, pdto = a.pg_old * dt;
+  const double pcoef = -1. / pdtn * pdto;
+  const double pdtc = uni(pdtn * pcoef), irefden = uni(1. / a.refden);
+  const R refdeninv = uni(R(a.refdeninv)), tension = uni(R(a.tension)), grav = uni(R(a.grav));
 
   // Row pointers are uniform and shifted to lane 0's column, so every access
   // is a scalar base plus the lane's byte offset.
@@ -190,9 +241,18 @@ __device__ __forceinline__ void march_body(const MarchArgs& a) {
   auto xf_row = [&](auto* f, int j) { return f + (ptrdiff_t)j * wf + cl - 1; };  // L::xf
   auto ci_row = [&](auto* f, int j) { return f + (ptrdiff_t)(j - 1) * nx + cl - 1; };  // L::ci
   auto fx_row = [&](auto* f, int j) { return f + (ptrdiff_t)(j - 1) * (nx - 1) + cl - 2; };  // L::fx
-  auto ld_vof = [&](int j) { return c_vof ? vof_row(a.vof, wrap_r(j, ny))[lane] : 0.; };
-  auto ld_ux = [&](int j) { return c_fx ? xf_row(a.ux, wrap_r(j, ny))[lane] : 0.; };
-  auto ld_cc = [&](const double* f, int j) { return c_vof ? cc_row(f, wrap_r(j, ny))[lane] : 0.; };
+  const unsigned loff = (unsigned)lane * 8u;
+  // Pinned (scalar-base) addressing frees the VGPRs the folded projection
+  // and the three-wavefront FP32 body need; the plain FP64 body is faster
+  // with the compiler's own per-lane pointers.
+  constexpr bool kPinned = CORR || std::is_same<R, float>::value;
+  auto AT = [&](auto* row) -> auto& {
+    if constexpr (kPinned) return lane_at(row, loff);
+    else return row[lane];
+  };
+  auto ld_vof = [&](int j) { return c_vof ? AT(vof_row(a.vof, wrap_r(j, ny))) : 0.; };
+  auto ld_ux = [&](int j) { return c_fx ? AT(xf_row(a.ux, wrap_r(j, ny))) : 0.; };
+  auto ld_cc = [&](const double* f, int j) { return c_vof ? AT(cc_row(f, wrap_r(j, ny))) : 0.; };
   auto DEN = [&](double v) { return mix_clamp(a.denr, a.dlo, a.dhi, v); };
   // viscosity at x face c from the vof on its two sides (viscosity.c:37-53)
   auto VISUX = [&](int col, R vl, R vr) {
@@ -209,9 +269,10 @@ __device__ __forceinline__ void march_body(const MarchArgs& a) {
   // column or row (the right face of cell c is the left face of cell c+1);
   // they are formed once and taken from the neighbouring lane (DPP) or from
   // the previous step, which gives the same bits.
-  double vm1 = 0., v0 = 0., vp1 = 0., vp2 = 0.;             // vof s-1..s+2
+  double v0 = 0., vp1 = 0., vp2 = 0.;                       // vof s..s+2
+  R rvm1 = 0;                                               // vof s-1, stencil type
   R g0x = 0, g0y = 0, g1x = 0, g1y = 0, g2x = 0, g2y = 0;   // corners s..s+2
-  R g0xr = 0, g0yr = 0, g1xr = 0, g1yr = 0;                 // corners of c+1, rows s, s+1
+  R g1xr = 0, g1yr = 0;                                     // corners of c+1, row s+1
   R n0x = 0, n0y = 0, n0z = 0, n1x = 0, n1y = 0, n1z = 0;   // normals s, s+1
   R km1 = 0, k0 = 0;                                        // curvature s-1, s
   R mfxm1 = 0, mfx0 = 0;                                    // x mass fluxes s-1, s
@@ -222,43 +283,61 @@ __device__ __forceinline__ void march_body(const MarchArgs& a) {
   double vn0 = 0., dnm1 = 0., dn0 = 0.;                     // new vof row s, its density s-1, s
   double ux0 = 0., uxp1 = 0., uy0 = 0., uyp1 = 0.;
   double pm1 = 0., p0 = 0.;
-  double psm1 = 0., ps0 = 0.;                               // psi[0] s-1, s
+  double psm1 = 0., ps0 = 0., psp1 = 0.;                    // psi[0] s-1..s+1 (s+1: folded projection)
+  double pop1 = 0.;                                         // previous potential s+1
   R qy_ux = 0, qy_uy = 0;                                   // y momentum fluxes at face s
   double q_uxs_m1 = 0., cyc_uys_m1 = 0.;                    // rhs terms of row s-1
   double d1x_m1 = 0., d1y = 0., cg_ym = 0.;                 // rhs densities / y gradient
-  const int s_first = jb - kWarm;
-  // prefetched inputs of the next step
+  const int s_first = jb - (CORR ? kWarmCorr : kWarm);
+  const double pcx0 = uni(pdtn / a.refden), pcy = uni(pdtn / a.refden / hy);
+  auto ring_ld = [&](int k, double reg) { return LR ? rings[k][lane] : reg; };
+  auto ring_st = [&](int k, double& reg, double v) {
+    if (LR) rings[k][lane] = v;
+    else reg = v;
+  };
+  // prefetched inputs of the next step (the folded projection needs psi one
+  // row further ahead)
   double nv = ld_vof(s_first + 2), nux = ld_ux(s_first + 1), nuy = ld_cc(a.uy, s_first + 1);
-  double np = ld_cc(a.p, s_first), nps = ld_cc(a.psi0, s_first);
+  double np = ld_cc(a.p, s_first), nps = ld_cc(a.psi0, CORR ? s_first + 1 : s_first);
+  double npo = CORR ? ld_cc(a.psi_old, s_first + 1) : 0.;
   for (int s = s_first; s <= je + 1; s++) {
     // -- new input rows -----------------------------------------------------
-    vm1 = v0; v0 = vp1; vp1 = vp2; vp2 = nv;
+    rvm1 = R(v0); v0 = vp1; vp1 = vp2; vp2 = nv;
     ux0 = uxp1; uxp1 = nux;
     uy0 = uyp1; uyp1 = nuy;
-    pm1 = p0; p0 = np;
-    psm1 = ps0; ps0 = nps;
+    if (!LR) pm1 = p0;
+    p0 = np;
+    if (CORR) {
+      ps0 = psp1; psp1 = nps;   // row s-1 of the potential is in LDS
+      pop1 = npo;
+    } else {
+      psm1 = ps0; ps0 = nps;
+    }
     const bool outrow = s >= jb && s <= je;
     double sbv = 0., sbx = 0., sby = 0.;
     if (has_b && c_in) {
       // the new vof of every marched row needs its beta source; the
       // predicted uy of row je+1 enters the Poisson rhs of row je
       const int sw = wrap_r(s, ny);
-      sbv = ci_row(a.srcb_vof, sw)[lane];
-      if (s >= jb) sby = ci_row(a.srcb_uy, sw)[lane];
-      if (outrow && c >= 2) sbx = fx_row(a.srcb_ux, s)[lane];
+      sbv = AT(ci_row(a.srcb_vof, sw));
+      if (s >= jb) sby = AT(ci_row(a.srcb_uy, sw));
+      if (outrow && c >= 2) sbx = AT(fx_row(a.srcb_ux, s));
     }
     if (s + 1 <= je + 1) {
       nv = ld_vof(s + 3);
       nux = ld_ux(s + 2);
       nuy = ld_cc(a.uy, s + 2);
       np = ld_cc(a.p, s + 1);
-      nps = ld_cc(a.psi0, s + 1);
+      nps = ld_cc(a.psi0, CORR ? s + 2 : s + 1);
+      if (CORR) npo = ld_cc(a.psi_old, s + 2);
     }
     const R rux0 = R(ux0), ruxp1 = R(uxp1), ruy0 = R(uy0), ruyp1 = R(uyp1);
-    const R rv0 = R(v0), rvp1 = R(vp1), rvm1 = R(vm1);
+    const R rv0 = R(v0), rvp1 = R(vp1);
     // -- corner gradients, row s+2 (curvature_tensor.c:16-49) ----------------
     g0x = g1x; g0y = g1y; g1x = g2x; g1y = g2y;
-    g0xr = g1xr; g0yr = g1yr;
+    // (corners of c+1, row s: shifted again rather than kept, one register
+    // pair fewer)
+    const R g0xr = A::right(g0x), g0yr = A::right(g0y);
     {
       const double mm = from_left(vp1), pm = vp1, mp = from_left(vp2), pp = vp2;
       R gx = 0, gy = 0;
@@ -397,13 +476,13 @@ __device__ __forceinline__ void march_body(const MarchArgs& a) {
     // -- densities (density.c) and their face averages -------------------------
     const double d_m1 = d_0;
     d_0 = DEN(v0);
-    dnm1 = dn0;
+    if (!LR) dnm1 = dn0;
     dn0 = DEN(vn0);
     const double d_0l = from_left(d_0), dn0l = from_left(dn0);
     const double dfx = 0.5 * d_0l + 0.5 * d_0;    // x face c, row s, old vof
     const double dfy = 0.5 * d_m1 + 0.5 * d_0;    // y face s, old vof
     const double d1x = 0.5 * dn0l + 0.5 * dn0;    // x face c, row s, new vof
-    d1y = 0.5 * dnm1 + 0.5 * dn0;                 // y face s, new vof
+    d1y = 0.5 * ring_ld(RG_DNM1, dnm1) + 0.5 * dn0;  // y face s, new vof
     // -- momentum fluxes shared between neighbouring faces ---------------------
     const R mfx0r = A::right(mfx0), mfyp1l = A::left(mfyp1);
     const R ruy0l = A::left(ruy0);
@@ -442,8 +521,8 @@ __device__ __forceinline__ void march_body(const MarchArgs& a) {
       }
       s_ += grav * rdfx;
       const double sd = double(s_);
-      if (a.srca_ux && outrow && c_out) fx_row(a.srca_ux, s)[lane] = sd;
-      uxs = dfx * ux0 + a.ca * dt * sd + a.cb * dt * sbx + a.cg * dt * g;
+      if (a.srca_ux && outrow && c_out) AT(fx_row(a.srca_ux, s)) = sd;
+      uxs = dfx * ux0 + dtca * sd + dtcb * sbx + dtcg * g;
       uxs /= d1x;
     }
     // -- predicted uy, row s (predict/uy.c, force.c) ---------------------------
@@ -455,15 +534,15 @@ __device__ __forceinline__ void march_body(const MarchArgs& a) {
       s_ += SB(T_DCX) * (-qf_txy0 + qf_txy0r);
       s_ += SB(T_DCX) * (-cyc_tyym1 + cyc_tyy0);
       double g = 0.;
-      g -= rhy * (-pm1 + p0);
+      g -= rhy * (-ring_ld(RG_PM1, pm1) + p0);
       {
         const R k = R(0.5) * km1 + R(0.5) * k0;
         const R delta = rhyr * (-rvm1 + rv0);
         s_ += R(dfy) * refdeninv * tension * k * delta;
       }
       const double sd = double(s_);
-      if (a.srca_uy && outrow && c_out) ci_row(a.srca_uy, s)[lane] = sd;
-      uys = dfy * uy0 + a.ca * dt * sd + a.cb * dt * sby + a.cg * dt * g;
+      if (a.srca_uy && outrow && c_out) AT(ci_row(a.srca_uy, s)) = sd;
+      uys = dfy * uy0 + dtca * sd + dtcb * sby + dtcg * g;
       uys /= d1y;
     }
     qy_ux = qy_ux_new;
@@ -475,14 +554,14 @@ __device__ __forceinline__ void march_body(const MarchArgs& a) {
     const double txy0r = double(A::right(txy_prev));
     if (outrow && c_out) {
       auto put = [&](double* f, int nh, auto row, double v) {
-        row(f, s)[lane] = v;
-        if (s > ny - nh) row(f, s - ny)[lane] = v;
-        if (s <= nh) row(f, s + ny)[lane] = v;
+        AT(row(f, s)) = v;
+        if (s > ny - nh) AT(row(f, s - ny)) = v;
+        if (s <= nh) AT(row(f, s + ny)) = v;
       };
       auto put_at = [&](double* f, int nh, auto row, int off, double v) {
-        row(f, s)[lane + off] = v;
-        if (s > ny - nh) row(f, s - ny)[lane + off] = v;
-        if (s <= nh) row(f, s + ny)[lane + off] = v;
+        AT(row(f, s) + off) = v;
+        if (s > ny - nh) AT(row(f, s - ny) + off) = v;
+        if (s <= nh) AT(row(f, s + ny) + off) = v;
       };
       auto rv = [&](double* f, int j) { return vof_row(f, j); };
       auto rc = [&](double* f, int j) { return cc_row(f, j); };
@@ -490,7 +569,7 @@ __device__ __forceinline__ void march_body(const MarchArgs& a) {
       put(a.vofn, 2, rv, vn0);
       if (c == 1) put_at(a.vofn, 2, rv, -1, 0.);
       if (c == nx) put_at(a.vofn, 2, rv, 1, 0.);
-      if (a.srca_vof) ci_row(a.srca_vof, s)[lane] = src;
+      if (a.srca_vof) AT(ci_row(a.srca_vof, s)) = src;
       put(a.uxs, 1, rx, c == 1 ? 0. : uxs);
       if (c == nx) put_at(a.uxs, 1, rx, 1, 0.);
       put(a.uys, 1, rc, uys);
@@ -506,47 +585,97 @@ __device__ __forceinline__ void march_body(const MarchArgs& a) {
         put(a.txy, 1, rx, double(txy_prev));
         if (c == nx) put_at(a.txy, 1, rx, 1, txy0r);
       }
+      if (CORR) {
+        // the previous stage's updated pressure, with its Neumann ghosts
+        put(a.p_out, 1, rc, p0);
+        if (c == 1) put_at(a.p_out, 1, rc, -1, p0);
+        if (c == nx) put_at(a.p_out, 1, rc, 1, p0);
+      }
     }
     // -- Poisson right-hand side of row s-1 (compute_potential.c:315-360), FP64
     {
       // u* divergence: the x-face terms of cell c+1 come from lane c+1
-      const double q_uxs = q_uxs_m1;                       // QF * ux* at face c, row s-1
+      const double q_uxs = ring_ld(RG_QUXS, q_uxs_m1);     // QF * ux* at face c, row s-1
       const double q_uxsr = from_right(q_uxs);
       const double cyc_uys0 = TB(T_CYC) * uys;
       // explicit split-pressure term: the gradient at face c+1 is lane c+1's
       // gradient at its left face; the top y face of row s-1 is the bottom
       // face of row s, kept for the next step
-      const double psl = from_left(psm1);
-      const double dxm = d1x_m1;
-      const double gxm = (a.refden / dxm - 1.) * (-psl + psm1) / TB(T_HXXF);
+      const double psm1v = ring_ld(RG_PSM1, psm1);
+      const double psl = from_left(psm1v);
+      const double dxm = ring_ld(RG_D1X, d1x_m1);
+      const double gxm = (a.refden / dxm - 1.) * (-psl + psm1v) / TB(T_HXXF);
       const double qgx = TB(T_QF) * gxm;
       const double qgxr = from_right(qgx);
-      const double gyp = (a.refden / d1y - 1.) * (-psm1 + ps0) / hy;
+      const double gyp = (a.refden / d1y - 1.) * (-psm1v + ps0) / hy;
       const double cg_yp = TB(T_CYC) * gyp;
       const int r = s - 1;
       if (r >= jb && r <= je && c_out) {
-        const double div = TB(T_RJC) * (-q_uxs + q_uxsr - cyc_uys_m1 + cyc_uys0);
-        const double add = TB(T_RJC) * (-qgx + qgxr - cg_ym + cg_yp);
-        ci_row(a.rhs, r)[lane] = c0 * div + c1 * add;
+        const double div = TB(T_RJC) * (-q_uxs + q_uxsr - ring_ld(RG_CYCUYS, cyc_uys_m1) + cyc_uys0);
+        const double add = TB(T_RJC) * (-qgx + qgxr - ring_ld(RG_CGY, cg_ym) + cg_yp);
+        AT(ci_row(a.rhs, r)) = c0 * div + c1 * add;
       }
-      cg_ym = cg_yp;
-      cyc_uys_m1 = cyc_uys0;
+      ring_st(RG_CGY, cg_ym, cg_yp);
+      ring_st(RG_CYCUYS, cyc_uys_m1, cyc_uys0);
     }
-    q_uxs_m1 = TB(T_QF) * uxs;
-    d1x_m1 = d1x;
+    ring_st(RG_QUXS, q_uxs_m1, TB(T_QF) * uxs);
+    ring_st(RG_D1X, d1x_m1, d1x);
+    if (CORR) ring_st(RG_PSM1, psm1, ps0);
+    if (LR) {
+      rings[RG_PM1][lane] = p0;
+      rings[RG_DNM1][lane] = dn0;
+    }
+    if (CORR && s + 1 <= je + 1) {
+      // The previous stage's projection of the rows prefetched for the next
+      // step, formed here where the stencil's temporaries are dead: ux of row
+      // s+2, uy of face s+2 (correct_velocity/ux.c:26-45, uy.c:26-43) and the
+      // pressure of row s+1 (update_pressure.c:13-24, written with the
+      // outputs of that row), with k_post's expressions and the lane's own
+      // face distance.  The vof is this stage's input, i.e. the previous
+      // stage's new vof.
+      const double da = DEN(vp1), db = DEN(vp2);   // rows s+1, s+2
+      const double dbl = from_left(db);
+      const double nps_l = from_left(nps), npo_l = from_left(npo);
+      {
+        double u = 0.;
+        if (c >= 2 && c <= nx) {
+          u = nux;
+          u -= pcx0 / TB(T_HXXF) * (-nps_l + nps);
+          const double d = 0.5 * dbl + 0.5 * db;
+          u += pdtc * (1. / d - irefden) / TB(T_HXXF) * (-npo_l + npo);
+        }
+        nux = u;
+      }
+      {
+        double u = 0.;
+        if (c == 0) u = kUyXm;
+        else if (c == nx + 1) u = kUyXp;
+        else if (c_in) {
+          u = nuy;
+          u -= pcy * (-psp1 + nps);
+          const double d = 0.5 * da + 0.5 * db;
+          u += pdtc * (1. / d - irefden) / hy * (-pop1 + npo);
+        }
+        nuy = u;
+      }
+      if (c_in) np = np + psp1;
+    }
   }
 }
 
-// the FP64 body needs 256 VGPRs (2 wavefronts per SIMD); the FP32 body
-// needs about 200, and forcing 3 or 4 wavefronts per SIMD spills to scratch,
-// which measured 1.7x and 3x slower, so both run 2 per SIMD
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_march(MarchArgs a) {
-  march_body<double>(a);
-}
-
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SBFS_MIXED_WAVES))) void k_march_mixed(
-    MarchArgs a) {
-  march_body<float>(a);
-}
+// The FP64 bodies need about 230-250 VGPRs, two wavefronts per SIMD (three
+// would spill dozens of registers to scratch, measured 1.4x slower).  The
+// FP32 bodies fit 168 VGPRs, three wavefronts per SIMD, which hides more of
+// their latency (the folded variant 1.35x faster than at two).
+#define SBFS_MARCH_KERNEL(name, R, CORR, WAVES)                                            \
+  __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES))) void name( \
+      MarchArgs a) {                                                                      \
+    march_body<R, CORR>(a);                                                               \
+  }
+SBFS_MARCH_KERNEL(k_march, double, false, SBFS_FP64_WAVES)
+SBFS_MARCH_KERNEL(k_march_corr, double, true, SBFS_FP64_WAVES)
+SBFS_MARCH_KERNEL(k_march_mixed, float, false, SBFS_MIXED_WAVES)
+SBFS_MARCH_KERNEL(k_march_mixed_corr, float, true, SBFS_MIXED_WAVES)
+#undef SBFS_MARCH_KERNEL
 
 }  // namespace sbfs

@@ -11,6 +11,7 @@ namespace sbfs {
 // next cell, whose normal needs the corner gradient one column further.
 constexpr int kStrip = 58;
 constexpr int kWarm = 6;    // rows marched before the first output row of a band
+constexpr int kWarmCorr = 8;  // the same when the inputs are corrected on entry
 
 // per-column quotients of the metrics (see fused.hip)
 struct Coef {
@@ -57,6 +58,14 @@ struct MarchArgs {
   double refdeninv, tension, grav, refden, norm;
   const double* dt;
   double ca, cb, cg, g_old, g_new;
+  // projection of the previous stage, folded into this one (stages 1, 2):
+  // ux / uy then hold the previous stage's predicted velocity, p its pressure
+  // before the update, psi0 its new potential and psi_old the potential it
+  // started from; the updated pressure of the rows this wavefront owns is
+  // written to p_out
+  const double* psi_old;
+  double* p_out;
+  double pg_old, pg_new;  // Runge-Kutta gamma of the previous stage's dt_old / dt_new
   int stage;
   int nx, ny, band;
 };
