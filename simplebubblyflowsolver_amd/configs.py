@@ -62,6 +62,14 @@ def bubble_swarm(nx: int = 2048, ny: int = 4096, nbubbles: int = 64,
                     Re=Re, We=We, steps=steps, bubbles=b, seed=seed)
 
 
+def static_drop(n: int = 256, steps: int = 5000, radius_cells: float = 32.0) -> Workload:
+    """A gas drop of radius 32 cells at the centre of a unit square (n x n),
+    no gravity (Fr = 1e30 makes the buoyancy term -1/Fr^2 vanish), at rest:
+    any velocity is a spurious current of the surface-tension model."""
+    return Workload("static_drop_%dx%d" % (n, n), n, n, lx=1.0, ly=1.0, Fr=1e30, steps=steps,
+                    bubbles=[(0.5, 0.5, radius_cells / n)])
+
+
 def large_swarm() -> Workload:
     """4096 x 8192, 1024 bubbles, Re=100, We=5, 500 steps."""
     return bubble_swarm(4096, 8192, 1024, 500, seed=4, Re=100.0, We=5.0)

@@ -499,6 +499,7 @@ struct FusedScratch {
   double* ux2 = nullptr;  // predicted velocity of stage 1 (folded projection)
   double* uy2 = nullptr;
   double* p = nullptr;    // pressure written by a stage kernel with the folded projection
+  float* kappa = nullptr;  // learned curvature of a stage's input vof (interior cells)
 };
 static FusedScratch g_scr;
 
@@ -506,6 +507,7 @@ static int scratch(const domain_t* d, const interface_t* s, const fluid_t* f) {
   if (g_scr.nx == d->mysizes[0] && g_scr.ny == d->mysizes[1]) return 0;
   (void)hipStreamSynchronize(stream());
   for (double* q : {g_scr.vof, g_scr.ux, g_scr.uy, g_scr.ux2, g_scr.uy2, g_scr.p}) (void)hipFree(q);
+  (void)hipFree(g_scr.kappa);
   g_scr = FusedScratch();
   const std::pair<double**, size_t> bufs[] = {{&g_scr.vof, s->vof.datasize}, {&g_scr.ux, f->ux.datasize},
                                               {&g_scr.uy, f->uy.datasize},   {&g_scr.ux2, f->ux.datasize},
@@ -514,6 +516,7 @@ static int scratch(const domain_t* d, const interface_t* s, const fluid_t* f) {
     SBFS_HIP(hipMalloc(b.first, b.second));
     SBFS_HIP(hipMemset(*b.first, 0, b.second));
   }
+  SBFS_HIP(hipMalloc(&g_scr.kappa, sizeof(float) * d->mysizes[0] * d->mysizes[1]));
   g_scr.nx = d->mysizes[0];
   g_scr.ny = d->mysizes[1];
   return 0;
@@ -659,7 +662,7 @@ static int march_band(int nx, int ny, bool mixed) {
 // pointer swaps the reference performs.  No allocation and no host
 // synchronisation, so the sequence can be captured into a graph.
 static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, interface_t* interface,
-                      const Coef& cf, double* rhs, int band, bool use_tiles) {
+                      const Coef& cf, double* rhs, int band, bool use_tiles, double learned_h) {
   const int nx = (int)c->nx, ny = (int)c->ny;
   if (compute_dt_device(domain, c, fluid, interface)) return 1;
   const double* dtp = c->dt_dev;
@@ -733,6 +736,11 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
       m.p_out = corr ? g_scr.p : nullptr;
       m.pg_old = rk((k + 1) % 3).g;  // gammas of stage k-1's dt_old / dt_new
       m.pg_new = rk((k + 2) % 3).g;
+      m.kappa = nullptr;
+      if (learned_h > 0.) {
+        if (curv_mlp_launch(a.vof, g_scr.kappa, nx, ny, learned_h)) return 1;
+        m.kappa = g_scr.kappa;
+      }
       m.srcb_vof = a.srcb_vof;
       m.srcb_ux = a.srcb_ux;
       m.srcb_uy = a.srcb_uy;
@@ -883,7 +891,9 @@ static StepKey step_key(const DomainCtx* c, fluid_t* f, interface_t* s, const Co
     key_add(k, x);
   k.v.push_back((uint64_t)band | ((uint64_t)use_tiles << 32));
   k.v.push_back((uint64_t)options().gauge | ((uint64_t)options().tdm << 8) |
-                ((uint64_t)options().chunk << 16) | ((uint64_t)options().precision << 48));
+                ((uint64_t)options().chunk << 16) | ((uint64_t)options().precision << 48) |
+                ((uint64_t)options().curvature << 56));
+  key_add(k, g_scr.kappa);
   return k;
 }
 
@@ -907,8 +917,8 @@ static bool graphs_enabled() {
 }
 
 static int step_graph(const domain_t* domain, DomainCtx* c, fluid_t* fluid, interface_t* interface,
-                      const Coef& cf, double* rhs, int band, bool use_tiles) {
-  if (!graphs_enabled()) return fused_body(domain, c, fluid, interface, cf, rhs, band, use_tiles);
+                      const Coef& cf, double* rhs, int band, bool use_tiles, double learned_h) {
+  if (!graphs_enabled()) return fused_body(domain, c, fluid, interface, cf, rhs, band, use_tiles, learned_h);
   const StepKey key = step_key(c, fluid, interface, cf, rhs, band, use_tiles);
   g_clock++;
   for (StepGraph& g : g_graph) {
@@ -924,7 +934,7 @@ static int step_graph(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
   if (!seen) {
     // first sight of this configuration: run eagerly (builds plans / tables)
     g_seen[g_clock & 1] = key;
-    return fused_body(domain, c, fluid, interface, cf, rhs, band, use_tiles);
+    return fused_body(domain, c, fluid, interface, cf, rhs, band, use_tiles, learned_h);
   }
   StepGraph& g = g_graph[0].last_use <= g_graph[1].last_use ? g_graph[0] : g_graph[1];
   if (g.exec) {
@@ -933,7 +943,7 @@ static int step_graph(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
   }
   hipGraph_t graph = nullptr;
   SBFS_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeRelaxed));
-  const int rc = fused_body(domain, c, fluid, interface, cf, rhs, band, use_tiles);
+  const int rc = fused_body(domain, c, fluid, interface, cf, rhs, band, use_tiles, learned_h);
   const hipError_t ce = hipStreamEndCapture(stream(), &graph);
   if (rc) {
     if (graph) (void)hipGraphDestroy(graph);
@@ -976,7 +986,18 @@ int integrate_fused(const domain_t* domain, fluid_t* fluid, interface_t* interfa
   const int band = march_band(nx, ny, options().precision == SBFS_PRECISION_MIXED);
   if (band <= 0) return 1;
   if (load_dt_coefs()) return 1;
-  if (step_graph(domain, c, fluid, interface, cf, rhs, band, use_tiles)) return 1;
+  double learned_h = 0.;
+  if (options().curvature == SBFS_CURVATURE_LEARNED) {
+    // the network sees 5 x 5 stencils of square cells of one size
+    SBFS_REQUIRE(!use_tiles, "learned curvature: needs the marching stage kernel");
+    const double h = domain->hy;
+    for (int i = 0; i < nx; i++)
+      SBFS_REQUIRE(fabs(domain->hxxc[i] - h) <= 1e-12 * h,
+                   "learned curvature: needs uniform square cells (HXXC(%d) = %g, hy = %g)", i + 1,
+                   domain->hxxc[i], h);
+    learned_h = h;
+  }
+  if (step_graph(domain, c, fluid, interface, cf, rhs, band, use_tiles, learned_h)) return 1;
   SBFS_HIP(hipMemcpyAsync(c->host_pin, c->dt_dev, sizeof(double), hipMemcpyDeviceToHost, stream()));
   SBFS_HIP(hipStreamSynchronize(stream()));
   *dt = c->host_pin[0];

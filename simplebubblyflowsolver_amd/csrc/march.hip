@@ -379,6 +379,10 @@ __device__ __forceinline__ void march_body(const MarchArgs& a) {
     km1 = k0;
     {
       R kk = 0;
+      // the learned value of row s replaces the plain one where the network
+      // gave one
+      float kl = 0.f;
+      if (a.kappa && c_in) kl = lane_at(ci_row(a.kappa, wrap_r(s, ny)), loff >> 1);
       if (c_in) {
         const R nxm = R(0.5) * g0x + R(0.5) * g1x;
         const R nxp = R(0.5) * g0xr + R(0.5) * g1xr;
@@ -386,6 +390,7 @@ __device__ __forceinline__ void march_body(const MarchArgs& a) {
         const R nyp = R(0.5) * g1y + R(0.5) * g1yr;
         const R div = SB(T_RJC) * (-SB(T_QF) * nxm + SR(T_QF) * nxp - SB(T_CYC) * nym + SB(T_CYC) * nyp);
         kk = R(-1) * div;
+        if (kl == kl && kl != 0.f) kk = R(kl);
       }
       k0 = kk;
     }

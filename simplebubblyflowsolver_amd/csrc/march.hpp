@@ -66,6 +66,9 @@ struct MarchArgs {
   const double* psi_old;
   double* p_out;
   double pg_old, pg_new;  // Runge-Kutta gamma of the previous stage's dt_old / dt_new
+  // learned curvature of the stage's input vof (interior cells, (j-1) Nx +
+  // (i-1)); NaN where the plain curvature stays; null: plain everywhere
+  const float* kappa;
   int stage;
   int nx, ny, band;
 };

@@ -1,0 +1,826 @@
+// Learned curvature: a 25 -> 64 -> 64 -> 1 MLP (GELU) that maps a 5 x 5
+// volume-fraction stencil to the grid-scaled curvature kappa * h of the
+// centre cell.  Everything runs on the device:
+//   * k_mlp_samples   rasterises training stencils from circles and ellipses
+//                     of known curvature (16 x 16 sub-cell sampling, random
+//                     sub-cell offsets, 8-fold symmetry) and from the solver's
+//                     own diffuse tanh initial profile;
+//   * k_mlp_grad      forward + backward of 64 samples per wavefront on BF16
+//                     MFMA with FP32 accumulation; writes one gradient partial
+//                     per wavefront;
+//   * k_mlp_adam      sums the partials in a fixed order and takes an Adam
+//                     step on the FP32 master weights, refreshing the BF16
+//                     copies the MFMA operands read;
+//   * k_curv_mlp      inference on the solver's vof, one 64-cell row segment
+//                     per wavefront, skipped where no cell is in the THINC
+//                     band; writes kappa (NaN where the plain curvature
+//                     stays).
+//
+// MFMA: v_mfma_f32_16x16x16bf16_1k.  Lane l holds A[l%16][4(l/16)+r],
+// B[4(l/16)+r][l%16] and D[4(l/16)+r][l%16] (r = 0..3).  Activations are
+// kept as (hidden x samples) matrices, the B operand, with the weights as the
+// A operand (out x in), so one layer's D output is the next layer's B operand
+// lane for lane.
+//
+// The reference computes the curvature from the normalised vof gradient
+// (src/interface/curvature_tensor.c:100-135); this network replaces it at
+// sharp interfaces (cut cells and their neighbours) whose stencil lies
+// inside the domain and whose prediction falls inside the training range.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "sbfs.h"
+#include "sbfs_internal.hpp"
+
+namespace sbfs {
+namespace mlp {
+
+constexpr int NI = 25, NIP = 32, NH = 64;
+// offsets of the FP32 parameter block
+constexpr int OW1 = 0;               // W1 [NH][NIP] (columns NI.. stay zero)
+constexpr int OB1 = OW1 + NH * NIP;  // b1 [NH]
+constexpr int OW2 = OB1 + NH;        // W2 [NH][NH] (out x in)
+constexpr int OB2 = OW2 + NH * NH;   // b2 [NH]
+constexpr int OW3 = OB2 + NH;        // w3 [NH]
+constexpr int OB3 = OW3 + NH;        // b3
+constexpr int NP = OB3 + 1;          // 6337 parameters
+constexpr int NPART = NP + 1;        // + the loss sum of the tile
+constexpr int kSub = 16;             // sub-cell samples per direction
+constexpr double kPi = 3.14159265358979323846;
+// A cell gets the learned value if its 3 x 3 block holds an interface cell
+// (vof in [kUseLo, kUseHi]), its 5 x 5 stencil holds near-pure cells of
+// both phases (vof < kPure and > 1 - kPure: a sharp interface; on the wide
+// tanh profile of the initial condition the plain curvature is the more
+// accurate one) and avoids the wall ghosts, and |prediction| lies in
+// [kKhLo, kKhHi] (kappa h; the training circles span 1/64 .. 1/2).
+constexpr double kUseLo = 0.02, kUseHi = 0.98, kPure = 1e-3;
+constexpr float kKhLo = 1.f / 96.f, kKhHi = 0.75f;
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// counter-based random numbers: splitmix64 finaliser of (seed, sample, k)
+// ---------------------------------------------------------------------------
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__host__ __device__ inline double uni(uint64_t seed, uint64_t i, uint64_t k) {
+  const uint64_t key = seed * 0x9E3779B97F4A7C15ull + i * 16ull + k;
+  return (double)(mix64(key) >> 11) * 0x1.0p-53;
+}
+
+// FP32 -> BF16, round to nearest even (finite inputs)
+__host__ __device__ inline unsigned short f2bf(float x) {
+  uint32_t u;
+  memcpy(&u, &x, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+__device__ inline float gelu(float x) { return 0.5f * x * (1.f + erff(x * 0.70710678118654752f)); }
+__device__ inline float dgelu(float x) {
+  return 0.5f * (1.f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
+}
+
+__device__ inline f4 mma(s4 a, s4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+__device__ inline s4 ld4(const unsigned short* p) { return *(const s4*)p; }
+__device__ inline s4 pack4(float a, float b, float c, float d) {
+  s4 r;
+  r[0] = (short)f2bf(a);
+  r[1] = (short)f2bf(b);
+  r[2] = (short)f2bf(c);
+  r[3] = (short)f2bf(d);
+  return r;
+}
+
+// features of a 5 x 5 stencil (row-major, centre at 12): differences to the
+// centre value, and the centre value minus 1/2
+__host__ __device__ inline void features(const double* v, float* f) {
+  for (int k = 0; k < NI; k++) f[k] = (float)(k == 12 ? v[12] - 0.5 : v[k] - v[12]);
+}
+
+// ---------------------------------------------------------------------------
+// training data
+// ---------------------------------------------------------------------------
+// One sample per thread.  Random numbers k of sample i:
+//   0 shape (ellipse if >= 1/2), 1 minor semi-axis b = 2 + 62 u cells,
+//   2 axis ratio q = 1 + 2 u (ellipses), 3 rotation, 4 point parameter t,
+//   5, 6 sub-cell offsets of the grid, 7 profile (diffuse if < 1/2, circles
+//   only), 8 diffuse slope s = 0.5 * 4^u per cell, 9 symmetry (8
+//   transforms), 10 normal offset of the centre cell (diffuse), 11
+//   complement (if < 1/2), 12 neighbour shift (sharp, if >= 1/2).
+// Sharp cells hold the gas fraction inside the shape from 16 x 16 sub-cell
+// points, and the stencil is centred on the cell holding the point
+// (a cos t, b sin t) or on one of its 8 neighbours.  Diffuse cells hold
+// 1/2 (1 + tanh(s (R - d))) at the cell centre, the solver's initial profile
+// (initial_condition/main.py:40-41), and the centre cell sits at a normal
+// offset uniform in +-2/s cells, where the centre vof is about 0.02 .. 0.98.
+// The label is the exact curvature of the interface times h at that point:
+// positive with the gas inside, the reference's sign convention.  The
+// complement (liquid inside, vof -> 1 - vof) flips its sign.
+__global__ void k_mlp_samples(uint64_t seed, long n0, long n, unsigned short* __restrict__ X,
+                              float* __restrict__ T, float* __restrict__ F) {
+  const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const uint64_t i = (uint64_t)(n0 + s);
+  double u[13];
+  for (int k = 0; k < 13; k++) u[k] = uni(seed, i, (uint64_t)k);
+  const bool ellipse = u[0] >= 0.5;
+  const double b = 2. + 62. * u[1];
+  const double q = ellipse ? 1. + 2. * u[2] : 1.;
+  const double a = q * b;
+  const double phi = 2. * kPi * u[3], t = 2. * kPi * u[4];
+  const double ox = u[5], oy = u[6];
+  const bool diffuse = !ellipse && u[7] < 0.5;
+  const double slope = 0.5 * pow(4., u[8]);
+  const int sym = (int)(u[9] * 8.);
+  const double cph = cos(phi), sph = sin(phi);
+  const double ex = a * cos(t), ey = b * sin(t);
+  double px = cph * ex - sph * ey, py = sph * ex + cph * ey;
+  if (diffuse) {
+    // circle: the outward normal at the point is the point over b
+    const double off = (2. * u[10] - 1.) * 2. / slope;
+    px += off * px / b;
+    py += off * py / b;
+  }
+  const double st = sin(t), ct = cos(t);
+  const bool comp = u[11] < 0.5;
+  const double label = (comp ? -1. : 1.) * a * b / pow(a * a * st * st + b * b * ct * ct, 1.5);
+  double ic = floor(px - ox), jc = floor(py - oy);
+  if (!diffuse && u[12] >= 0.5) {
+    const int m = (int)((u[12] - 0.5) * 16.);  // one of the 8 neighbours
+    const int sh = m < 4 ? m : m + 1;          // skip the centre of the 3 x 3 block
+    ic += sh % 3 - 1;
+    jc += sh / 3 - 1;
+  }
+  double v[NI];
+  for (int dj = -2; dj <= 2; dj++) {
+    for (int di = -2; di <= 2; di++) {
+      const double x0 = ic + di + ox, y0 = jc + dj + oy;
+      double val;
+      if (diffuse) {
+        const double cx = x0 + 0.5, cy = y0 + 0.5;
+        val = 0.5 * (1. + tanh(slope * (b - sqrt(cx * cx + cy * cy))));
+      } else {
+        int cnt = 0;
+        for (int qy = 0; qy < kSub; qy++) {
+          for (int qx = 0; qx < kSub; qx++) {
+            const double x = x0 + (qx + 0.5) / kSub, y = y0 + (qy + 0.5) / kSub;
+            const double xe = cph * x + sph * y, ye = -sph * x + cph * y;
+            cnt += (xe / a) * (xe / a) + (ye / b) * (ye / b) < 1. ? 1 : 0;
+          }
+        }
+        val = cnt / (double)(kSub * kSub);
+      }
+      // 8-fold symmetry: bit 0 transposes, bit 1 mirrors x, bit 2 mirrors y
+      int ti = di, tj = dj;
+      if (sym & 1) {
+        const int w = ti;
+        ti = tj;
+        tj = w;
+      }
+      if (sym & 2) ti = -ti;
+      if (sym & 4) tj = -tj;
+      v[(tj + 2) * 5 + (ti + 2)] = comp ? 1. - val : val;
+    }
+  }
+  float f[NI];
+  features(v, f);
+  for (int k = 0; k < NIP; k++) X[(size_t)s * NIP + k] = k < NI ? f2bf(f[k]) : 0;
+  T[s] = (float)label;
+  if (F)
+    for (int k = 0; k < NI; k++) F[(size_t)s * NI + k] = (float)v[k];
+}
+
+// ---------------------------------------------------------------------------
+// forward pass of one 16-sample subtile.  The weights stay resident as A
+// operands: aw1[mt][ks] = W1[16 mt + l%16][16 ks + 4 (l/16) ..],
+// aw2[mt][ks] = W2[16 mt + l%16][16 ks + 4 (l/16) ..].
+// ---------------------------------------------------------------------------
+struct Weights {
+  const unsigned short* w1;   // [NH][NIP] bf16
+  const unsigned short* w2;   // [NH][NH] bf16
+  const unsigned short* w2t;  // [NH][NH] bf16, transposed
+  const float* p;             // FP32 parameters (biases, w3, b3)
+};
+
+struct AOps {
+  s4 w1[4][2];
+  s4 w2[4][4];
+};
+
+__device__ inline void load_aops(const Weights& w, AOps& a) {
+  const int l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++) {
+#pragma unroll
+    for (int ks = 0; ks < 2; ks++) a.w1[mt][ks] = ld4(w.w1 + (16 * mt + lr) * NIP + 16 * ks + 4 * lg);
+#pragma unroll
+    for (int ks = 0; ks < 4; ks++) a.w2[mt][ks] = ld4(w.w2 + (16 * mt + lr) * NH + 16 * ks + 4 * lg);
+  }
+}
+
+// x0, x1: the B operands of the subtile (features 0-15, 16-31); h1, h2: the
+// pre-activations (D layout: hidden 16 mt + 4 (l/16) + r, sample l%16);
+// returns y of sample l%16 of the subtile (in every lane group)
+__device__ inline float forward16(const Weights& w, const AOps& a, s4 x0, s4 x1, f4 (&h1)[4], f4 (&h2)[4]) {
+  const int lg = (threadIdx.x & 63) >> 4;
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++) {
+    f4 c = {0.f, 0.f, 0.f, 0.f};
+    c = mma(a.w1[mt][0], x0, c);
+    c = mma(a.w1[mt][1], x1, c);
+#pragma unroll
+    for (int r = 0; r < 4; r++) c[r] += w.p[OB1 + 16 * mt + 4 * lg + r];
+    h1[mt] = c;
+  }
+  s4 ab[4];
+#pragma unroll
+  for (int ks = 0; ks < 4; ks++) ab[ks] = pack4(gelu(h1[ks][0]), gelu(h1[ks][1]), gelu(h1[ks][2]), gelu(h1[ks][3]));
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++) {
+    f4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 4; ks++) c = mma(a.w2[mt][ks], ab[ks], c);
+#pragma unroll
+    for (int r = 0; r < 4; r++) c[r] += w.p[OB2 + 16 * mt + 4 * lg + r];
+    h2[mt] = c;
+  }
+  float s = 0.f;
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) s += w.p[OW3 + 16 * mt + 4 * lg + r] * gelu(h2[mt][r]);
+  s += __shfl_xor(s, 16);
+  s += __shfl_xor(s, 32);
+  return s + w.p[OB3];
+}
+
+// B operands of subtile nt from 64 samples [sample][NIP] bf16 in LDS
+__device__ inline void load_x(const unsigned short* X, int nt, s4& x0, s4& x1) {
+  const int l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
+  x0 = ld4(X + (16 * nt + lr) * NIP + 4 * lg);
+  x1 = ld4(X + (16 * nt + lr) * NIP + 16 + 4 * lg);
+}
+
+// sum over the 16 lanes l%16 of a group (butterfly)
+__device__ inline float sum16(float v) {
+  v += __shfl_xor(v, 1);
+  v += __shfl_xor(v, 2);
+  v += __shfl_xor(v, 4);
+  v += __shfl_xor(v, 8);
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// training: one wavefront = one tile of 64 samples; part[tile][NPART].
+// Forward and the deltas run per 16-sample subtile; the weight gradients,
+// which sum over the samples, read the staged activations and deltas from
+// LDS as [hidden][sample].
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_mlp_grad(Weights w, const unsigned short* __restrict__ X,
+                                                 const float* __restrict__ T, long n, long first,
+                                                 long stride, float scale, float* __restrict__ part) {
+  __shared__ __align__(16) unsigned short sx[64 * NIP];   // [sample][feature]
+  __shared__ __align__(16) unsigned short sxt[NIP * 64];  // [feature][sample]
+  __shared__ __align__(16) unsigned short sa1[NH * 64];   // gelu(h1) [hidden][sample]
+  __shared__ __align__(16) unsigned short sd2[NH * 64];   // dh2 [hidden][sample]
+  __shared__ __align__(16) unsigned short sd1[NH * 64];   // dh1 [hidden][sample]
+  __shared__ float st[64];
+  const int l = threadIdx.x, lr = l & 15, lg = l >> 4;
+  const long tile = blockIdx.x;
+  // sample of this lane: a stride permutation of the data set
+  const long idx = (first + (tile * 64 + l) * stride) % n;
+  {
+    const unsigned short* src = X + (size_t)idx * NIP;
+    for (int k = 0; k < NIP; k++) {
+      const unsigned short v = src[k];
+      sx[l * NIP + k] = v;
+      sxt[k * 64 + l] = v;
+    }
+    st[l] = T[idx];
+  }
+  AOps a;
+  load_aops(w, a);
+  __syncthreads();
+  float lsum = 0.f, db3 = 0.f;
+  float gw3[4][4], gb2[4][4], gb1[4][4];  // per-lane partial sums, hidden 16 mt + 4 lg + r
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) gw3[mt][r] = gb2[mt][r] = gb1[mt][r] = 0.f;
+#pragma unroll 1
+  for (int nt = 0; nt < 4; nt++) {
+    s4 x0, x1;
+    load_x(sx, nt, x0, x1);
+    f4 h1[4], h2[4];
+    const float y = forward16(w, a, x0, x1, h1, h2);
+    // output layer: dy = scale (y - t), scale = 2 / batch
+    const float e = y - st[16 * nt + lr];
+    const float dy = scale * e;
+    lsum += e * e;
+    db3 += dy;
+    s4 d2b[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+      f4 d;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int hid = 16 * mt + 4 * lg + r;
+        const float h = h2[mt][r];
+        gw3[mt][r] += gelu(h) * dy;
+        d[r] = w.p[OW3 + hid] * dy * dgelu(h);
+        gb2[mt][r] += d[r];
+        sa1[hid * 64 + 16 * nt + lr] = f2bf(gelu(h1[mt][r]));
+      }
+      d2b[mt] = pack4(d[0], d[1], d[2], d[3]);
+#pragma unroll
+      for (int r = 0; r < 4; r++) sd2[(16 * mt + 4 * lg + r) * 64 + 16 * nt + lr] = (unsigned short)d2b[mt][r];
+    }
+    // dh1 = gelu'(h1) * (W2^T dh2)
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+      f4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < 4; ks++) c = mma(ld4(w.w2t + (16 * mt + lr) * NH + 16 * ks + 4 * lg), d2b[ks], c);
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const float d = c[r] * dgelu(h1[mt][r]);
+        gb1[mt][r] += d;
+        sd1[(16 * mt + 4 * lg + r) * 64 + 16 * nt + lr] = f2bf(d);
+      }
+    }
+  }
+  float* out = part + (size_t)tile * NPART;
+  lsum = sum16(lsum);
+  db3 = sum16(db3);
+  if (l == 0) {
+    out[OB3] = db3;
+    out[NP] = lsum;
+  }
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const float g3 = sum16(gw3[mt][r]), g2 = sum16(gb2[mt][r]), g1 = sum16(gb1[mt][r]);
+      if (lr == 0) {
+        const int hid = 16 * mt + 4 * lg + r;
+        out[OW3 + hid] = g3;
+        out[OB2 + hid] = g2;
+        out[OB1 + hid] = g1;
+      }
+    }
+  __syncthreads();
+  // dW2[out][in] = sum_s dh2[out][s] gelu(h1)[in][s]
+#pragma unroll 1
+  for (int mt = 0; mt < 4; mt++) {
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+      f4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < 4; ks++)
+        c = mma(ld4(sd2 + (16 * mt + lr) * 64 + 16 * ks + 4 * lg), ld4(sa1 + (16 * nt + lr) * 64 + 16 * ks + 4 * lg), c);
+#pragma unroll
+      for (int r = 0; r < 4; r++) out[OW2 + (16 * mt + 4 * lg + r) * NH + 16 * nt + lr] = c[r];
+    }
+  }
+  // dW1[out][in] = sum_s dh1[out][s] x[s][in]
+#pragma unroll 1
+  for (int mt = 0; mt < 4; mt++) {
+#pragma unroll
+    for (int nt = 0; nt < 2; nt++) {
+      f4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < 4; ks++)
+        c = mma(ld4(sd1 + (16 * mt + lr) * 64 + 16 * ks + 4 * lg), ld4(sxt + (16 * nt + lr) * 64 + 16 * ks + 4 * lg), c);
+#pragma unroll
+      for (int r = 0; r < 4; r++) out[OW1 + (16 * mt + 4 * lg + r) * NIP + 16 * nt + lr] = c[r];
+    }
+  }
+}
+
+// gradient sums over the tiles (fixed order), then Adam on the FP32 master
+// weights and a refresh of the BF16 copies; `adam` 0 only sums (gradient check)
+struct AdamState {
+  float* p;    // parameters
+  float* m;
+  float* v;
+  unsigned short* w1b;
+  unsigned short* w2b;
+  unsigned short* w2tb;
+};
+
+__global__ void k_mlp_adam(AdamState s, const float* __restrict__ part, int tiles, int adam, float lr,
+                           float b1t, float b2t, float* __restrict__ grad, float* __restrict__ loss) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > NP) return;
+  float g = 0.f;
+  for (int t = 0; t < tiles; t++) g += part[(size_t)t * NPART + k];
+  if (grad) grad[k] = g;
+  if (k == NP) {
+    if (loss) *loss = g;
+    return;
+  }
+  if (!adam) return;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const float m = b1 * s.m[k] + (1.f - b1) * g;
+  const float v = b2 * s.v[k] + (1.f - b2) * g * g;
+  s.m[k] = m;
+  s.v[k] = v;
+  const float p = s.p[k] - lr * (m / b1t) / (sqrtf(v / b2t) + eps);
+  s.p[k] = p;
+  if (k >= OW1 && k < OB1) s.w1b[k - OW1] = f2bf(p);
+  if (k >= OW2 && k < OB2) {
+    const int o = (k - OW2) / NH, i = (k - OW2) % NH;
+    s.w2b[o * NH + i] = f2bf(p);
+    s.w2tb[i * NH + o] = f2bf(p);
+  }
+}
+
+__global__ void k_mlp_bf16(AdamState s) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= NP) return;
+  const float p = s.p[k];
+  if (k >= OW1 && k < OB1) s.w1b[k - OW1] = f2bf(p);
+  if (k >= OW2 && k < OB2) {
+    const int o = (k - OW2) / NH, i = (k - OW2) % NH;
+    s.w2b[o * NH + i] = f2bf(p);
+    s.w2tb[i * NH + o] = f2bf(p);
+  }
+}
+
+// prediction for host-given features [n][NIP] bf16: y[n]
+__global__ __launch_bounds__(64) void k_mlp_predict(Weights w, const unsigned short* __restrict__ X, long n,
+                                                    float* __restrict__ y) {
+  __shared__ __align__(16) unsigned short sx[64 * NIP];
+  const int l = threadIdx.x, lr = l & 15, lg = l >> 4;
+  const long s = (long)blockIdx.x * 64 + l;
+  for (int k = 0; k < NIP; k++) sx[l * NIP + k] = s < n ? X[(size_t)s * NIP + k] : 0;
+  AOps a;
+  load_aops(w, a);
+  __syncthreads();
+#pragma unroll 1
+  for (int nt = 0; nt < 4; nt++) {
+    s4 x0, x1;
+    load_x(sx, nt, x0, x1);
+    f4 h1[4], h2[4];
+    const float v = forward16(w, a, x0, x1, h1, h2);
+    const long o = (long)blockIdx.x * 64 + 16 * nt + lr;
+    if (lg == 0 && o < n) y[o] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// inference on the solver's vof: kappa = y / h in the cells the rule above
+// admits, NaN elsewhere, where the plain curvature stays.  Layout of kappa:
+// interior cells, (j-1) Nx + (i-1).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_curv_mlp(Weights w, const double* __restrict__ vof,
+                                                 float* __restrict__ kappa, int nx, int ny, float inv_h) {
+  __shared__ __align__(16) unsigned short sx[64 * NIP];
+  const int l = threadIdx.x, lg = l >> 4;
+  const int i = 1 + (int)blockIdx.x * 64 + l, j = 1 + (int)blockIdx.y;
+  const bool cell = i <= nx;
+  const size_t wc = (size_t)nx + 2;
+  bool ok = cell && i >= 3 && i <= nx - 2;
+  if (ok) {
+    double lo = 1., hi = 0.;
+    bool cut = false;
+    for (int dj = -2; dj <= 2; dj++)
+      for (int di = -2; di <= 2; di++) {
+        const double v = vof[(size_t)(j + dj + 1) * wc + i + di];
+        lo = fmin(lo, v);
+        hi = fmax(hi, v);
+        if (dj >= -1 && dj <= 1 && di >= -1 && di <= 1) cut = cut || (v >= kUseLo && v <= kUseHi);
+      }
+    ok = cut && lo < kPure && hi > 1. - kPure;
+  }
+  float* kout = kappa + (size_t)(j - 1) * nx + (i - 1);
+  if (__ballot(ok) == 0) {
+    if (cell) *kout = __int_as_float(0x7FC00000);
+    return;
+  }
+  {
+    double v[NI];
+    const int ic = ok ? i : 3;
+    for (int dj = -2; dj <= 2; dj++)
+      for (int di = -2; di <= 2; di++) v[(dj + 2) * 5 + di + 2] = vof[(size_t)(j + dj + 1) * wc + ic + di];
+    float f[NI];
+    features(v, f);
+    for (int k = 0; k < NIP; k++) sx[l * NIP + k] = k < NI ? f2bf(f[k]) : 0;
+  }
+  AOps a;
+  load_aops(w, a);
+  __syncthreads();
+  // lane l = 16 lg + l%16 keeps the prediction of subtile lg
+  float y = 0.f;
+#pragma unroll 1
+  for (int nt = 0; nt < 4; nt++) {
+    s4 x0, x1;
+    load_x(sx, nt, x0, x1);
+    f4 h1[4], h2[4];
+    const float v = forward16(w, a, x0, x1, h1, h2);
+    if (lg == nt) y = v;
+  }
+  const bool use = ok && fabsf(y) >= kKhLo && fabsf(y) <= kKhHi;
+  if (cell) *kout = use ? y * inv_h : __int_as_float(0x7FC00000);
+}
+
+// ---------------------------------------------------------------------------
+// host state: one network per process
+// ---------------------------------------------------------------------------
+struct Net {
+  float* p = nullptr;  // NP parameters, then m, v
+  float* m = nullptr;
+  float* v = nullptr;
+  unsigned short* w1b = nullptr;
+  unsigned short* w2b = nullptr;
+  unsigned short* w2tb = nullptr;
+  bool trained = false;
+  int step = 0;
+};
+static Net g_net;
+
+static int net_alloc() {
+  if (g_net.p) return 0;
+  SBFS_HIP(hipMalloc(&g_net.p, sizeof(float) * NP * 3));
+  g_net.m = g_net.p + NP;
+  g_net.v = g_net.p + 2 * NP;
+  SBFS_HIP(hipMalloc(&g_net.w1b, sizeof(unsigned short) * (NH * NIP + 2 * NH * NH)));
+  g_net.w2b = g_net.w1b + NH * NIP;
+  g_net.w2tb = g_net.w2b + NH * NH;
+  SBFS_HIP(hipMemset(g_net.p, 0, sizeof(float) * NP * 3));
+  return 0;
+}
+
+static Weights weights() {
+  Weights w;
+  w.w1 = g_net.w1b;
+  w.w2 = g_net.w2b;
+  w.w2t = g_net.w2tb;
+  w.p = g_net.p;
+  return w;
+}
+
+static AdamState adam_state() {
+  AdamState s;
+  s.p = g_net.p;
+  s.m = g_net.m;
+  s.v = g_net.v;
+  s.w1b = g_net.w1b;
+  s.w2b = g_net.w2b;
+  s.w2tb = g_net.w2tb;
+  return s;
+}
+
+// upload FP32 parameters, zero the Adam moments, refresh the BF16 copies
+static int set_params(const float* host) {
+  if (net_alloc()) return 1;
+  SBFS_HIP(hipMemcpyAsync(g_net.p, host, sizeof(float) * NP, hipMemcpyHostToDevice, stream()));
+  SBFS_HIP(hipMemsetAsync(g_net.m, 0, sizeof(float) * NP * 2, stream()));
+  hipLaunchKernelGGL(k_mlp_bf16, dim3(cdiv(NP, 256)), dim3(256), 0, stream(), adam_state());
+  if (check_launch("mlp: bf16 weights")) return 1;
+  g_net.step = 0;
+  return 0;
+}
+
+// Glorot-uniform weights from the counter generator, zero biases
+static void init_params(uint64_t seed, std::vector<float>& p) {
+  p.assign(NP, 0.f);
+  const double r1 = sqrt(6. / (NI + NH)), r2 = sqrt(6. / (NH + NH)), r3 = sqrt(6. / (NH + 1));
+  for (int o = 0; o < NH; o++)
+    for (int k = 0; k < NI; k++) p[OW1 + o * NIP + k] = (float)(r1 * (2. * uni(seed, 1000000000ull + o * NIP + k, 0) - 1.));
+  for (int k = 0; k < NH * NH; k++) p[OW2 + k] = (float)(r2 * (2. * uni(seed, 2000000000ull + k, 0) - 1.));
+  for (int k = 0; k < NH; k++) p[OW3 + k] = (float)(r3 * (2. * uni(seed, 3000000000ull + k, 0) - 1.));
+}
+
+struct Samples {
+  unsigned short* x = nullptr;
+  float* t = nullptr;
+  long n = 0;
+  uint64_t seed = 0;
+};
+
+static int make_samples(uint64_t seed, long n, Samples& s, float* stencils_dev) {
+  SBFS_REQUIRE(n > 0, "mlp: no samples");
+  SBFS_HIP(hipMalloc(&s.x, sizeof(unsigned short) * NIP * (size_t)n));
+  SBFS_HIP(hipMalloc(&s.t, sizeof(float) * (size_t)n));
+  s.n = n;
+  s.seed = seed;
+  hipLaunchKernelGGL(k_mlp_samples, dim3(cdiv(n, 128)), dim3(128), 0, stream(), seed, 0L, n, s.x, s.t,
+                     stencils_dev);
+  return check_launch("mlp: samples");
+}
+
+static void free_samples(Samples& s) {
+  (void)hipFree(s.x);
+  (void)hipFree(s.t);
+  s = Samples();
+}
+
+// one pass of `tiles` tiles starting at permuted position `first`
+static int grad_pass(const Samples& s, long first, long stride, int tiles, float scale, float* part) {
+  hipLaunchKernelGGL(k_mlp_grad, dim3(tiles), dim3(64), 0, stream(), weights(), s.x, s.t, s.n, first, stride,
+                     scale, part);
+  return check_launch("mlp: gradients");
+}
+
+}  // namespace mlp
+
+// the network for the stage kernels: null unless trained or loaded
+int curv_mlp_launch(const double* vof, float* kappa, int nx, int ny, double h) {
+  using namespace mlp;
+  SBFS_REQUIRE(g_net.trained, "learned curvature: no trained network (sbfs_mlp_train or sbfs_mlp_set_params)");
+  const dim3 grid(cdiv(nx, 64), ny);
+  SBFS_TIMED("k_curv_mlp", hipLaunchKernelGGL(k_curv_mlp, grid, dim3(64), 0, stream(), weights(), vof, kappa, nx,
+                                              ny, (float)(1. / h)));
+  return check_launch("k_curv_mlp");
+}
+
+}  // namespace sbfs
+
+using namespace sbfs;
+using namespace sbfs::mlp;
+
+extern "C" int sbfs_mlp_num_params(void) { return NP; }
+
+extern "C" int sbfs_mlp_samples(uint64_t seed, size_t first, size_t n, float* features, float* labels,
+                                float* stencils) {
+  SBFS_REQUIRE(n > 0 && features && labels, "sbfs_mlp_samples: bad arguments");
+  unsigned short* x = nullptr;
+  float *t = nullptr, *f = nullptr;
+  SBFS_HIP(hipMalloc(&x, sizeof(unsigned short) * NIP * n));
+  SBFS_HIP(hipMalloc(&t, sizeof(float) * n));
+  if (stencils) SBFS_HIP(hipMalloc(&f, sizeof(float) * NI * n));
+  hipLaunchKernelGGL(k_mlp_samples, dim3(cdiv((long)n, 128)), dim3(128), 0, stream(), seed, (long)first, (long)n,
+                     x, t, f);
+  int rc = check_launch("sbfs_mlp_samples");
+  std::vector<unsigned short> hx(NIP * n);
+  if (!rc) {
+    SBFS_HIP(hipMemcpyAsync(hx.data(), x, sizeof(unsigned short) * NIP * n, hipMemcpyDeviceToHost, stream()));
+    SBFS_HIP(hipMemcpyAsync(labels, t, sizeof(float) * n, hipMemcpyDeviceToHost, stream()));
+    if (stencils) SBFS_HIP(hipMemcpyAsync(stencils, f, sizeof(float) * NI * n, hipMemcpyDeviceToHost, stream()));
+    SBFS_HIP(hipStreamSynchronize(stream()));
+    for (size_t k = 0; k < NIP * n; k++) {
+      const uint32_t u = (uint32_t)hx[k] << 16;
+      memcpy(&features[k], &u, 4);
+    }
+  }
+  (void)hipFree(x);
+  (void)hipFree(t);
+  (void)hipFree(f);
+  return rc;
+}
+
+extern "C" int sbfs_mlp_set_params(const float* params, size_t n) {
+  SBFS_REQUIRE(params && n == (size_t)NP, "sbfs_mlp_set_params: need %d parameters", NP);
+  if (set_params(params)) return 1;
+  SBFS_HIP(hipStreamSynchronize(stream()));
+  g_net.trained = true;
+  return 0;
+}
+
+extern "C" int sbfs_mlp_get_params(float* params, size_t n) {
+  SBFS_REQUIRE(params && n == (size_t)NP, "sbfs_mlp_get_params: need %d parameters", NP);
+  SBFS_REQUIRE(g_net.p, "sbfs_mlp_get_params: no network");
+  SBFS_HIP(hipMemcpyAsync(params, g_net.p, sizeof(float) * NP, hipMemcpyDeviceToHost, stream()));
+  SBFS_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+// gradient of the mean squared error over n samples (n a multiple of 64)
+// given as BF16-representable features [n][32] and labels, at the current
+// parameters; grad[NP], *loss = the mean squared error
+extern "C" int sbfs_mlp_gradient(const float* features, const float* labels, size_t n, float* grad,
+                                 double* loss) {
+  SBFS_REQUIRE(features && labels && grad && loss && n > 0 && n % 64 == 0, "sbfs_mlp_gradient: bad arguments");
+  SBFS_REQUIRE(g_net.p, "sbfs_mlp_gradient: no parameters set");
+  std::vector<unsigned short> hx(NIP * n);
+  for (size_t k = 0; k < NIP * n; k++) hx[k] = f2bf(features[k]);
+  Samples s;
+  SBFS_HIP(hipMalloc(&s.x, sizeof(unsigned short) * NIP * n));
+  SBFS_HIP(hipMalloc(&s.t, sizeof(float) * n));
+  s.n = (long)n;
+  const int tiles = (int)(n / 64);
+  float *part = nullptr, *g = nullptr;
+  SBFS_HIP(hipMalloc(&part, sizeof(float) * NPART * tiles));
+  SBFS_HIP(hipMalloc(&g, sizeof(float) * NPART));
+  SBFS_HIP(hipMemcpyAsync(s.x, hx.data(), sizeof(unsigned short) * NIP * n, hipMemcpyHostToDevice, stream()));
+  SBFS_HIP(hipMemcpyAsync(s.t, labels, sizeof(float) * n, hipMemcpyHostToDevice, stream()));
+  int rc = grad_pass(s, 0, 1, tiles, 2.f / (float)n, part);
+  if (!rc) {
+    hipLaunchKernelGGL(k_mlp_adam, dim3(cdiv(NPART, 256)), dim3(256), 0, stream(), adam_state(), (const float*)part,
+                       tiles, 0, 0.f, 1.f, 1.f, g, (float*)nullptr);
+    rc = check_launch("sbfs_mlp_gradient");
+  }
+  std::vector<float> hg(NPART);
+  if (!rc) {
+    SBFS_HIP(hipMemcpyAsync(hg.data(), g, sizeof(float) * NPART, hipMemcpyDeviceToHost, stream()));
+    SBFS_HIP(hipStreamSynchronize(stream()));
+    memcpy(grad, hg.data(), sizeof(float) * NP);
+    *loss = hg[NP] / (double)n;
+  }
+  (void)hipFree(part);
+  (void)hipFree(g);
+  free_samples(s);
+  return rc;
+}
+
+// predictions y (= kappa h) for n stencils of 25 vof values (row-major, centre 12)
+extern "C" int sbfs_mlp_predict(const double* stencils, size_t n, double* out) {
+  SBFS_REQUIRE(stencils && out && n > 0, "sbfs_mlp_predict: bad arguments");
+  SBFS_REQUIRE(g_net.trained, "sbfs_mlp_predict: no trained network");
+  std::vector<unsigned short> hx(NIP * n, 0);
+  for (size_t s = 0; s < n; s++) {
+    float f[NI];
+    features(stencils + s * NI, f);
+    for (int k = 0; k < NI; k++) hx[s * NIP + k] = f2bf(f[k]);
+  }
+  unsigned short* x = nullptr;
+  float* y = nullptr;
+  SBFS_HIP(hipMalloc(&x, sizeof(unsigned short) * NIP * n));
+  SBFS_HIP(hipMalloc(&y, sizeof(float) * n));
+  SBFS_HIP(hipMemcpyAsync(x, hx.data(), sizeof(unsigned short) * NIP * n, hipMemcpyHostToDevice, stream()));
+  hipLaunchKernelGGL(k_mlp_predict, dim3(cdiv((long)n, 64)), dim3(64), 0, stream(), weights(), x, (long)n, y);
+  int rc = check_launch("sbfs_mlp_predict");
+  std::vector<float> hy(n);
+  if (!rc) {
+    SBFS_HIP(hipMemcpyAsync(hy.data(), y, sizeof(float) * n, hipMemcpyDeviceToHost, stream()));
+    SBFS_HIP(hipStreamSynchronize(stream()));
+    for (size_t s = 0; s < n; s++) out[s] = hy[s];
+  }
+  (void)hipFree(x);
+  (void)hipFree(y);
+  return rc;
+}
+
+// train from Glorot-uniform weights on `nsamples` generated stencils (a
+// multiple of `batch`, itself a multiple of 64) for `epochs` passes with Adam
+// (learning rate `lr`, cosine decay to lr / 20); history[e] = mean squared
+// error of epoch e (may be null)
+extern "C" int sbfs_mlp_train(uint64_t seed, size_t nsamples, int epochs, int batch, double lr,
+                              double* history) {
+  SBFS_REQUIRE(nsamples > 0 && epochs > 0 && batch > 0 && batch % 64 == 0 && nsamples % (size_t)batch == 0,
+               "sbfs_mlp_train: nsamples must be a multiple of batch, batch a multiple of 64");
+  std::vector<float> p0;
+  init_params(seed, p0);
+  if (set_params(p0.data())) return 1;
+  Samples s;
+  if (make_samples(seed, (long)nsamples, s, nullptr)) return 1;
+  const int tiles = batch / 64;
+  const long steps = (long)(nsamples / (size_t)batch);
+  float *part = nullptr, *loss = nullptr;
+  SBFS_HIP(hipMalloc(&part, sizeof(float) * NPART * tiles));
+  SBFS_HIP(hipMalloc(&loss, sizeof(float) * steps * epochs));
+  int rc = 0;
+  const long n = (long)nsamples;
+  for (int e = 0; e < epochs && !rc; e++) {
+    // stride permutation of the data set per epoch (a stride prime to n)
+    long stride = (long)(mix64(seed + 7919ull * (uint64_t)e) % (uint64_t)n) | 1;
+    auto gcd = [](long x, long y) {
+      while (y) {
+        const long r = x % y;
+        x = y;
+        y = r;
+      }
+      return x;
+    };
+    while (gcd(stride, n) != 1) stride += 2;
+    const long offset = (long)(mix64(seed ^ (0xABCDull + (uint64_t)e)) % (uint64_t)n);
+    for (long k = 0; k < steps && !rc; k++) {
+      const double frac = (double)(e * steps + k) / (double)(epochs * steps);
+      const float lrk = (float)(lr * (0.05 + 0.95 * 0.5 * (1. + cos(M_PI * frac))));
+      g_net.step++;
+      const float b1t = 1.f - powf(0.9f, (float)g_net.step), b2t = 1.f - powf(0.999f, (float)g_net.step);
+      rc = grad_pass(s, offset + k * batch * stride, stride, tiles, 2.f / (float)batch, part);
+      if (!rc) {
+        hipLaunchKernelGGL(k_mlp_adam, dim3(cdiv(NPART, 256)), dim3(256), 0, stream(), adam_state(),
+                           (const float*)part, tiles, 1, lrk, b1t, b2t, (float*)nullptr, loss + e * steps + k);
+        rc = check_launch("sbfs_mlp_train");
+      }
+    }
+  }
+  if (!rc && history) {
+    std::vector<float> hl((size_t)steps * epochs);
+    SBFS_HIP(hipMemcpyAsync(hl.data(), loss, sizeof(float) * hl.size(), hipMemcpyDeviceToHost, stream()));
+    SBFS_HIP(hipStreamSynchronize(stream()));
+    for (int e = 0; e < epochs; e++) {
+      double acc = 0.;
+      for (long k = 0; k < steps; k++) acc += hl[(size_t)e * steps + k];
+      history[e] = acc / (double)nsamples;
+    }
+  }
+  SBFS_HIP(hipStreamSynchronize(stream()));
+  (void)hipFree(part);
+  (void)hipFree(loss);
+  free_samples(s);
+  if (!rc) g_net.trained = true;
+  return rc;
+}

@@ -412,6 +412,12 @@ int sbfs_set_precision(int precision) {
   return 0;
 }
 
+int sbfs_set_curvature(int mode) {
+  SBFS_REQUIRE(mode == SBFS_CURVATURE_PLAIN || mode == SBFS_CURVATURE_LEARNED, "unknown curvature mode %d", mode);
+  options().curvature = mode;
+  return 0;
+}
+
 int sbfs_set_tdm(int kind, int chunk) {
   SBFS_REQUIRE(kind == SBFS_TDM_SEQUENTIAL || kind == SBFS_TDM_PARTITIONED, "unknown tdm kind %d", kind);
   SBFS_REQUIRE(chunk >= 2 && chunk <= 64 && (chunk & (chunk - 1)) == 0,

@@ -132,6 +132,7 @@ struct Options {
   int tdm = 1;         // SBFS_TDM_PARTITIONED
   int chunk = 32;
   int precision = 0;   // SBFS_PRECISION_FP64
+  int curvature = 0;   // SBFS_CURVATURE_PLAIN
   bool coefs_set = false;
   double cadv = 0., cdif = 0., cint = 0.;
 };
@@ -147,6 +148,10 @@ DomainCtx* domain_ctx(const domain_t* d);
 int npy_write(const char* dir, const char* name, const char* descr,
               const std::vector<size_t>& shape, size_t elem, const void* data);
 int load_dt_coefs();
+
+// learned curvature (mlp.hip): kappa of the interior cells of `vof`, NaN
+// where the plain curvature stays; h is the (uniform, square) cell size
+int curv_mlp_launch(const double* vof, float* kappa, int nx, int ny, double h);
 
 // kernel timing (profiler.hip): events around launches while profiling
 bool prof_active();  // kernel timing on (sbfs_profile_start)

@@ -125,6 +125,73 @@ def product_backend(path: os.PathLike | str | None = None) -> Backend:
     return Backend(lib, "", "sbfs_", True, error_fn="sbfs_last_error")
 
 
+class CurvatureNet:
+    """The learned-curvature network of libsbfs.so (include/sbfs.h)."""
+
+    PLAIN, LEARNED = 0, 1
+
+    def __init__(self, backend: Backend):
+        self.be = backend
+        lib = backend.lib
+        f = C.c_float
+        lib.sbfs_mlp_num_params.restype = C.c_int
+        self.num_params = int(lib.sbfs_mlp_num_params())
+        lib.sbfs_mlp_train.argtypes = [C.c_uint64, C.c_size_t, C.c_int, C.c_int, C.c_double, _P(C.c_double)]
+        lib.sbfs_mlp_set_params.argtypes = [_P(f), C.c_size_t]
+        lib.sbfs_mlp_get_params.argtypes = [_P(f), C.c_size_t]
+        lib.sbfs_mlp_samples.argtypes = [C.c_uint64, C.c_size_t, C.c_size_t, _P(f), _P(f), _P(f)]
+        lib.sbfs_mlp_gradient.argtypes = [_P(f), _P(f), C.c_size_t, _P(f), _P(C.c_double)]
+        lib.sbfs_mlp_predict.argtypes = [_P(C.c_double), C.c_size_t, _P(C.c_double)]
+        lib.sbfs_set_curvature.argtypes = [C.c_int]
+        self.lib = lib
+
+    @staticmethod
+    def _p(a, t):
+        return a.ctypes.data_as(_P(t))
+
+    def train(self, seed: int, nsamples: int, epochs: int, batch: int, lr: float) -> np.ndarray:
+        hist = np.zeros(epochs)
+        self.be.check(self.lib.sbfs_mlp_train(seed, nsamples, epochs, batch, lr, self._p(hist, C.c_double)),
+                      "sbfs_mlp_train")
+        return hist
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.num_params, np.float32)
+        self.be.check(self.lib.sbfs_mlp_get_params(self._p(p, C.c_float), p.size), "sbfs_mlp_get_params")
+        return p
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        self.be.check(self.lib.sbfs_mlp_set_params(self._p(p, C.c_float), p.size), "sbfs_mlp_set_params")
+
+    def samples(self, seed: int, first: int, n: int):
+        x = np.zeros((n, 32), np.float32)
+        t = np.zeros(n, np.float32)
+        st = np.zeros((n, 25), np.float32)
+        self.be.check(self.lib.sbfs_mlp_samples(seed, first, n, self._p(x, C.c_float), self._p(t, C.c_float),
+                                                self._p(st, C.c_float)), "sbfs_mlp_samples")
+        return x, t, st
+
+    def gradient(self, x: np.ndarray, t: np.ndarray):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(self.num_params, np.float32)
+        loss = C.c_double(0.0)
+        self.be.check(self.lib.sbfs_mlp_gradient(self._p(x, C.c_float), self._p(t, C.c_float), t.size,
+                                                 self._p(g, C.c_float), C.byref(loss)), "sbfs_mlp_gradient")
+        return loss.value, g
+
+    def predict(self, stencils: np.ndarray) -> np.ndarray:
+        st = np.ascontiguousarray(stencils, np.float64).reshape(-1, 25)
+        y = np.zeros(st.shape[0])
+        self.be.check(self.lib.sbfs_mlp_predict(self._p(st, C.c_double), st.shape[0], self._p(y, C.c_double)),
+                      "sbfs_mlp_predict")
+        return y
+
+    def use(self, learned: bool) -> None:
+        self.be.check(self.lib.sbfs_set_curvature(self.LEARNED if learned else self.PLAIN), "sbfs_set_curvature")
+
+
 class Case:
     """One simulation: domain_t + fluid_t + interface_t on a backend."""
 
