@@ -154,7 +154,72 @@ def mixed_run(be, w, warmup, steps, t_fp64, d0, d1_fp64) -> dict:
             "volume_drift": abs(d["volume"] - d0["volume"]) / d0["volume"], "divmax": d["divmax"],
             "precision": "FP32 interface and momentum stencils; FP64 vof / velocity updates, "
                          "Poisson solve, projection and all fields in memory",
-            "learned_curvature": None}
+            "learned_curvature": None}  # filled by the caller
+
+
+# the learned curvature's training run (the chosen form's 2^21 stencils)
+TRAIN = dict(seed=2, nsamples=1 << 21, epochs=40, batch=4096, lr=1e-3)
+MLP_FLOP_PER_SAMPLE = 3 * 2 * (25 * 64 + 64 * 64 + 64 * 1)  # forward + backward, 25-64-64-1
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def learned_run(be, w, warmup, steps, t_fp64, d0, d1_fp64) -> dict:
+    """Mixed precision with the learned curvature. The network is trained on
+    the device first, with its training kernel timed by HIP events for the
+    MFMA rate; then the workload runs from the same initial state."""
+    lib = be.lib
+    net = solver.CurvatureNet(be)
+    be.check(lib.sbfs_profile_start(), "profile_start")
+    net.train(seed=3, nsamples=1 << 18, epochs=1, batch=TRAIN["batch"], lr=1e-3)
+    be.check(lib.sbfs_profile_stop(), "profile_stop")
+    grad = {e[0]: e for e in profile_entries(lib)}.get("k_mlp_grad")
+    t0 = time.perf_counter()
+    hist = net.train(**TRAIN)
+    be.check(lib.sbfs_synchronize(), "sync")
+    train_s = time.perf_counter() - t0
+    be.check(lib.sbfs_set_precision(1), "set_precision")
+    net.use(True)
+    try:
+        g = solver.Case(be, w.nx, w.ny, w.lx, w.ly, w.Re, w.Fr, w.denr, w.visr, w.We)
+        g.load_initial_condition(w.vof())
+        for _ in range(warmup):
+            g.integrate()
+        be.check(lib.sbfs_synchronize(), "sync")
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            g.integrate()
+        be.check(lib.sbfs_synchronize(), "sync")
+        t = time.perf_counter() - t0
+        d = g.diagnostics()
+        cover = net.coverage(g)
+        be.check(lib.sbfs_profile_start(), "profile_start")
+        for _ in range(3):
+            g.integrate()
+        be.check(lib.sbfs_profile_stop(), "profile_stop")
+        ent = {e[0]: e for e in profile_entries(lib)}
+        curv, gate = ent.get("k_curv_mlp"), ent.get("k_curv_gate")
+        g.close()
+    finally:
+        net.use(False)
+        be.check(lib.sbfs_set_precision(0), "set_precision")
+    ur = d1_fp64["rise_velocity"]
+    out = {"value": round(w.cells * steps / t / 1e6, 3), "ms_per_step": round(t / steps * 1e3, 4),
+           "speedup_vs_fp64": round(t_fp64 / t, 3),
+           "rise_velocity_rel_error": abs(d["rise_velocity"] - ur) / max(abs(ur), 1e-300),
+           "volume_drift": abs(d["volume"] - d0["volume"]) / d0["volume"], "divmax": d["divmax"],
+           "cells_learned_fraction": cover,
+           "train": {"samples": TRAIN["nsamples"], "epochs": TRAIN["epochs"], "batch": TRAIN["batch"],
+                     "seconds": round(train_s, 3), "final_mse": float(hist[-1])}}
+    if grad:
+        avg = grad[2] / grad[1] / 1e3
+        tf = MLP_FLOP_PER_SAMPLE * TRAIN["batch"] / avg / 1e12
+        out["train"]["k_mlp_grad"] = {"avg_us": round(avg * 1e6, 2), "tflops": round(tf, 2),
+                                      "mfma_frac_of_bf16_peak": round(tf / BF16_PEAK_TFLOPS, 5)}
+    if curv:
+        out["k_curv_mlp_avg_us"] = round(curv[2] / curv[1] * 1e3, 2)
+    if gate:
+        out["k_curv_gate_avg_us"] = round(gate[2] / gate[1] * 1e3, 2)
+    return out
 
 
 def main() -> int:
@@ -202,6 +267,7 @@ def main() -> int:
     mixed = None
     if a.mixed and a.precision == "fp64" and a.path == "fused":
         mixed = mixed_run(be, w, a.warmup, a.steps, t, d0, d1)
+        mixed["learned_curvature"] = learned_run(be, w, a.warmup, a.steps, t, d0, d1)
     # per-kernel times: a separate pass with HIP events around every launch
     # (the events keep the step out of graph replay, so it is not the timed loop)
     prof = []

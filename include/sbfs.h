@@ -99,6 +99,10 @@ extern int sbfs_mlp_gradient(const float * features, const float * labels,
                              size_t n, float * grad, double * loss);
 /* kappa h predicted for n stencils of 25 vof values (row-major, centre 12) */
 extern int sbfs_mlp_predict(const double * stencils, size_t n, double * out);
+/* fraction of the interior cells of the current vof where LEARNED takes
+ * the network's value */
+extern int sbfs_mlp_coverage(const domain_t * domain,
+                             const interface_t * interface, double * fraction);
 
 /* uniform-x domain built as the reference's IC generator does
  * (initial_condition/main.py:15-24): xf = linspace(0, lx, Nx+1),

@@ -300,6 +300,9 @@ __device__ __forceinline__ void march_body(const MarchArgs& a) {
   double nv = ld_vof(s_first + 2), nux = ld_ux(s_first + 1), nuy = ld_cc(a.uy, s_first + 1);
   double np = ld_cc(a.p, s_first), nps = ld_cc(a.psi0, CORR ? s_first + 1 : s_first);
   double npo = CORR ? ld_cc(a.psi_old, s_first + 1) : 0.;
+  // learned curvature of row s, prefetched a row ahead (0 where absent)
+  auto ld_kap = [&](int j) { return (a.kappa && c_in) ? lane_at(ci_row(a.kappa, wrap_r(j, ny)), loff >> 1) : 0.f; };
+  float nkap = ld_kap(s_first);
   for (int s = s_first; s <= je + 1; s++) {
     // -- new input rows -----------------------------------------------------
     rvm1 = R(v0); v0 = vp1; vp1 = vp2; vp2 = nv;
@@ -314,6 +317,7 @@ __device__ __forceinline__ void march_body(const MarchArgs& a) {
       psm1 = ps0; ps0 = nps;
     }
     const bool outrow = s >= jb && s <= je;
+    const float kap = nkap;
     double sbv = 0., sbx = 0., sby = 0.;
     if (has_b && c_in) {
       // the new vof of every marched row needs its beta source; the
@@ -330,6 +334,7 @@ __device__ __forceinline__ void march_body(const MarchArgs& a) {
       np = ld_cc(a.p, s + 1);
       nps = ld_cc(a.psi0, CORR ? s + 2 : s + 1);
       if (CORR) npo = ld_cc(a.psi_old, s + 2);
+      nkap = ld_kap(s + 1);
     }
     const R rux0 = R(ux0), ruxp1 = R(uxp1), ruy0 = R(uy0), ruyp1 = R(uyp1);
     const R rv0 = R(v0), rvp1 = R(vp1);
@@ -381,8 +386,7 @@ __device__ __forceinline__ void march_body(const MarchArgs& a) {
       R kk = 0;
       // the learned value of row s replaces the plain one where the network
       // gave one
-      float kl = 0.f;
-      if (a.kappa && c_in) kl = lane_at(ci_row(a.kappa, wrap_r(s, ny)), loff >> 1);
+      const float kl = kap;
       if (c_in) {
         const R nxm = R(0.5) * g0x + R(0.5) * g1x;
         const R nxp = R(0.5) * g0xr + R(0.5) * g1xr;

@@ -478,59 +478,125 @@ __global__ __launch_bounds__(64) void k_mlp_predict(Weights w, const unsigned sh
 }
 
 // ---------------------------------------------------------------------------
-// inference on the solver's vof: kappa = y / h in the cells the rule above
-// admits, NaN elsewhere, where the plain curvature stays.  Layout of kappa:
-// interior cells, (j-1) Nx + (i-1).
+// inference on the solver's vof, in two launches.  k_curv_gate (one thread
+// per cell, light) writes NaN to every cell and appends the 64-cell row
+// segments whose rows j-1..j+1, columns i-1..i+1 hold a cut cell (vof in
+// [kUseLo, kUseHi]) to a list; k_curv_mlp walks that list with a fixed grid
+// and writes kappa = y / h in the cells the rule above admits (NaN in the
+// others).  Layout of kappa: interior cells, (j-1) Nx + (i-1).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_curv_mlp(Weights w, const double* __restrict__ vof,
-                                                 float* __restrict__ kappa, int nx, int ny, float inv_h) {
-  __shared__ __align__(16) unsigned short sx[64 * NIP];
-  const int l = threadIdx.x, lg = l >> 4;
-  const int i = 1 + (int)blockIdx.x * 64 + l, j = 1 + (int)blockIdx.y;
-  const bool cell = i <= nx;
+__global__ void k_curv_reset(unsigned* __restrict__ count) {
+  if (threadIdx.x == 0) *count = 0u;
+}
+
+// one 64-cell column strip of kGateRows rows per wavefront, four per
+// workgroup; the three rows around row j roll along the strip, and each step
+// prefetches the next row
+constexpr int kGateRows = 16;
+__global__ __launch_bounds__(256) void k_curv_gate(const double* __restrict__ vof, float* __restrict__ kappa,
+                                                   int nx, int ny, unsigned* __restrict__ count,
+                                                   unsigned* __restrict__ list) {
+  const int l = threadIdx.x & 63;
+  const int i = 1 + (int)blockIdx.x * 64 + l;
+  const int j0 = 1 + ((int)blockIdx.y * 4 + (int)(threadIdx.x >> 6)) * kGateRows;
+  if (j0 > ny) return;  // whole wavefronts
+  const int j1 = min(ny, j0 + kGateRows - 1);
   const size_t wc = (size_t)nx + 2;
-  bool ok = cell && i >= 3 && i <= nx - 2;
-  if (ok) {
-    double lo = 1., hi = 0.;
-    bool cut = false;
-    for (int dj = -2; dj <= 2; dj++)
-      for (int di = -2; di <= 2; di++) {
-        const double v = vof[(size_t)(j + dj + 1) * wc + i + di];
-        lo = fmin(lo, v);
-        hi = fmax(hi, v);
-        if (dj >= -1 && dj <= 1 && di >= -1 && di <= 1) cut = cut || (v >= kUseLo && v <= kUseHi);
-      }
-    ok = cut && lo < kPure && hi > 1. - kPure;
+  auto in_use = [](double v) { return (v >= kUseLo) & (v <= kUseHi); };
+  // this lane's column, and for the edge lanes the column beyond the strip
+  // (cells i-1 and i+1 of the 3 x 3 block); columns outside 1..nx never count
+  const bool own = i <= nx;
+  const int ie = l == 0 ? i - 1 : i + 1;
+  const bool edge = (l == 0 || l == 63) && ie >= 1 && ie <= nx;
+  const int ic = min(i, nx), iec = min(max(ie, 1), nx);
+  auto row_cut = [&](int jj) {
+    const double* r = vof + (size_t)(jj + 1) * wc;
+    const double a = r[ic], b = r[iec];
+    return (own & in_use(a)) | (edge & in_use(b));
+  };
+  // every row load of the strip is issued before the first test (rows past
+  // ny + 1 of the last strip are clamped and unused)
+  bool cut[kGateRows + 2];
+#pragma unroll
+  for (int k = 0; k < kGateRows + 2; k++) cut[k] = row_cut(min(j0 - 1 + k, ny + 1));
+  // rows of the strip with a cut cell in their 3 x 3 blocks; one atomic per
+  // wavefront reserves their slots in the list (one counter shared by all
+  // wavefronts: per-row atomics on it serialise)
+  unsigned mask = 0u;
+#pragma unroll
+  for (int k = 0; k < kGateRows; k++) {
+    const int j = j0 + k;
+    if (j > j1) break;
+    if (own) kappa[(size_t)(j - 1) * nx + (i - 1)] = __int_as_float(0x7FC00000);
+    if (__ballot(cut[k] | cut[k + 1] | cut[k + 2]) != 0) mask |= 1u << k;
   }
-  float* kout = kappa + (size_t)(j - 1) * nx + (i - 1);
-  if (__ballot(ok) == 0) {
-    if (cell) *kout = __int_as_float(0x7FC00000);
-    return;
+  if (mask != 0u && l == 0) {
+    unsigned q = atomicAdd(count, (unsigned)__popc(mask));
+    for (int k = 0; k < kGateRows; k++)
+      if (mask & (1u << k)) list[q++] = (unsigned)(j0 + k - 1) * gridDim.x + blockIdx.x;
   }
-  {
-    double v[NI];
-    const int ic = ok ? i : 3;
-    for (int dj = -2; dj <= 2; dj++)
-      for (int di = -2; di <= 2; di++) v[(dj + 2) * 5 + di + 2] = vof[(size_t)(j + dj + 1) * wc + ic + di];
-    float f[NI];
-    features(v, f);
-    for (int k = 0; k < NIP; k++) sx[l * NIP + k] = k < NI ? f2bf(f[k]) : 0;
-  }
+}
+
+__global__ __launch_bounds__(64) void k_curv_mlp(Weights w, const double* __restrict__ vof,
+                                                 float* __restrict__ kappa, int nx, int ny, float inv_h,
+                                                 const unsigned* __restrict__ count,
+                                                 const unsigned* __restrict__ list, unsigned segs_x) {
+  __shared__ __align__(16) unsigned short sx[64 * NIP];
+  __shared__ double tile[5][68];  // rows j-2..j+2, columns i0-2..i0+65
+  const int l = threadIdx.x, lg = l >> 4;
+  const unsigned n = *count;
+  const size_t wc = (size_t)nx + 2;
   AOps a;
   load_aops(w, a);
-  __syncthreads();
-  // lane l = 16 lg + l%16 keeps the prediction of subtile lg
-  float y = 0.f;
+  // every wavefront leaves the loop after at most ceil(n / gridDim.x) segments
+  for (unsigned q = blockIdx.x; q < n; q += gridDim.x) {
+    const unsigned seg = list[q];
+    const int i0 = 1 + (int)(seg % segs_x) * 64, j = 1 + (int)(seg / segs_x);
+    const int i = i0 + l;
+    const bool cell = i <= nx;
+    // the vof layout has 2 halo rows, so rows j-2..j+2 exist; columns are
+    // clamped to the wall ghosts 0 and nx+1 (cells whose stencil reaches
+    // them are not admitted)
+    __syncthreads();
+    for (int k = l; k < 5 * 68; k += 64) {
+      const int r = k / 68, c = k % 68;
+      const int ii = min(max(i0 + c - 2, 0), nx + 1);
+      tile[r][c] = vof[(size_t)(j + r - 2 + 1) * wc + ii];
+    }
+    __syncthreads();
+    bool ok = cell && i >= 3 && i <= nx - 2;
+    double lo = 1., hi = 0.;
+    bool cut = false;
+    double v[NI];
+    for (int r = 0; r < 5; r++)
+      for (int d = -2; d <= 2; d++) {
+        const double x = tile[r][l + 2 + d];
+        v[r * 5 + d + 2] = x;
+        lo = fmin(lo, x);
+        hi = fmax(hi, x);
+        if (r >= 1 && r <= 3 && d >= -1 && d <= 1) cut = cut || (x >= kUseLo && x <= kUseHi);
+      }
+    ok = ok && cut && lo < kPure && hi > 1. - kPure;
+    if (__ballot(ok) == 0) continue;  // uniform: the NaN written by the gate stays
+    {
+      float f[NI];
+      features(v, f);
+      for (int k = 0; k < NIP; k++) sx[l * NIP + k] = k < NI ? f2bf(f[k]) : 0;
+    }
+    __syncthreads();
+    // lane l = 16 lg + l%16 keeps the prediction of subtile lg
+    float y = 0.f;
 #pragma unroll 1
-  for (int nt = 0; nt < 4; nt++) {
-    s4 x0, x1;
-    load_x(sx, nt, x0, x1);
-    f4 h1[4], h2[4];
-    const float v = forward16(w, a, x0, x1, h1, h2);
-    if (lg == nt) y = v;
+    for (int nt = 0; nt < 4; nt++) {
+      s4 x0, x1;
+      load_x(sx, nt, x0, x1);
+      f4 h1[4], h2[4];
+      const float p = forward16(w, a, x0, x1, h1, h2);
+      if (lg == nt) y = p;
+    }
+    const bool use = ok && fabsf(y) >= kKhLo && fabsf(y) <= kKhHi;
+    if (use) kappa[(size_t)(j - 1) * nx + (i - 1)] = y * inv_h;
   }
-  const bool use = ok && fabsf(y) >= kKhLo && fabsf(y) <= kKhHi;
-  if (cell) *kout = use ? y * inv_h : __int_as_float(0x7FC00000);
 }
 
 // ---------------------------------------------------------------------------
@@ -627,20 +693,36 @@ static void free_samples(Samples& s) {
 
 // one pass of `tiles` tiles starting at permuted position `first`
 static int grad_pass(const Samples& s, long first, long stride, int tiles, float scale, float* part) {
-  hipLaunchKernelGGL(k_mlp_grad, dim3(tiles), dim3(64), 0, stream(), weights(), s.x, s.t, s.n, first, stride,
-                     scale, part);
+  SBFS_TIMED("k_mlp_grad", hipLaunchKernelGGL(k_mlp_grad, dim3(tiles), dim3(64), 0, stream(), weights(), s.x, s.t,
+                                              s.n, first, stride, scale, part));
   return check_launch("mlp: gradients");
 }
 
 }  // namespace mlp
 
-// the network for the stage kernels: null unless trained or loaded
+// the learned curvature of a stage's input vof: the gate, then the network
+// on the listed segments (list and counter kept per process, grown with the
+// grid)
 int curv_mlp_launch(const double* vof, float* kappa, int nx, int ny, double h) {
   using namespace mlp;
   SBFS_REQUIRE(g_net.trained, "learned curvature: no trained network (sbfs_mlp_train or sbfs_mlp_set_params)");
-  const dim3 grid(cdiv(nx, 64), ny);
-  SBFS_TIMED("k_curv_mlp", hipLaunchKernelGGL(k_curv_mlp, grid, dim3(64), 0, stream(), weights(), vof, kappa, nx,
-                                              ny, (float)(1. / h)));
+  static unsigned* buf = nullptr;  // [0] counter, then the segment list
+  static size_t cap = 0;
+  const unsigned segs_x = cdiv(nx, 64);
+  const size_t segs = (size_t)segs_x * ny;
+  if (segs + 1 > cap) {
+    (void)hipStreamSynchronize(stream());
+    (void)hipFree(buf);
+    buf = nullptr;
+    SBFS_HIP(hipMalloc(&buf, sizeof(unsigned) * (segs + 1)));
+    cap = segs + 1;
+  }
+  hipLaunchKernelGGL(k_curv_reset, dim3(1), dim3(64), 0, stream(), buf);
+  SBFS_TIMED("k_curv_gate", hipLaunchKernelGGL(k_curv_gate, dim3(segs_x, cdiv(ny, 4 * kGateRows)), dim3(256), 0,
+                                               stream(), vof, kappa, nx, ny, buf, buf + 1));
+  SBFS_TIMED("k_curv_mlp", hipLaunchKernelGGL(k_curv_mlp, dim3(2048), dim3(64), 0, stream(), weights(), vof, kappa,
+                                              nx, ny, (float)(1. / h), (const unsigned*)buf,
+                                              (const unsigned*)(buf + 1), segs_x));
   return check_launch("k_curv_mlp");
 }
 
@@ -801,8 +883,9 @@ extern "C" int sbfs_mlp_train(uint64_t seed, size_t nsamples, int epochs, int ba
       const float b1t = 1.f - powf(0.9f, (float)g_net.step), b2t = 1.f - powf(0.999f, (float)g_net.step);
       rc = grad_pass(s, offset + k * batch * stride, stride, tiles, 2.f / (float)batch, part);
       if (!rc) {
-        hipLaunchKernelGGL(k_mlp_adam, dim3(cdiv(NPART, 256)), dim3(256), 0, stream(), adam_state(),
-                           (const float*)part, tiles, 1, lrk, b1t, b2t, (float*)nullptr, loss + e * steps + k);
+        SBFS_TIMED("k_mlp_adam", hipLaunchKernelGGL(k_mlp_adam, dim3(cdiv(NPART, 256)), dim3(256), 0, stream(),
+                                                    adam_state(), (const float*)part, tiles, 1, lrk, b1t, b2t,
+                                                    (float*)nullptr, loss + e * steps + k));
         rc = check_launch("sbfs_mlp_train");
       }
     }
@@ -822,5 +905,45 @@ extern "C" int sbfs_mlp_train(uint64_t seed, size_t nsamples, int epochs, int ba
   (void)hipFree(loss);
   free_samples(s);
   if (!rc) g_net.trained = true;
+  return rc;
+}
+
+namespace sbfs {
+namespace mlp {
+__global__ void k_count_learned(const float* __restrict__ k, long n, unsigned long long* __restrict__ cnt) {
+  unsigned long long c = 0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    c += k[i] == k[i] ? 1ull : 0ull;
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if ((threadIdx.x & 63) == 0) atomicAdd(cnt, c);
+}
+}  // namespace mlp
+}  // namespace sbfs
+
+// fraction of the interior cells of the current vof that the learned
+// curvature would take from the network
+extern "C" int sbfs_mlp_coverage(const domain_t* domain, const interface_t* interface, double* fraction) {
+  SBFS_REQUIRE(domain && interface && fraction, "sbfs_mlp_coverage: null argument");
+  if (check_array(domain, &interface->vof, NA_VOF, 8, "vof")) return 1;
+  const int nx = (int)domain->mysizes[0], ny = (int)domain->mysizes[1];
+  const long n = (long)nx * ny;
+  float* k = nullptr;
+  unsigned long long* cnt = nullptr;
+  SBFS_HIP(hipMalloc(&k, sizeof(float) * n));
+  SBFS_HIP(hipMalloc(&cnt, sizeof(unsigned long long)));
+  SBFS_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), stream()));
+  int rc = curv_mlp_launch((const double*)interface->vof.data, k, nx, ny, domain->hy);
+  unsigned long long h = 0;
+  if (!rc) {
+    hipLaunchKernelGGL(k_count_learned, dim3(1024), dim3(256), 0, stream(), (const float*)k, n, cnt);
+    rc = check_launch("sbfs_mlp_coverage");
+  }
+  if (!rc) {
+    SBFS_HIP(hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, stream()));
+    SBFS_HIP(hipStreamSynchronize(stream()));
+    *fraction = (double)h / (double)n;
+  }
+  (void)hipFree(k);
+  (void)hipFree(cnt);
   return rc;
 }

@@ -143,6 +143,7 @@ class CurvatureNet:
         lib.sbfs_mlp_gradient.argtypes = [_P(f), _P(f), C.c_size_t, _P(f), _P(C.c_double)]
         lib.sbfs_mlp_predict.argtypes = [_P(C.c_double), C.c_size_t, _P(C.c_double)]
         lib.sbfs_set_curvature.argtypes = [C.c_int]
+        lib.sbfs_mlp_coverage.argtypes = [_P(abi.DomainT), _P(abi.InterfaceT), _P(C.c_double)]
         self.lib = lib
 
     @staticmethod
@@ -187,6 +188,13 @@ class CurvatureNet:
         self.be.check(self.lib.sbfs_mlp_predict(self._p(st, C.c_double), st.shape[0], self._p(y, C.c_double)),
                       "sbfs_mlp_predict")
         return y
+
+    def coverage(self, case: "Case") -> float:
+        """Fraction of the case's interior cells where the learned value is taken."""
+        f = C.c_double(0.0)
+        self.be.check(self.lib.sbfs_mlp_coverage(C.byref(case.domain), C.byref(case.interface), C.byref(f)),
+                      "sbfs_mlp_coverage")
+        return f.value
 
     def use(self, learned: bool) -> None:
         self.be.check(self.lib.sbfs_set_curvature(self.LEARNED if learned else self.PLAIN), "sbfs_set_curvature")
