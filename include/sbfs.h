@@ -167,6 +167,38 @@ extern int sbfs_logging_values(const domain_t * domain, const fluid_t * fluid,
                                const interface_t * interface,
                                double out[SBFS_NLOG]);
 
+/*
+ * Ensembles of independent cases of uneven sizes (no counterpart in the
+ * reference, which runs one case per process).  Every kernel of the fused
+ * step runs once per Runge-Kutta stage for all cases together, and each of
+ * the two pointer states a step alternates between is replayed as a HIP
+ * graph.  Per case the step is integrate()'s fused step with the projection
+ * folded into the next stage and the sequential tridiagonal solve, in the
+ * precision sbfs_set_precision selects, with the plain curvature: a case
+ * gives the same bits as when run alone with sbfs_set_tdm(SBFS_TDM_SEQUENTIAL).
+ * Cases are set up through their structs (sbfs_ensemble_case) with the
+ * usual calls (sbfs_array_upload, the boundary routines).  Between ensemble
+ * steps nothing else may swap a case's arrays (integrate() on a case does).
+ */
+typedef struct sbfs_ensemble sbfs_ensemble_t;
+typedef struct {
+  size_t nx, ny;                     /* grid (nx a power of two <= 4096) */
+  double lx, ly, Re, Fr, denr, visr, We;
+} sbfs_case_t;
+extern int sbfs_ensemble_create(const sbfs_case_t * cases, int n,
+                                sbfs_ensemble_t ** ensemble);
+extern int sbfs_ensemble_case(sbfs_ensemble_t * ensemble, int k,
+                              domain_t ** domain, fluid_t ** fluid,
+                              interface_t ** interface);
+/* one step of every case; dts[k] (may be null) receives case k's dt */
+extern int sbfs_ensemble_step(sbfs_ensemble_t * ensemble, double * dts);
+/* band height of the stage kernel (rows per wavefront; default 32):
+ * autotune times every candidate on the ensemble's current state, restores
+ * it and keeps the fastest; both need an even number of steps taken */
+extern int sbfs_ensemble_autotune(sbfs_ensemble_t * ensemble, int * band);
+extern int sbfs_ensemble_set_band(sbfs_ensemble_t * ensemble, int band);
+extern int sbfs_ensemble_destroy(sbfs_ensemble_t * ensemble);
+
 /* step counter and time of an NPY directory (step.npy '<u8', time.npy
  * '<f8', both 0-d), as src/main.c:63-70 reads and :28-29 writes them */
 extern int sbfs_read_step_time(const char dirname[], size_t * step,

@@ -70,6 +70,23 @@ def static_drop(n: int = 256, steps: int = 5000, radius_cells: float = 32.0) -> 
                     bubbles=[(0.5, 0.5, radius_cells / n)])
 
 
+def ensemble(n: int = 256, seed: int = 3) -> list:
+    """Independent rising bubbles: grids drawn from {64x128, 128x256,
+    256x512} on a 1 x 2 domain, density and viscosity ratios log-uniform in
+    [1e-3, 1e-1], We uniform in [1, 50], Re uniform in [20, 200], one
+    centred bubble of diameter 1/4 of the x extent."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        nx = int(rng.choice([64, 128, 256]))
+        w = Workload("ensemble_case_%d_%dx%d" % (k, nx, 2 * nx), nx, 2 * nx, lx=1.0, ly=2.0,
+                     Re=float(rng.uniform(20.0, 200.0)), We=float(rng.uniform(1.0, 50.0)), Fr=1.0,
+                     denr=float(10.0 ** rng.uniform(-3.0, -1.0)), visr=float(10.0 ** rng.uniform(-3.0, -1.0)),
+                     steps=1000, bubbles=[(0.5, 1.0, 0.125)])
+        out.append(w)
+    return out
+
+
 def large_swarm() -> Workload:
     """4096 x 8192, 1024 bubbles, Re=100, We=5, 500 steps."""
     return bubble_swarm(4096, 8192, 1024, 500, seed=4, Re=100.0, We=5.0)

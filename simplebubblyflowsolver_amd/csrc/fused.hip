@@ -426,27 +426,9 @@ __global__ __launch_bounds__(TX* TY) void k_pre(PreArgs a) {
 // ---------------------------------------------------------------------------
 // projection + pressure (correct_velocity/ux.c, uy.c; update_pressure.c)
 // ---------------------------------------------------------------------------
-struct PostArgs {
-  const double* uxs;
-  const double* uys;
-  const double* psi1;
-  const double* psi0;
-  const double* vofn;
-  double* ux;
-  double* uy;
-  double* p;
-  const double* hxxf;
-  double hy;
-  double denr, dlo, dhi, refden;
-  const double* dt;
-  double g_new, g_old;
-  int nx, ny;
-};
-
-__global__ __launch_bounds__(256) void k_post(PostArgs a) {
+// cell (i, j), i = 0..nx+1, j = 1..ny
+__device__ __forceinline__ void post_body(const PostArgs& a, int i, int j) {
   const int nx = a.nx, ny = a.ny;
-  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);  // 0..nx+1
-  const int j = 1 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
   if (i > nx + 1 || j > ny) return;
   const double dt = *a.dt;
   const double dtn = a.g_new * dt;
@@ -484,6 +466,18 @@ __global__ __launch_bounds__(256) void k_post(PostArgs a) {
     if (i == 1) store_with_halo(a.p, 0, j, ny, 1, v, icc);
     if (i == nx) store_with_halo(a.p, nx + 1, j, ny, 1, v, icc);
   }
+}
+
+__global__ __launch_bounds__(256) void k_post(PostArgs a) {
+  post_body(a, (int)(blockIdx.x * blockDim.x + threadIdx.x), 1 + (int)(blockIdx.y * blockDim.y + threadIdx.y));
+}
+
+// one workgroup of 64 x 4 cells per map entry (case, local tile)
+__global__ __launch_bounds__(256) void k_post_batch(const PostArgs* __restrict__ tab, const uint2* __restrict__ map) {
+  const uint2 m = map[blockIdx.x];
+  const PostArgs& a = tab[m.x];
+  const int gx = (a.nx + 2 + 63) / 64;
+  post_body(a, (int)((m.y % gx) * 64 + threadIdx.x), 1 + (int)((m.y / gx) * 4 + threadIdx.y));
 }
 
 // ---------------------------------------------------------------------------
@@ -531,11 +525,9 @@ struct CoefTables {
 };
 static CoefTables g_coef;
 
-static int coef_tables(const domain_t* d, const DomainCtx* c, double Re, Coef* out) {
-  if (g_coef.ctx == c && g_coef.hash == c->hash && g_coef.re == Re && g_coef.buf) {
-    *out = g_coef.cf;
-    return 0;
-  }
+// the 14 per-column quotient tables of one domain and Re in a new device
+// buffer (*buf, owned by the caller)
+int coef_build(const domain_t* d, double Re, double** buf, Coef* out) {
   const size_t nx = d->mysizes[0], w = nx + 2;
   std::vector<double> h(14 * w, 0.);
   double* t[14];
@@ -562,12 +554,9 @@ static int coef_tables(const domain_t* d, const DomainCtx* c, double Re, Coef* o
     t[11][i] = diffusivity / jc;
     t[13][i] = hc;
   }
-  (void)hipStreamSynchronize(stream());
-  (void)hipFree(g_coef.buf);
-  g_coef.buf = nullptr;
-  SBFS_HIP(hipMalloc(&g_coef.buf, sizeof(double) * h.size()));
-  SBFS_HIP(hipMemcpy(g_coef.buf, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
-  const double* b = g_coef.buf;
+  SBFS_HIP(hipMalloc(buf, sizeof(double) * h.size()));
+  SBFS_HIP(hipMemcpy(*buf, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+  const double* b = *buf;
   Coef cf;
   cf.rhf = b + 0 * w;
   cf.rhc = b + 1 * w;
@@ -584,11 +573,23 @@ static int coef_tables(const domain_t* d, const DomainCtx* c, double Re, Coef* o
   cf.hxxf1 = b + 12 * w;
   cf.hxxc1 = b + 13 * w;
   cf.rhy = 1. / hy;
+  *out = cf;
+  return 0;
+}
+
+static int coef_tables(const domain_t* d, const DomainCtx* c, double Re, Coef* out) {
+  if (g_coef.ctx == c && g_coef.hash == c->hash && g_coef.re == Re && g_coef.buf) {
+    *out = g_coef.cf;
+    return 0;
+  }
+  (void)hipStreamSynchronize(stream());
+  (void)hipFree(g_coef.buf);
+  g_coef.buf = nullptr;
+  if (coef_build(d, Re, &g_coef.buf, &g_coef.cf)) return 1;
   g_coef.ctx = c;
   g_coef.hash = c->hash;
   g_coef.re = Re;
-  g_coef.cf = cf;
-  *out = cf;
+  *out = g_coef.cf;
   return 0;
 }
 
@@ -655,6 +656,89 @@ static int march_band(int nx, int ny, bool mixed) {
   long band = (ny + bands - 1) / bands;
   if (band < 4) band = 4;
   return (int)(band < ny ? band : ny);
+}
+
+// Arguments of stage k's stage kernel from the state's current pointers:
+// with the fold (corr) it reads the previous stage's predicted velocity
+// us/vs[k-1], writes us/vs[k], and writes the updated pressure to p_out.
+static void fill_march(const domain_t* domain, const DomainCtx* c, const fluid_t* fluid,
+                       const interface_t* interface, const Coef& cf, double* rhs, int band, int k, bool corr,
+                       double* const us[3], double* const vs[3], double* vofn, double* p_out,
+                       const double* dtp, MarchArgs& m) {
+  const RK r = rk(k);
+  m.vof = D(interface->vof);
+  m.ux = corr ? us[k - 1] : D(fluid->ux);
+  m.uy = corr ? vs[k - 1] : D(fluid->uy);
+  m.p = D(fluid->p);
+  m.psi0 = D(fluid->psi[0]);
+  m.psi_old = corr ? D(fluid->psi[1]) : nullptr;
+  m.p_out = corr ? p_out : nullptr;
+  m.pg_old = rk((k + 1) % 3).g;  // gammas of stage k-1's dt_old / dt_new
+  m.pg_new = rk((k + 2) % 3).g;
+  m.kappa = nullptr;
+  m.srcb_vof = D(interface->src[1]);
+  m.srcb_ux = D(fluid->srcux[1]);
+  m.srcb_uy = D(fluid->srcuy[1]);
+  m.vofn = vofn;
+  m.uxs = us[k];
+  m.uys = vs[k];
+  m.rhs = rhs;
+  // the alpha sources of the last stage are never read again (beta of
+  // stage 0 is zero), so they are not written
+  m.srca_vof = k < 2 ? D(interface->src[0]) : nullptr;
+  m.srca_ux = k < 2 ? D(fluid->srcux[0]) : nullptr;
+  m.srca_uy = k < 2 ? D(fluid->srcuy[0]) : nullptr;
+  m.den1 = k == 2 ? D(fluid->den[1]) : nullptr;
+  m.txy = k == 2 ? D(fluid->txy) : nullptr;
+  m.hxxf1 = cf.hxxf1;
+  m.hxxc1 = cf.hxxc1;
+  m.cf = cf;
+  m.hy = c->hy;
+  m.denr = fluid->denr;
+  m.dlo = fmin(1., fluid->denr);
+  m.dhi = fmax(1., fluid->denr);
+  m.visr = fluid->visr;
+  m.vlo = fmin(1., fluid->visr);
+  m.vhi = fmax(1., fluid->visr);
+  m.refdeninv = 2. / (1. + fluid->denr);
+  m.tension = 1. / interface->We;
+  m.grav = -1. / fluid->Fr / fluid->Fr;
+  m.refden = fluid->refden;
+  m.norm = 2. * (double)domain->glsizes[0];
+  m.dt = dtp;
+  m.ca = r.a;
+  m.cb = r.b;
+  m.cg = r.g;
+  m.g_old = rk((k + 2) % 3).g;
+  m.g_new = r.g;
+  m.stage = k;
+  m.nx = (int)c->nx;
+  m.ny = (int)c->ny;
+  m.band = band;
+}
+
+// arguments of the projection after stage k (predicted velocity uxs / uys)
+static void fill_post(const DomainCtx* c, const fluid_t* fluid, const interface_t* interface, double* uxs,
+                      double* uys, const double* dtp, int k, PostArgs& q) {
+  q.uxs = uxs;
+  q.uys = uys;
+  q.psi1 = D(fluid->psi[1]);
+  q.psi0 = D(fluid->psi[0]);
+  q.vofn = D(interface->vof);
+  q.ux = D(fluid->ux);
+  q.uy = D(fluid->uy);
+  q.p = D(fluid->p);
+  q.hxxf = c->hxxf;
+  q.hy = c->hy;
+  q.denr = fluid->denr;
+  q.dlo = fmin(1., fluid->denr);
+  q.dhi = fmax(1., fluid->denr);
+  q.refden = fluid->refden;
+  q.dt = dtp;
+  q.g_new = rk(k).g;
+  q.g_old = rk((k + 2) % 3).g;
+  q.nx = (int)c->nx;
+  q.ny = (int)c->ny;
 }
 
 // Every launch of one fused time step (decide_dt, then per Runge-Kutta stage
@@ -727,57 +811,11 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
       if (check_launch("integrate: k_pre")) return 1;
     } else {
       MarchArgs m;
-      m.vof = a.vof;
-      m.ux = corr ? us[k - 1] : a.ux;
-      m.uy = corr ? vs[k - 1] : a.uy;
-      m.p = a.p;
-      m.psi0 = D(fluid->psi[0]);
-      m.psi_old = corr ? D(fluid->psi[1]) : nullptr;
-      m.p_out = corr ? g_scr.p : nullptr;
-      m.pg_old = rk((k + 1) % 3).g;  // gammas of stage k-1's dt_old / dt_new
-      m.pg_new = rk((k + 2) % 3).g;
-      m.kappa = nullptr;
+      fill_march(domain, c, fluid, interface, cf, rhs, band, k, corr, us, vs, g_scr.vof, g_scr.p, dtp, m);
       if (learned_h > 0.) {
         if (curv_mlp_launch(a.vof, g_scr.kappa, nx, ny, learned_h)) return 1;
         m.kappa = g_scr.kappa;
       }
-      m.srcb_vof = a.srcb_vof;
-      m.srcb_ux = a.srcb_ux;
-      m.srcb_uy = a.srcb_uy;
-      m.vofn = a.vofn;
-      m.uxs = a.uxs;
-      m.uys = a.uys;
-      m.rhs = rhs;
-      m.srca_vof = a.srca_vof;
-      m.srca_ux = a.srca_ux;
-      m.srca_uy = a.srca_uy;
-      m.den1 = a.den1;
-      m.txy = a.txy;
-      m.hxxf1 = cf.hxxf1;
-      m.hxxc1 = cf.hxxc1;
-      m.cf = cf;
-      m.hy = c->hy;
-      m.denr = a.denr;
-      m.dlo = a.dlo;
-      m.dhi = a.dhi;
-      m.visr = a.visr;
-      m.vlo = a.vlo;
-      m.vhi = a.vhi;
-      m.refdeninv = a.refdeninv;
-      m.tension = a.tension;
-      m.grav = a.grav;
-      m.refden = fluid->refden;
-      m.norm = 2. * (double)domain->glsizes[0];
-      m.dt = dtp;
-      m.ca = r.a;
-      m.cb = r.b;
-      m.cg = r.g;
-      m.g_old = g_old;
-      m.g_new = g_new;
-      m.stage = k;
-      m.nx = nx;
-      m.ny = ny;
-      m.band = band;
       const dim3 grid(cdiv(nx, kStrip), cdiv(ny, band)), block(64);
       const bool mixed = options().precision == SBFS_PRECISION_MIXED;
       if (mixed && corr)
@@ -814,25 +852,7 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
       continue;
     }
     PostArgs q;
-    q.uxs = us[k];
-    q.uys = vs[k];
-    q.psi1 = D(fluid->psi[1]);
-    q.psi0 = D(fluid->psi[0]);
-    q.vofn = D(interface->vof);
-    q.ux = D(fluid->ux);
-    q.uy = D(fluid->uy);
-    q.p = D(fluid->p);
-    q.hxxf = c->hxxf;
-    q.hy = c->hy;
-    q.denr = fluid->denr;
-    q.dlo = fmin(1., fluid->denr);
-    q.dhi = fmax(1., fluid->denr);
-    q.refden = fluid->refden;
-    q.dt = dtp;
-    q.g_new = g_new;
-    q.g_old = g_old;
-    q.nx = nx;
-    q.ny = ny;
+    fill_post(c, fluid, interface, us[k], vs[k], dtp, k, q);
     const dim3 pg(cdiv(nx + 2, 64), cdiv(ny, 4)), pb(64, 4);
     SBFS_TIMED("k_post", hipLaunchKernelGGL(k_post, pg, pb, 0, stream(), q));
     if (check_launch("integrate: k_post")) return 1;
@@ -1001,6 +1021,44 @@ int integrate_fused(const domain_t* domain, fluid_t* fluid, interface_t* interfa
   SBFS_HIP(hipMemcpyAsync(c->host_pin, c->dt_dev, sizeof(double), hipMemcpyDeviceToHost, stream()));
   SBFS_HIP(hipStreamSynchronize(stream()));
   *dt = c->host_pin[0];
+  return 0;
+}
+
+// One fused step of one ensemble case, planned without launching: the
+// pointer choreography of fused_body (projection folded, plain curvature,
+// sequential tridiagonal solve) applied to the case's structs and scratch,
+// recording the arguments of every launch.  On return the structs and
+// scratch hold the pointer state after the step.
+int ensemble_plan_step(const domain_t* domain, DomainCtx* c, fluid_t* fluid, interface_t* interface,
+                       const Coef& cf, int band, CaseScratch& scr, CaseStep& out) {
+  if (dt_args(domain, c, fluid, interface, &out.dt)) return 1;
+  const double* dtp = c->dt_dev;
+  double* rhs = nullptr;
+  if (poisson_transform_fused(domain, c, fluid, &rhs)) return 1;
+  double* const us[3] = {scr.ux[0], scr.ux[1], scr.ux[0]};
+  double* const vs[3] = {scr.uy[0], scr.uy[1], scr.uy[0]};
+  for (int k = 0; k < 3; k++) {
+    const bool corr = k > 0;
+    if (k != 0) {
+      swap_data(interface->src[0], interface->src[1]);
+      swap_data(fluid->srcux[0], fluid->srcux[1]);
+      swap_data(fluid->srcuy[0], fluid->srcuy[1]);
+    }
+    fill_march(domain, c, fluid, interface, cf, rhs, band, k, corr, us, vs, scr.vof, scr.p, dtp, out.march[k]);
+    if (corr) {
+      double* t = (double*)fluid->p.data;
+      fluid->p.data = scr.p;
+      scr.p = t;
+    }
+    {
+      void* t = interface->vof.data;
+      interface->vof.data = scr.vof;
+      scr.vof = (double*)t;
+    }
+    if (poisson_row_args(domain, c, (double*)fluid->psi[1].data, &out.rows[k])) return 1;
+    if (k == 2) fill_post(c, fluid, interface, us[k], vs[k], dtp, k, out.post);
+    swap_data(fluid->psi[0], fluid->psi[1]);
+  }
   return 0;
 }
 

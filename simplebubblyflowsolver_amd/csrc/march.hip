@@ -176,17 +176,18 @@ __device__ __forceinline__ R mixr(double ratio, double lo, double hi, R v) {
 // momentum sources (viscosities, stresses, advection, surface tension,
 // buoyancy) in FP32; the vof update, the density-weighted velocity update,
 // the pressure gradient and the Poisson right-hand side stay FP64.
+// (bx, by): the strip and band of this wavefront
 template <typename R, bool CORR>
-__device__ __forceinline__ void march_body(const MarchArgs& a) {
+__device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
   using A = Arith<R>;
   using V3 = V3R<R>;
   __shared__ double st[T_N][64];
   const int lane = threadIdx.x;
   const int nx = a.nx, ny = a.ny;
-  const int i0 = 1 + blockIdx.x * kStrip;
+  const int i0 = 1 + bx * kStrip;
   const int cl = i0 - 3;  // column held by lane 0
   const int c = cl + lane;
-  const int jb = 1 + blockIdx.y * a.band;
+  const int jb = 1 + by * a.band;
   const int je = min(ny, jb + a.band - 1);
   const bool c_vof = c >= 0 && c <= nx + 1;   // cells incl. wall ghosts
   const bool c_in = c >= 1 && c <= nx;        // interior cells
@@ -679,7 +680,14 @@ __device__ __forceinline__ void march_body(const MarchArgs& a) {
 #define SBFS_MARCH_KERNEL(name, R, CORR, WAVES)                                            \
   __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES))) void name( \
       MarchArgs a) {                                                                      \
-    march_body<R, CORR>(a);                                                               \
+    march_body<R, CORR>(a, blockIdx.x, blockIdx.y);                                       \
+  }                                                                                       \
+  __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES))) void name##_batch( \
+      const MarchArgs* __restrict__ tab, const uint2* __restrict__ map) {                \
+    const uint2 m = map[blockIdx.x];                                                      \
+    const MarchArgs& a = tab[m.x];                                                        \
+    const int strips = (a.nx + kStrip - 1) / kStrip;                                      \
+    march_body<R, CORR>(a, (int)m.y % strips, (int)m.y / strips);                         \
   }
 SBFS_MARCH_KERNEL(k_march, double, false, SBFS_FP64_WAVES)
 SBFS_MARCH_KERNEL(k_march_corr, double, true, SBFS_FP64_WAVES)

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "sbfs_internal.hpp"
+#include "poisson.hpp"
 
 namespace sbfs {
 
@@ -72,5 +73,45 @@ struct MarchArgs {
   int stage;
   int nx, ny, band;
 };
+
+// projection of u and the pressure update after the last stage (k_post)
+struct PostArgs {
+  const double* uxs;
+  const double* uys;
+  const double* psi1;
+  const double* psi0;
+  const double* vofn;
+  double* ux;
+  double* uy;
+  double* p;
+  const double* hxxf;
+  double hy;
+  double denr, dlo, dhi, refden;
+  const double* dt;
+  double g_new, g_old;
+  int nx, ny;
+};
+
+// an ensemble case's step (fused.hip plans it, ensemble.hip launches it)
+struct CaseScratch {
+  double* vof;
+  double* ux[2];
+  double* uy[2];
+  double* p;
+};
+struct CaseStep {
+  DtArgs dt;
+  MarchArgs march[3];
+  RowArgs rows[3];
+  PostArgs post;
+};
+int coef_build(const domain_t* d, double Re, double** buf, Coef* out);
+int ensemble_plan_step(const domain_t* domain, DomainCtx* c, fluid_t* fluid, interface_t* interface,
+                       const Coef& cf, int band, CaseScratch& scr, CaseStep& out);
+__global__ void k_march_batch(const MarchArgs* tab, const uint2* map);
+__global__ void k_march_corr_batch(const MarchArgs* tab, const uint2* map);
+__global__ void k_march_mixed_batch(const MarchArgs* tab, const uint2* map);
+__global__ void k_march_mixed_corr_batch(const MarchArgs* tab, const uint2* map);
+__global__ void k_post_batch(const PostArgs* tab, const uint2* map);
 
 }  // namespace sbfs

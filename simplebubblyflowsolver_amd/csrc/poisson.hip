@@ -222,18 +222,24 @@ __global__ void k_dct2_rhs_fused(const double* __restrict__ ux, const double* __
   dct2_row_lds(A, B, nx, m, tw, t1, t2, out + (size_t)(j - 1) * nx);
 }
 
-// plain REDFT10 of rows (tests of the transform alone)
-__global__ void k_dct2_rows(const double* __restrict__ in, double* __restrict__ out, int n, int m,
-                            const double2* __restrict__ tw, const double2* __restrict__ t1,
-                            const double2* __restrict__ t2) {
+// plain REDFT10 of row r (0-based) of `in`
+__device__ __forceinline__ void dct2_rows_body(const double* __restrict__ in, double* __restrict__ out, int n, int m,
+                                               const double2* __restrict__ tw, const double2* __restrict__ t1,
+                                               const double2* __restrict__ t2, int r) {
   extern __shared__ double2 lds[];
   double2* A = lds;
   double2* B = lds + m;
   double* row = (double*)B;
-  const double* src = in + (size_t)blockIdx.x * n;
+  const double* src = in + (size_t)r * n;
   for (int t = threadIdx.x; t < n; t += blockDim.x) row[t] = src[t];
   __syncthreads();
-  dct2_row_lds(A, B, n, m, tw, t1, t2, out + (size_t)blockIdx.x * n);
+  dct2_row_lds(A, B, n, m, tw, t1, t2, out + (size_t)r * n);
+}
+
+__global__ void k_dct2_rows(const double* __restrict__ in, double* __restrict__ out, int n, int m,
+                            const double2* __restrict__ tw, const double2* __restrict__ t1,
+                            const double2* __restrict__ t2) {
+  dct2_rows_body(in, out, n, m, tw, t1, t2, blockIdx.x);
 }
 
 // REDFT01 of a row held in LDS (`row` aliasing B) into LDS; returns the
@@ -280,13 +286,13 @@ __global__ void k_dct3_rows(const double* __restrict__ in, double* __restrict__ 
 }
 
 // REDFT01 of mode row j, then psi[1](., j) with Neumann x ghosts and halo
-__global__ void k_dct3_psi(const double* __restrict__ in, double* __restrict__ psi, int nx,
-                           int ny, int m, const double2* __restrict__ tw,
-                           const double2* __restrict__ t1, const double2* __restrict__ t2) {
+__device__ __forceinline__ void dct3_psi_body(const double* __restrict__ in, double* __restrict__ psi, int nx,
+                                              int ny, int m, const double2* __restrict__ tw,
+                                              const double2* __restrict__ t1, const double2* __restrict__ t2,
+                                              int j) {
   extern __shared__ double2 lds[];
   double2* A = lds;
   double2* B = lds + m;
-  const int j = blockIdx.x + 1;
   double* row = (double*)B;
   const double* src = in + (size_t)(j - 1) * nx;
   for (int t = threadIdx.x; t < nx; t += blockDim.x) row[t] = src[t];
@@ -301,15 +307,20 @@ __global__ void k_dct3_psi(const double* __restrict__ in, double* __restrict__ p
   }
 }
 
+__global__ void k_dct3_psi(const double* __restrict__ in, double* __restrict__ psi, int nx,
+                           int ny, int m, const double2* __restrict__ tw,
+                           const double2* __restrict__ t1, const double2* __restrict__ t2) {
+  dct3_psi_body(in, psi, nx, ny, m, tw, t1, t2, blockIdx.x + 1);
+}
+
 // ---------------------------------------------------------------------------
 // tridiagonal solves, reference order: one Thomas + Sherman-Morrison chain
 // per x-mode (src/tdm.c:17-106 with constant l, c, u)
 // ---------------------------------------------------------------------------
-__global__ void k_tdm_sequential(double* __restrict__ q, double* __restrict__ v,
-                                 double* __restrict__ q1, int nx, int ny,
-                                 const double* __restrict__ evals, double l, double u,
-                                 int pin_mode0) {
-  const int mo = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void tdm_sequential_body(double* __restrict__ q, double* __restrict__ v,
+                                                    double* __restrict__ q1, int nx, int ny,
+                                                    const double* __restrict__ evals, double l, double u,
+                                                    int pin_mode0, int mo) {
   if (mo >= nx) return;
   const double c = -l - u + evals[mo];
   const int n = ny, N = ny - 1;
@@ -349,6 +360,33 @@ __global__ void k_tdm_sequential(double* __restrict__ q, double* __restrict__ v,
 #undef V
 #undef Q1
 }
+
+__global__ void k_tdm_sequential(double* __restrict__ q, double* __restrict__ v,
+                                 double* __restrict__ q1, int nx, int ny,
+                                 const double* __restrict__ evals, double l, double u,
+                                 int pin_mode0) {
+  tdm_sequential_body(q, v, q1, nx, ny, evals, l, u, pin_mode0, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// ensembles: one map entry (case, row or 64-mode group) per workgroup
+__global__ void k_dct2_rows_batch(const RowArgs* __restrict__ tab, const uint2* __restrict__ map) {
+  const uint2 mp = map[blockIdx.x];
+  const RowArgs& a = tab[mp.x];
+  dct2_rows_body(a.rhs, a.buf, a.nx, a.m, a.tw, a.t1, a.t2, (int)mp.y);
+}
+
+__global__ void k_tdm_sequential_batch(const RowArgs* __restrict__ tab, const uint2* __restrict__ map) {
+  const uint2 mp = map[blockIdx.x];
+  const RowArgs& a = tab[mp.x];
+  tdm_sequential_body(a.buf, a.v, a.q1, a.nx, a.ny, a.evals, a.l, a.l, a.pin, (int)mp.y * 64 + threadIdx.x);
+}
+
+__global__ void k_dct3_psi_batch(const RowArgs* __restrict__ tab, const uint2* __restrict__ map) {
+  const uint2 mp = map[blockIdx.x];
+  const RowArgs& a = tab[mp.x];
+  dct3_psi_body(a.buf, a.psi, a.nx, a.ny, a.m, a.tw, a.t1, a.t2, (int)mp.y + 1);
+}
+
 
 // ---------------------------------------------------------------------------
 // plan
@@ -499,6 +537,28 @@ int poisson_solve_fused(const domain_t* d, DomainCtx* c, const double* dt_dev, d
                                 p->t1, p->t2, p->buf));
   if (check_launch("poisson: rhs + DCT-II")) return 1;
   return poisson_tail(d, p, f);
+}
+
+// the Poisson arguments of one case (psi: the stage's psi[1])
+int poisson_row_args(const domain_t* d, DomainCtx* c, double* psi, RowArgs* a) {
+  PoissonPlan* p = plan_get(d, c);
+  if (!p) return 1;
+  if (!p->rhs) SBFS_HIP(hipMalloc(&p->rhs, sizeof(double) * (size_t)p->nx * p->ny));
+  a->rhs = p->rhs;
+  a->buf = p->buf;
+  a->v = p->v;
+  a->q1 = p->q1;
+  a->psi = psi;
+  a->evals = p->evals;
+  a->tw = p->tw;
+  a->t1 = p->t1;
+  a->t2 = p->t2;
+  a->l = 1. / d->hy / d->hy;
+  a->nx = p->nx;
+  a->ny = p->ny;
+  a->m = p->m;
+  a->pin = options().gauge == SBFS_GAUGE_PINNED ? 1 : 0;
+  return 0;
 }
 
 }  // namespace sbfs

@@ -27,6 +27,25 @@ struct PoissonPlan {
   double* X = nullptr;       // [P][nx]: separators
 };
 
+// one case's Poisson transforms in an ensemble (sequential tridiagonal)
+struct RowArgs {
+  const double* rhs;  // right-hand side rows (the stage kernel's output)
+  double* buf;        // modes
+  double* v;
+  double* q1;
+  double* psi;        // psi[1] of the stage
+  const double* evals;
+  const double2* tw;
+  const double2* t1;
+  const double2* t2;
+  double l;
+  int nx, ny, m, pin;
+};
+int poisson_row_args(const domain_t* d, DomainCtx* c, double* psi, RowArgs* a);
+__global__ void k_dct2_rows_batch(const RowArgs* tab, const uint2* map);
+__global__ void k_tdm_sequential_batch(const RowArgs* tab, const uint2* map);
+__global__ void k_dct3_psi_batch(const RowArgs* tab, const uint2* map);
+
 int partition_chunk(int n, int requested);
 int partition_setup(PoissonPlan* p, const domain_t* d, int L);
 int partition_solve(PoissonPlan* p, int pin_mode0);

@@ -164,6 +164,20 @@ void prof_end();
     ::sbfs::prof_end();       \
   } while (0)
 
+// arguments of one case's decide_dt kernels in an ensemble (step.hip)
+struct DtArgs {
+  const double* ux;
+  const double* uy;
+  const double* hxxf;
+  double hy, cadv, dif0, dif1, dint;
+  double* partial;
+  double* dt;
+  int nx, ny, rows, gx, nblk;
+};
+int dt_args(const domain_t* d, DomainCtx* c, const fluid_t* f, const interface_t* s, DtArgs* a);
+__global__ void k_dt_partial_batch(const DtArgs* tab, const uint2* map);
+__global__ void k_dt_final_batch(const DtArgs* tab);
+
 // grid-dimension helpers
 inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
 

@@ -24,16 +24,14 @@ static const double kDtMax = 1.;                              // decide_dt.c:25
 // thread one column, so the loads of a thread are independent and in flight
 // together; a min does not depend on the combination order
 
-__global__ __launch_bounds__(kRedThreads) void k_dt_partial(const double* __restrict__ ux,
-                                                           const double* __restrict__ uy,
-                                                           const double* __restrict__ hxxf,
-                                                           double hy, int nx, int ny, int rows,
-                                                           double* __restrict__ partial) {
+__device__ __forceinline__ void dt_partial_body(const double* __restrict__ ux, const double* __restrict__ uy,
+                                                const double* __restrict__ hxxf, double hy, int nx, int ny,
+                                                int rows, double* __restrict__ partial, int bx, int by, int gx) {
   __shared__ double sh[kRedThreads / 64];
   const double small = 1.e-8;
   double m = kDtMax;
-  const int i = 1 + (int)(blockIdx.x * kRedThreads + threadIdx.x);
-  const int j0 = 1 + (int)blockIdx.y * rows;
+  const int i = 1 + (int)(bx * kRedThreads + threadIdx.x);
+  const int j0 = 1 + by * rows;
   if (i <= nx) {
     const double hx = hxxf[i - 1];
 #pragma unroll 16
@@ -45,12 +43,20 @@ __global__ __launch_bounds__(kRedThreads) void k_dt_partial(const double* __rest
     }
   }
   m = block_reduce<0>(m, sh);
-  if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = m;
+  if (threadIdx.x == 0) partial[by * gx + bx] = m;
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_dt_partial(const double* __restrict__ ux,
+                                                           const double* __restrict__ uy,
+                                                           const double* __restrict__ hxxf,
+                                                           double hy, int nx, int ny, int rows,
+                                                           double* __restrict__ partial) {
+  dt_partial_body(ux, uy, hxxf, hy, nx, ny, rows, partial, blockIdx.x, blockIdx.y, gridDim.x);
 }
 
 // combine the limits (decide_dt.c:157-167)
-__global__ void k_dt_final(const double* __restrict__ partial, int n, double cadv,
-                           double dif0, double dif1, double dint, double* __restrict__ dt) {
+__device__ __forceinline__ void dt_final_body(const double* __restrict__ partial, int n, double cadv, double dif0,
+                                              double dif1, double dint, double* __restrict__ dt) {
   __shared__ double sh[kRedThreads / 64];
   double m = kDtMax;
   for (int k = threadIdx.x; k < n; k += blockDim.x) m = fmin(m, partial[k]);
@@ -62,6 +68,48 @@ __global__ void k_dt_final(const double* __restrict__ partial, int n, double cad
     r = fmin(r, dint);
     dt[0] = r;
   }
+}
+
+__global__ void k_dt_final(const double* __restrict__ partial, int n, double cadv,
+                           double dif0, double dif1, double dint, double* __restrict__ dt) {
+  dt_final_body(partial, n, cadv, dif0, dif1, dint, dt);
+}
+
+// ensembles: one map entry (case, local block) per workgroup / one
+// workgroup per case
+__global__ __launch_bounds__(kRedThreads) void k_dt_partial_batch(const DtArgs* __restrict__ tab,
+                                                                 const uint2* __restrict__ map) {
+  const uint2 m = map[blockIdx.x];
+  const DtArgs& a = tab[m.x];
+  dt_partial_body(a.ux, a.uy, a.hxxf, a.hy, a.nx, a.ny, a.rows, a.partial, (int)(m.y % a.gx), (int)(m.y / a.gx), a.gx);
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_dt_final_batch(const DtArgs* __restrict__ tab) {
+  const DtArgs& a = tab[blockIdx.x];
+  dt_final_body(a.partial, a.nblk, a.cadv, a.dif0, a.dif1, a.dint, a.dt);
+}
+
+// the arguments of one case's decide_dt (dt limits from the host, as
+// compute_dt_device forms them)
+int dt_args(const domain_t* d, DomainCtx* c, const fluid_t* f, const interface_t* s, DtArgs* a) {
+  if (load_dt_coefs()) return 1;
+  const int nx = (int)c->nx, ny = (int)c->ny;
+  dt_limits(d, f, s, &a->dif0, &a->dif1, &a->dint);
+  int rows = 16;
+  while ((long)cdiv(nx, kRedThreads) * cdiv(ny, rows) > kRedSlots - 64) rows *= 2;
+  a->ux = (const double*)f->ux.data;
+  a->uy = (const double*)f->uy.data;
+  a->hxxf = c->hxxf;
+  a->hy = c->hy;
+  a->cadv = options().cadv;
+  a->partial = c->red;
+  a->dt = c->dt_dev;
+  a->nx = nx;
+  a->ny = ny;
+  a->rows = rows;
+  a->gx = (int)cdiv(nx, kRedThreads);
+  a->nblk = a->gx * (int)cdiv(ny, rows);
+  return 0;
 }
 
 // grid-only limits (decide_dt.c:79-130); host arithmetic identical to the

@@ -216,6 +216,19 @@ class Case:
         be.check(be.interface_create(C.byref(self.domain), We, C.byref(self.interface)), "interface_create")
         self._open = True
 
+    @classmethod
+    def view(cls, backend: Backend, domain: int, fluid: int, interface: int) -> "Case":
+        """A case over structs another object owns (an ensemble member):
+        the addresses of its domain_t, fluid_t and interface_t."""
+        self = cls.__new__(cls)
+        self.be = backend
+        self.domain = abi.DomainT.from_address(domain)
+        self.fluid = abi.FluidT.from_address(fluid)
+        self.interface = abi.InterfaceT.from_address(interface)
+        self.nx, self.ny = int(self.domain.mysizes[0]), int(self.domain.mysizes[1])
+        self._open = False
+        return self
+
     # -- lifetime ---------------------------------------------------------
     def close(self) -> None:
         if self._open:
@@ -372,3 +385,61 @@ def staged_step(case: Case) -> float:
         case.correct_velocity(dt_old, dt_new)
         case.update_pressure()
     return dt
+
+
+class CaseSpec(C.Structure):
+    """sbfs_case_t (include/sbfs.h)."""
+    _fields_ = [("nx", C.c_size_t), ("ny", C.c_size_t), ("lx", C.c_double), ("ly", C.c_double),
+                ("Re", C.c_double), ("Fr", C.c_double), ("denr", C.c_double), ("visr", C.c_double),
+                ("We", C.c_double)]
+
+
+class Ensemble:
+    """Independent cases stepped together (sbfs_ensemble_*); `cases[k]` is a
+    Case view of member k for set-up and read-back."""
+
+    def __init__(self, backend: Backend, specs: list):
+        self.be = backend
+        lib = backend.lib
+        lib.sbfs_ensemble_create.argtypes = [_P(CaseSpec), C.c_int, _P(C.c_void_p)]
+        lib.sbfs_ensemble_case.argtypes = [C.c_void_p, C.c_int, _P(C.c_void_p), _P(C.c_void_p), _P(C.c_void_p)]
+        lib.sbfs_ensemble_step.argtypes = [C.c_void_p, _P(C.c_double)]
+        lib.sbfs_ensemble_autotune.argtypes = [C.c_void_p, _P(C.c_int)]
+        lib.sbfs_ensemble_set_band.argtypes = [C.c_void_p, C.c_int]
+        lib.sbfs_ensemble_destroy.argtypes = [C.c_void_p]
+        arr = (CaseSpec * len(specs))(*[CaseSpec(s["nx"], s["ny"], s["lx"], s["ly"], s["Re"], s["Fr"], s["denr"],
+                                                 s["visr"], s["We"]) for s in specs])
+        self.handle = C.c_void_p()
+        backend.check(lib.sbfs_ensemble_create(arr, len(specs), C.byref(self.handle)), "sbfs_ensemble_create")
+        self.cases = []
+        for k in range(len(specs)):
+            d, f, i = C.c_void_p(), C.c_void_p(), C.c_void_p()
+            backend.check(lib.sbfs_ensemble_case(self.handle, k, C.byref(d), C.byref(f), C.byref(i)),
+                          "sbfs_ensemble_case")
+            self.cases.append(Case.view(backend, d.value, f.value, i.value))
+        self.lib = lib
+
+    def step(self) -> np.ndarray:
+        dts = np.zeros(len(self.cases))
+        self.be.check(self.lib.sbfs_ensemble_step(self.handle, dts.ctypes.data_as(_P(C.c_double))),
+                      "sbfs_ensemble_step")
+        return dts
+
+    def autotune(self) -> int:
+        b = C.c_int(0)
+        self.be.check(self.lib.sbfs_ensemble_autotune(self.handle, C.byref(b)), "sbfs_ensemble_autotune")
+        return b.value
+
+    def set_band(self, band: int) -> None:
+        self.be.check(self.lib.sbfs_ensemble_set_band(self.handle, band), "sbfs_ensemble_set_band")
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.sbfs_ensemble_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
