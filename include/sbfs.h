@@ -172,10 +172,10 @@ extern int sbfs_logging_values(const domain_t * domain, const fluid_t * fluid,
  * reference, which runs one case per process).  Every kernel of the fused
  * step runs once per Runge-Kutta stage for all cases together, and each of
  * the two pointer states a step alternates between is replayed as a HIP
- * graph.  Per case the step is integrate()'s fused step with the projection
- * folded into the next stage and the sequential tridiagonal solve, in the
- * precision sbfs_set_precision selects, with the plain curvature: a case
- * gives the same bits as when run alone with sbfs_set_tdm(SBFS_TDM_SEQUENTIAL).
+ * graph.  Per case the step is integrate()'s fused step (projection folded
+ * into the next stage, the tridiagonal algorithm and precision the options
+ * select when the ensemble is first stepped, plain curvature): a case gives
+ * the same bits as when run alone with the same options.
  * Cases are set up through their structs (sbfs_ensemble_case) with the
  * usual calls (sbfs_array_upload, the boundary routines).  Between ensemble
  * steps nothing else may swap a case's arrays (integrate() on a case does).

@@ -8,9 +8,9 @@
 // planned once (ensemble_plan_step, fused.hip, on copies of each case's
 // structs) and each state's launch sequence is captured into a HIP graph the
 // second time it runs.  Per case the step is the single-case fused step with
-// the projection folded and the sequential tridiagonal solve, so a case run
-// in an ensemble gives the same bits as the case run alone with
-// sbfs_set_tdm(SBFS_TDM_SEQUENTIAL, .).  The band height of the stage kernel
+// the projection folded and the tridiagonal algorithm the options select
+// for that case (partitioned or sequential), so a case run in an ensemble
+// gives the same bits as the case run alone.  The band height of the stage kernel
 // is the only tile shape the ensemble chooses; sbfs_ensemble_autotune times
 // the candidates on the ensemble itself.
 #include <hip/hip_runtime.h>
@@ -81,9 +81,9 @@ struct sbfs_ensemble {
   RowArgs* rows[2] = {};     // [3 * n]
   PostArgs* post[2] = {};
   uint2* maps = nullptr;
-  unsigned nb_dt = 0, nb_march = 0, nb_row = 0, nb_mode = 0, nb_post = 0;
-  unsigned off_march = 0, off_row = 0, off_mode = 0, off_post = 0;
-  size_t lds = 0;
+  unsigned nb_dt = 0, nb_march = 0, nb_row = 0, nb_mode = 0, nb_post = 0, nb_part = 0, nb_conv = 0;
+  unsigned off_march = 0, off_row = 0, off_mode = 0, off_post = 0, off_part = 0, off_conv = 0;
+  size_t lds = 0, lds_conv = 0;
   int row_threads = 256;
   double* dts = nullptr;
   hipGraphExec_t graph[2] = {};
@@ -157,9 +157,29 @@ static int plan(sbfs_ensemble* e) {
     mmax = std::max(mmax, steps[0][c].rows[0].m);
   }
   e->nb_row = (unsigned)map.size() - e->off_row;
+  // tridiagonal: sequential cases by 64-mode groups; partitioned cases by
+  // tiles of 64 modes x 4 chunks and by groups of the convolution's modes
   e->off_mode = (unsigned)map.size();
-  for (int c = 0; c < n; c++) add(c, cdiv(steps[0][c].rows[0].nx, 64));
+  for (int c = 0; c < n; c++)
+    if (steps[0][c].rows[0].L == 0) add(c, cdiv(steps[0][c].rows[0].nx, 64));
   e->nb_mode = (unsigned)map.size() - e->off_mode;
+  e->off_part = (unsigned)map.size();
+  int pmax = 1;
+  for (int c = 0; c < n; c++) {
+    const auto& r = steps[0][c].rows[0];
+    if (r.L != 0) {
+      add(c, cdiv(r.nx, 64) * cdiv(r.P, 4));
+      pmax = std::max(pmax, r.P);
+    }
+  }
+  e->nb_part = (unsigned)map.size() - e->off_part;
+  e->off_conv = (unsigned)map.size();
+  for (int c = 0; c < n; c++) {
+    const auto& r = steps[0][c].rows[0];
+    if (r.L != 0) add(c, cdiv(r.nx, kConvModesBatch));
+  }
+  e->nb_conv = (unsigned)map.size() - e->off_conv;
+  e->lds_conv = sizeof(double) * 3 * (size_t)pmax * kConvModesBatch;
   e->off_post = (unsigned)map.size();
   for (int c = 0; c < n; c++) {
     const auto& q = steps[0][c].post;
@@ -223,8 +243,17 @@ static int launch_step(sbfs_ensemble* e, int p) {
       SBFS_TIMED("k_march", hipLaunchKernelGGL(k_march_batch, dim3(e->nb_march), dim3(64), 0, stream(), ma, mm));
     SBFS_TIMED("k_dct2_rows", hipLaunchKernelGGL(k_dct2_rows_batch, dim3(e->nb_row), dim3(e->row_threads), e->lds,
                                                  stream(), ro, mp + e->off_row));
-    SBFS_TIMED("k_tdm_sequential", hipLaunchKernelGGL(k_tdm_sequential_batch, dim3(e->nb_mode), dim3(64), 0,
-                                                      stream(), ro, mp + e->off_mode));
+    if (e->nb_mode)
+      SBFS_TIMED("k_tdm_sequential", hipLaunchKernelGGL(k_tdm_sequential_batch, dim3(e->nb_mode), dim3(64), 0,
+                                                        stream(), ro, mp + e->off_mode));
+    if (e->nb_part) {
+      SBFS_TIMED("k_part_reduce", hipLaunchKernelGGL(k_part_reduce_batch, dim3(e->nb_part), dim3(64, 4), 0,
+                                                     stream(), ro, mp + e->off_part));
+      SBFS_TIMED("k_part_conv", hipLaunchKernelGGL(k_part_conv_batch, dim3(e->nb_conv), dim3(256), e->lds_conv,
+                                                   stream(), ro, mp + e->off_conv));
+      SBFS_TIMED("k_part_expand", hipLaunchKernelGGL(k_part_expand_batch, dim3(e->nb_part), dim3(64, 4), 0,
+                                                     stream(), ro, mp + e->off_part));
+    }
     SBFS_TIMED("k_dct3_psi", hipLaunchKernelGGL(k_dct3_psi_batch, dim3(e->nb_row), dim3(e->row_threads), e->lds,
                                                 stream(), ro, mp + e->off_row));
   }

@@ -558,6 +558,25 @@ int poisson_row_args(const domain_t* d, DomainCtx* c, double* psi, RowArgs* a) {
   a->ny = p->ny;
   a->m = p->m;
   a->pin = options().gauge == SBFS_GAUGE_PINNED ? 1 : 0;
+  // the tridiagonal algorithm poisson_tail would take for this case
+  int req = options().chunk;
+  while (req < 64 && p->ny / req > 128) req *= 2;
+  const int L = partition_chunk(p->ny, req);
+  a->L = 0;
+  a->P = 0;
+  a->ptab = a->htab = nullptr;
+  a->gfl = a->X = nullptr;
+  a->pa = 0.;
+  if (options().tdm == SBFS_TDM_PARTITIONED && L != 0 && a->pin && p->ny / L <= 128) {
+    if (p->L != L && partition_setup(p, d, L)) return 1;
+    a->L = L;
+    a->P = p->P;
+    a->ptab = p->ptab;
+    a->htab = p->htab;
+    a->gfl = p->gfl;
+    a->X = p->X;
+    a->pa = p->a;
+  }
   return 0;
 }
 

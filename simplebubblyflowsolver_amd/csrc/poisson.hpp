@@ -40,11 +40,22 @@ struct RowArgs {
   const double2* t2;
   double l;
   int nx, ny, m, pin;
+  // partitioned solve (L != 0; L == 0: sequential)
+  const double* ptab;
+  const double* htab;
+  double* gfl;
+  double* X;
+  double pa;
+  int L, P;
 };
 int poisson_row_args(const domain_t* d, DomainCtx* c, double* psi, RowArgs* a);
 __global__ void k_dct2_rows_batch(const RowArgs* tab, const uint2* map);
 __global__ void k_tdm_sequential_batch(const RowArgs* tab, const uint2* map);
 __global__ void k_dct3_psi_batch(const RowArgs* tab, const uint2* map);
+__global__ void k_part_reduce_batch(const RowArgs* tab, const uint2* map);
+__global__ void k_part_conv_batch(const RowArgs* tab, const uint2* map);
+__global__ void k_part_expand_batch(const RowArgs* tab, const uint2* map);
+constexpr int kConvModesBatch = 16;  // modes per workgroup of k_part_conv_batch
 
 int partition_chunk(int n, int requested);
 int partition_setup(PoissonPlan* p, const domain_t* d, int L);

@@ -311,4 +311,123 @@ int partition_solve(PoissonPlan* p, int pin) {
   return check_launch("poisson: partitioned tridiagonal");
 }
 
+// ---------------------------------------------------------------------------
+// ensembles: the three kernels for every case in one launch each.  A case's
+// chunk length L is uniform over its workgroups, so the switch over the
+// instantiations does not diverge.
+// ---------------------------------------------------------------------------
+template <int L>
+__device__ __forceinline__ void part_reduce_body(const RowArgs& a, int mo, int ch) {
+  constexpr int K = L - 1;
+  if (mo >= a.nx || ch >= a.P) return;
+  const double* inv = a.ptab;
+  const double* vv = a.ptab + (size_t)K * a.nx;
+  double g[K];
+  chunk_thomas<K>(a.buf, a.nx, mo, ch * L, inv, vv, a.pa, g);
+  a.gfl[(size_t)(2 * ch) * a.nx + mo] = g[0];
+  a.gfl[(size_t)(2 * ch + 1) * a.nx + mo] = g[K - 1];
+}
+
+template <int L>
+__device__ __forceinline__ void part_expand_body(const RowArgs& a, int mo, int ch) {
+  constexpr int K = L - 1;
+  const int nx = a.nx, P = a.P;
+  if (mo >= nx || ch >= P) return;
+  const double* inv = a.ptab;
+  const double* vv = a.ptab + (size_t)K * nx;
+  const double* w = a.ptab + (size_t)2 * K * nx;
+  const double* z = a.ptab + (size_t)3 * K * nx;
+  double g[K];
+  chunk_thomas<K>(a.buf, nx, mo, ch * L, inv, vv, a.pa, g);
+  const double xl = a.X[(size_t)((ch + P - 1) % P) * nx + mo];
+  const double xu = a.X[(size_t)ch * nx + mo];
+#pragma unroll
+  for (int k = 0; k < K; k++)
+    a.buf[(size_t)(ch * L + k) * nx + mo] = (g[k] + xl * w[(size_t)k * nx + mo]) + xu * z[(size_t)k * nx + mo];
+  a.buf[(size_t)(ch * L + K) * nx + mo] = xu;
+}
+
+#define SBFS_PART_SWITCH(BODY)                  \
+  switch (a.L) {                                \
+    case 2: BODY<2>(a, mo, ch); break;          \
+    case 4: BODY<4>(a, mo, ch); break;          \
+    case 8: BODY<8>(a, mo, ch); break;          \
+    case 16: BODY<16>(a, mo, ch); break;        \
+    case 32: BODY<32>(a, mo, ch); break;        \
+    default: BODY<64>(a, mo, ch); break;        \
+  }
+
+// map entry (case, tile of 64 modes x 4 chunks)
+__global__ __launch_bounds__(256) void k_part_reduce_batch(const RowArgs* __restrict__ tab,
+                                                           const uint2* __restrict__ map) {
+  const uint2 m = map[blockIdx.x];
+  const RowArgs& a = tab[m.x];
+  const int gx = (a.nx + 63) / 64;
+  const int mo = (int)(m.y % gx) * 64 + threadIdx.x, ch = (int)(m.y / gx) * 4 + threadIdx.y;
+  SBFS_PART_SWITCH(part_reduce_body)
+}
+
+__global__ __launch_bounds__(256) void k_part_expand_batch(const RowArgs* __restrict__ tab,
+                                                           const uint2* __restrict__ map) {
+  const uint2 m = map[blockIdx.x];
+  const RowArgs& a = tab[m.x];
+  const int gx = (a.nx + 63) / 64;
+  const int mo = (int)(m.y % gx) * 64 + threadIdx.x, ch = (int)(m.y / gx) * 4 + threadIdx.y;
+  SBFS_PART_SWITCH(part_expand_body)
+}
+#undef SBFS_PART_SWITCH
+
+// map entry (case, group of kConvModes modes); LDS for the largest P
+__global__ __launch_bounds__(256) void k_part_conv_batch(const RowArgs* __restrict__ tab,
+                                                         const uint2* __restrict__ map) {
+  const uint2 mp = map[blockIdx.x];
+  const RowArgs& a = tab[mp.x];
+  const int nx = a.nx, P = a.P, L = a.L, K = L - 1;
+  constexpr int G = 256 / kConvModes;
+  extern __shared__ double lds[];
+  double* R = lds;
+  double* H = lds + (size_t)P * kConvModes;
+  double* Y = lds + (size_t)2 * P * kConvModes;
+  const int tm = threadIdx.x % kConvModes, g = threadIdx.x / kConvModes;
+  const int mo = (int)mp.y * kConvModes + tm;
+  const bool live = mo < nx;
+  const int mm = live ? mo : nx - 1;
+  for (int c = g; c < P; c += G) {
+    const double gl = a.gfl[(size_t)(2 * c + 1) * nx + mm];
+    const double gf = a.gfl[(size_t)(2 * ((c + 1) % P)) * nx + mm];
+    R[c * kConvModes + tm] = (a.buf[(size_t)(c * L + K) * nx + mm] - a.pa * gl) - a.pa * gf;
+    H[c * kConvModes + tm] = a.htab[(size_t)c * nx + mm];
+  }
+  __syncthreads();
+  for (int i0 = g * kConvS; i0 < P; i0 += G * kConvS) {
+    double acc[kConvS], w[kConvS];
+#pragma unroll
+    for (int r = 0; r < kConvS; r++) {
+      acc[r] = 0.;
+      w[r] = H[((i0 + r) % P) * kConvModes + tm];
+    }
+    int next = i0 > 0 ? i0 - 1 : P - 1;
+    for (int k = 0; k < P; k++) {
+      const double rk = R[k * kConvModes + tm];
+#pragma unroll
+      for (int r = 0; r < kConvS; r++) acc[r] += w[r] * rk;
+#pragma unroll
+      for (int r = kConvS - 1; r > 0; r--) w[r] = w[r - 1];
+      w[0] = H[next * kConvModes + tm];
+      next = next > 0 ? next - 1 : P - 1;
+    }
+#pragma unroll
+    for (int r = 0; r < kConvS; r++)
+      if (i0 + r < P) Y[(i0 + r) * kConvModes + tm] = acc[r];
+  }
+  __syncthreads();
+  const bool pin = a.pin && mo == 0;
+  const double xl = Y[(P - 1) * kConvModes + tm];
+  if (!live) return;
+  for (int i = g; i < P; i += G) {
+    const double x = Y[i * kConvModes + tm];
+    a.X[(size_t)i * nx + mo] = pin ? x - xl : x;
+  }
+}
+
 }  // namespace sbfs

@@ -1,8 +1,8 @@
 """Ensembles of independent cases (sbfs_ensemble_*): every case of a batched
-step gives the same bits as the case stepped alone with the sequential
-tridiagonal solve, over steps that cover both pointer states, the eager
-steps, the graph capture and its replays; the autotuner restores the state
-it times the candidates on."""
+step gives the same bits as the case stepped alone with the same options
+(partitioned or sequential tridiagonal solve, FP64 or mixed), over steps
+that cover both pointer states, the eager steps, the graph capture and its
+replays; the autotuner restores the state it times the candidates on."""
 import numpy as np
 import pytest
 
@@ -26,7 +26,6 @@ def small_ensemble():
 
 
 def alone(product, w, steps):
-    assert product.lib.sbfs_set_tdm(0, 32) == 0
     g = solver.Case(product, w.nx, w.ny, w.lx, w.ly, w.Re, w.Fr, w.denr, w.visr, w.We)
     try:
         g.load_initial_condition(w.vof())
@@ -34,14 +33,14 @@ def alone(product, w, steps):
         return dts, {n: g.get(n) for n in FIELDS}
     finally:
         g.close()
-        product.lib.sbfs_set_tdm(1, 32)
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_ensemble_matches_cases_alone(product, precision):
+@pytest.mark.parametrize("precision,tdm", [(0, 1), (1, 1), (1, 0)])
+def test_ensemble_matches_cases_alone(product, precision, tdm):
     ws = small_ensemble()
     steps = 6
     assert product.lib.sbfs_set_precision(precision) == 0
+    assert product.lib.sbfs_set_tdm(tdm, 32) == 0
     try:
         e = solver.Ensemble(product, specs_of(ws))
         try:
@@ -60,6 +59,7 @@ def test_ensemble_matches_cases_alone(product, precision):
                 assert np.array_equal(got[k][n], f1[n]), (k, n)
     finally:
         product.lib.sbfs_set_precision(0)
+        product.lib.sbfs_set_tdm(1, 32)
 
 
 def test_autotune_restores_state(product):
