@@ -17,6 +17,11 @@ without per-launch events, as a production run does); its algorithmic bytes
 per cell (distinct fields read + written once, FP64) are listed in
 BYTES_PER_CELL.  `cpu_baseline` times the CPU oracle (the FP64 C
 restatement, OpenMP) on a bounded sample of the same workload.
+
+Beside the headline: `mixed` reruns the workload in mixed precision, and
+`mixed.learned_curvature` with the learned curvature after training its
+network on the device; `ensemble` steps 256 independent rising bubbles of
+uneven grids as one batched ensemble.
 """
 from __future__ import annotations
 
@@ -222,6 +227,44 @@ def learned_run(be, w, warmup, steps, t_fp64, d0, d1_fp64) -> dict:
     return out
 
 
+def ensemble_run(be, steps: int) -> dict:
+    """The batched ensemble: 256 independent rising bubbles of uneven grids
+    (configs.ensemble), mixed precision, band autotuned, `steps` timed steps
+    after 6 untimed ones; Mcell-steps/s summed over the cases."""
+    lib = be.lib
+    ws = configs.ensemble(256, seed=3)
+    cells = sum(w.nx * w.ny for w in ws)
+    be.check(lib.sbfs_set_precision(1), "set_precision")
+    try:
+        e = solver.Ensemble(be, [dict(nx=w.nx, ny=w.ny, lx=w.lx, ly=w.ly, Re=w.Re, Fr=w.Fr, denr=w.denr,
+                                      visr=w.visr, We=w.We) for w in ws])
+        try:
+            for c, w in zip(e.cases, ws):
+                c.load_initial_condition(w.vof())
+            for _ in range(2):
+                e.step()
+            band = e.autotune()
+            for _ in range(4):
+                e.step()
+            be.check(lib.sbfs_synchronize(), "sync")
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                e.step()
+            be.check(lib.sbfs_synchronize(), "sync")
+            t = time.perf_counter() - t0
+            vols = [c.diagnostics()["volume"] for c in e.cases[:8]]
+        finally:
+            e.close()
+    finally:
+        be.check(lib.sbfs_set_precision(0), "set_precision")
+    sizes = {}
+    for w in ws:
+        sizes["%dx%d" % (w.nx, w.ny)] = sizes.get("%dx%d" % (w.nx, w.ny), 0) + 1
+    return {"value": round(cells * steps / t / 1e6, 3), "unit": "Mcell-steps/s", "ms_per_step": round(t / steps * 1e3, 4),
+            "steps": steps, "cases": len(ws), "cells": cells, "grids": sizes, "band_autotuned": band,
+            "precision": "mixed", "finite": bool(np.all(np.isfinite(vols)))}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -238,6 +281,9 @@ def main() -> int:
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-mixed", dest="mixed", action="store_false",
                     help="skip the mixed-precision comparison run")
+    ap.add_argument("--no-ensemble", dest="ensemble", action="store_false",
+                    help="skip the batched-ensemble run")
+    ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--profile-steps", type=int, default=20,
                     help="steps of the separate per-kernel timing pass")
     a = ap.parse_args()
@@ -328,6 +374,8 @@ def main() -> int:
         "kernels": kernels,
         "mixed": mixed,
     }
+    if a.ensemble and a.path == "fused":
+        out["ensemble"] = ensemble_run(be, a.ensemble_steps)
     if not a.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(w, a.cpu_steps, threads)

@@ -347,7 +347,8 @@ extern "C" int sbfs_ensemble_create(const sbfs_case_t* cases, int n, sbfs_ensemb
       const size_t sizes[] = {k.s.vof.datasize, k.f.ux.datasize, k.f.ux.datasize,
                               k.f.uy.datasize,  k.f.uy.datasize, k.f.p.datasize};
       for (int b = 0; b < 6 && !rc; b++) {
-        if (hipMalloc(bufs[b], sizes[b]) != hipSuccess || hipMemset(*bufs[b], 0, sizes[b]) != hipSuccess) {
+        if (hipMalloc(bufs[b], sizes[b]) != hipSuccess ||
+            hipMemsetAsync(*bufs[b], 0, sizes[b], stream()) != hipSuccess) {
           set_error("sbfs_ensemble_create: out of device memory");
           rc = 1;
         }

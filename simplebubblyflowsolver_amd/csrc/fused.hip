@@ -508,7 +508,7 @@ static int scratch(const domain_t* d, const interface_t* s, const fluid_t* f) {
                                               {&g_scr.uy2, f->uy.datasize},  {&g_scr.p, f->p.datasize}};
   for (const auto& b : bufs) {
     SBFS_HIP(hipMalloc(b.first, b.second));
-    SBFS_HIP(hipMemset(*b.first, 0, b.second));
+    SBFS_HIP(hipMemsetAsync(*b.first, 0, b.second, stream()));
   }
   SBFS_HIP(hipMalloc(&g_scr.kappa, sizeof(float) * d->mysizes[0] * d->mysizes[1]));
   g_scr.nx = d->mysizes[0];

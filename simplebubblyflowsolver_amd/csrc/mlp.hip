@@ -622,7 +622,9 @@ static int net_alloc() {
   SBFS_HIP(hipMalloc(&g_net.w1b, sizeof(unsigned short) * (NH * NIP + 2 * NH * NH)));
   g_net.w2b = g_net.w1b + NH * NIP;
   g_net.w2tb = g_net.w2b + NH * NH;
-  SBFS_HIP(hipMemset(g_net.p, 0, sizeof(float) * NP * 3));
+  // on the library stream (non-blocking): a null-stream memset is not
+  // ordered with the parameter upload queued right after it
+  SBFS_HIP(hipMemsetAsync(g_net.p, 0, sizeof(float) * NP * 3, stream()));
   return 0;
 }
 
