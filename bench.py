@@ -58,8 +58,8 @@ BYTES_PER_CELL = {
     "k_density": 16, "k_visux": 16, "k_visuy": 16, "k_dvof": 24, "k_normal_curv": 56,
     "k_force": 40, "k_fluxx": 48, "k_fluxy": 48, "k_vof_advance": 48, "k_mass_flux": 48,
     "k_stress": 56, "k_rhs_ux": 80, "k_rhs_uy": 72, "k_predict_ux": 56, "k_predict_uy": 56,
-    "k_dct2_rhs": 40, "k_tdm_sequential": 16, "k_part_reduce": 8, "k_part_conv": 0,
-    "k_part_expand": 16, "k_dct3_psi": 16, "k_correct_ux": 40, "k_correct_uy": 40,
+    "k_dct2_rhs": 40, "k_tdm_sequential": 16, "k_part_reduce": 16, "k_part_conv": 0,
+    "k_dct3_psi": 16, "k_correct_ux": 40, "k_correct_uy": 40,
     "k_update_p": 24, "k_dt_partial": 16, "k_dt_final": 0,
     # fused path
     "k_march": (12 + 15 + 14) * 8 / 3, "k_march_mixed": (12 + 15 + 14) * 8 / 3, "k_pre": (10 + 13 + 12) * 8 / 3, "k_dct2_rows": 16,

@@ -84,7 +84,7 @@ struct sbfs_ensemble {
   unsigned nb_dt = 0, nb_march = 0, nb_row = 0, nb_mode = 0, nb_post = 0, nb_part = 0, nb_conv = 0;
   unsigned off_march = 0, off_row = 0, off_mode = 0, off_post = 0, off_part = 0, off_conv = 0;
   size_t lds = 0, lds_conv = 0;
-  int row_threads = 256;
+  int row_threads = 64;
   double* dts = nullptr;
   hipGraphExec_t graph[2] = {};
   int runs[2] = {};
@@ -137,7 +137,7 @@ static int plan(sbfs_ensemble* e) {
     }
     SBFS_REQUIRE(same_state(st, k.start[0]), "ensemble: the step's pointer state does not return after two steps");
   }
-  // block maps: dt partials, stage kernel, Poisson rows, 64-mode groups, projection tiles
+  // block maps: dt partials, stage kernel, groups of Poisson rows, 64-mode groups, projection tiles
   std::vector<uint2> map;
   auto add = [&](int c, unsigned count) {
     for (unsigned b = 0; b < count; b++) map.push_back(make_uint2((unsigned)c, b));
@@ -151,10 +151,16 @@ static int plan(sbfs_ensemble* e) {
   }
   e->nb_march = (unsigned)map.size() - e->off_march;
   e->off_row = (unsigned)map.size();
-  int mmax = 1;
+  // row transforms: a workgroup of row_threads threads holds
+  // row_threads / unit units of the case's transform length
+  e->row_threads = 64;
+  for (int c = 0; c < n; c++) e->row_threads = std::max(e->row_threads, wave_unit(steps[0][c].rows[0].m));
+  size_t lds_rows = 0;
   for (int c = 0; c < n; c++) {
-    add(c, (unsigned)steps[0][c].rows[0].ny);
-    mmax = std::max(mmax, steps[0][c].rows[0].m);
+    const auto& r = steps[0][c].rows[0];
+    const int units = e->row_threads / wave_unit(r.m);
+    add(c, cdiv(r.ny, units * wave_rows(r.m)));
+    lds_rows = std::max(lds_rows, units * wave_lds(r.m));
   }
   e->nb_row = (unsigned)map.size() - e->off_row;
   // tridiagonal: sequential cases by 64-mode groups; partitioned cases by
@@ -186,8 +192,7 @@ static int plan(sbfs_ensemble* e) {
     add(c, cdiv(q.nx + 2, 64) * cdiv(q.ny, 4));
   }
   e->nb_post = (unsigned)map.size() - e->off_post;
-  e->lds = sizeof(double2) * 2 * (size_t)mmax;
-  e->row_threads = mmax >= 1024 ? 512 : 256;
+  e->lds = lds_rows;
   SBFS_HIP(hipMalloc(&e->maps, sizeof(uint2) * map.size()));
   SBFS_HIP(hipMemcpy(e->maps, map.data(), sizeof(uint2) * map.size(), hipMemcpyHostToDevice));
   for (int p = 0; p < 2; p++) {
@@ -241,8 +246,8 @@ static int launch_step(sbfs_ensemble* e, int p) {
                  hipLaunchKernelGGL(k_march_corr_batch, dim3(e->nb_march), dim3(64), 0, stream(), ma, mm));
     else
       SBFS_TIMED("k_march", hipLaunchKernelGGL(k_march_batch, dim3(e->nb_march), dim3(64), 0, stream(), ma, mm));
-    SBFS_TIMED("k_dct2_rows", hipLaunchKernelGGL(k_dct2_rows_batch, dim3(e->nb_row), dim3(e->row_threads), e->lds,
-                                                 stream(), ro, mp + e->off_row));
+    SBFS_TIMED("k_dct2_rows", hipLaunchKernelGGL(k_dct2_rows_batch, dim3(e->nb_row), dim3(e->row_threads), e->lds, stream(),
+                                                 ro, mp + e->off_row));
     if (e->nb_mode)
       SBFS_TIMED("k_tdm_sequential", hipLaunchKernelGGL(k_tdm_sequential_batch, dim3(e->nb_mode), dim3(64), 0,
                                                         stream(), ro, mp + e->off_mode));
@@ -251,11 +256,9 @@ static int launch_step(sbfs_ensemble* e, int p) {
                                                      stream(), ro, mp + e->off_part));
       SBFS_TIMED("k_part_conv", hipLaunchKernelGGL(k_part_conv_batch, dim3(e->nb_conv), dim3(256), e->lds_conv,
                                                    stream(), ro, mp + e->off_conv));
-      SBFS_TIMED("k_part_expand", hipLaunchKernelGGL(k_part_expand_batch, dim3(e->nb_part), dim3(64, 4), 0,
-                                                     stream(), ro, mp + e->off_part));
     }
-    SBFS_TIMED("k_dct3_psi", hipLaunchKernelGGL(k_dct3_psi_batch, dim3(e->nb_row), dim3(e->row_threads), e->lds,
-                                                stream(), ro, mp + e->off_row));
+    SBFS_TIMED("k_dct3_psi", hipLaunchKernelGGL(k_dct3_psi_batch, dim3(e->nb_row), dim3(e->row_threads), e->lds, stream(), ro,
+                                                mp + e->off_row));
   }
   SBFS_TIMED("k_post", hipLaunchKernelGGL(k_post_batch, dim3(e->nb_post), dim3(64, 4), 0, stream(),
                                           (const PostArgs*)e->post[p], mp + e->off_post));

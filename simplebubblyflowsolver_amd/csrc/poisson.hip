@@ -5,13 +5,18 @@
 //                  the row in LDS (assign_input, :285-363) and applies
 //                  FFTW's REDFT10 along x through an N/2-point complex
 //                  Stockham FFT in LDS; writes the row of modes r[j][m].
+//                  The marching path's k_dct2_rows transforms rows that the
+//                  stage kernel wrote, one wavefront per group of rows
+//                  (rowfft.hpp), with the same arithmetic.
 //  2. tridiagonal  per x-mode m along the periodic y direction, directly on
 //                  the [j][m] layout (consecutive modes are consecutive
 //                  addresses, so every sweep step is a coalesced row access;
 //                  the reference's x1 <-> y1 pencil transposes disappear).
-//  3. k_dct3_psi   one workgroup per row: REDFT01 along x, then psi[1] with
-//                  its Neumann x ghosts and periodic y halo (extract_output,
-//                  :365-380, src/fluid/boundary/psi.c).
+//  3. k_dct3_psi   one wavefront per group of rows: loads the solved modes
+//                  (expanding the partitioned solve's chunks on the way),
+//                  REDFT01 along x, then psi[1] with its Neumann x ghosts
+//                  and periodic y halo (extract_output, :365-380,
+//                  src/fluid/boundary/psi.c).
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -19,6 +24,7 @@
 
 #include "sbfs.h"
 #include "poisson.hpp"
+#include "rowfft.hpp"
 #include "sbfs_internal.hpp"
 #include "stencils.hpp"
 
@@ -222,96 +228,240 @@ __global__ void k_dct2_rhs_fused(const double* __restrict__ ux, const double* __
   dct2_row_lds(A, B, nx, m, tw, t1, t2, out + (size_t)(j - 1) * nx);
 }
 
-// plain REDFT10 of row r (0-based) of `in`
-__device__ __forceinline__ void dct2_rows_body(const double* __restrict__ in, double* __restrict__ out, int n, int m,
-                                               const double2* __restrict__ tw, const double2* __restrict__ t1,
-                                               const double2* __restrict__ t2, int r) {
-  extern __shared__ double2 lds[];
-  double2* A = lds;
-  double2* B = lds + m;
-  double* row = (double*)B;
-  const double* src = in + (size_t)r * n;
-  for (int t = threadIdx.x; t < n; t += blockDim.x) row[t] = src[t];
-  __syncthreads();
-  dct2_row_lds(A, B, n, m, tw, t1, t2, out + (size_t)r * n);
-}
+// ---------------------------------------------------------------------------
+// Row transforms of the marching path, the partitioned solve's expansion and
+// the tests: one wavefront per group of R rows (rowfft.hpp).  The arithmetic
+// is dct2_row_lds / fft_lds's, operation for operation.
+// ---------------------------------------------------------------------------
 
-__global__ void k_dct2_rows(const double* __restrict__ in, double* __restrict__ out, int n, int m,
-                            const double2* __restrict__ tw, const double2* __restrict__ t1,
-                            const double2* __restrict__ t2) {
-  dct2_rows_body(in, out, n, m, tw, t1, t2, blockIdx.x);
-}
-
-// REDFT01 of a row held in LDS (`row` aliasing B) into LDS; returns the
-// buffer holding the n output doubles
-__device__ double* dct3_row_lds(double2* A, double2* B, int n, int m,
-                                const double2* __restrict__ tw, const double2* __restrict__ t1,
-                                const double2* __restrict__ t2) {
-  const int tid = threadIdx.x, nt = blockDim.x;
-  const double* x = (const double*)B;
-  for (int k = tid; k < m; k += nt) {
-    const double pr = x[k], pim = -((k == 0) ? 0. : x[n - k]);
-    const double2 vk = cmul(make_double2(pr, pim), make_double2(t2[k].x, -t2[k].y));
-    const double qr = x[k + m], qim = -x[m - k];
-    const double2 vm = cmul(make_double2(qr, qim), make_double2(t2[k + m].x, -t2[k + m].y));
-    const double sr = vk.x + vm.x, si = vk.y + vm.y;
-    const double2 f = cmul(make_double2(vk.x - vm.x, vk.y - vm.y), make_double2(t1[k].x, -t1[k].y));
-    A[k] = make_double2(sr - f.y, -(si + f.x));
+// REDFT10 of rows row0 .. row0 + R - 1 (of `rows`) of `in` into `out`
+// the unit's rows in LDS: row rr at lds + rr PM, doubles through zd().
+// Rows past the last (rr >= nrows >= 1) load the last row again, so every
+// load is unconditional and all are in flight before the first LDS write.
+template <int LOG, typename Src>
+__device__ __forceinline__ void rows_to_lds(double2* lds, int lane, int nrows, Src src) {
+  using F = WaveFft<LOG>;
+  constexpr int H = F::N / 2;  // double2 per row
+  if constexpr (F::R * H >= F::U) {
+    constexpr int PER = F::R * H / F::U;
+    double2 v[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int t = lane + i * F::U;
+      v[i] = src(min(t / H, nrows - 1), t % H);
+    }
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int t = lane + i * F::U;
+      lds[(t / H) * F::PM + fpad(t % H)] = v[i];
+    }
+  } else {
+    const int rr = lane / H, h = lane % H;
+    if (lane < F::R * H) lds[rr * F::PM + fpad(h)] = src(min(rr, nrows - 1), h);
   }
-  __syncthreads();
-  double2* z = fft_lds(A, B, m, tw);
-  double* y = (double*)((z == A) ? B : A);
-  for (int k = tid; k < m; k += nt) {
-    const double v0 = z[k].x, v1 = -z[k].y;
+}
+
+// (the unit's LDS rows at `lds`, its thread `lane` in [0, U))
+template <int LOG>
+__device__ __forceinline__ void dct2_wave(const double* __restrict__ in, double* __restrict__ out, int rows,
+                                          int row0, const double2* __restrict__ tw,
+                                          const double2* __restrict__ t1, const double2* __restrict__ t2,
+                                          double2* lds, int lane) {
+  using F = WaveFft<LOG>;
+  constexpr int M = F::M, N = F::N, G = F::G;
+  const int rg = lane / G, lt = lane % G;
+  const int nrows = min(F::R, rows - row0);
+  if (nrows <= 0) return;  // a second unit past the last row (one wavefront, no barrier)
+  const double2* src = (const double2*)(in + (size_t)row0 * N);
+  rows_to_lds<LOG>(lds, lane, nrows, [src](int rr, int h) { return src[rr * (N / 2) + h]; });
+  F::sync();
+  double2* Z = lds + rg * F::PM;
+  // v[k] = x[2k], v[n-1-k] = x[2k+1]; z[k] = v[2k] + i v[2k+1]
+  auto ld0 = [Z](int k) {
     const int e = 2 * k, o = 2 * k + 1;
-    if (e < m) y[2 * e] = v0; else y[2 * (n - 1 - e) + 1] = v0;
-    if (o < m) y[2 * o] = v1; else y[2 * (n - 1 - o) + 1] = v1;
-  }
-  __syncthreads();
-  return y;
-}
-
-__global__ void k_dct3_rows(const double* __restrict__ in, double* __restrict__ out, int n, int m,
-                            const double2* __restrict__ tw, const double2* __restrict__ t1,
-                            const double2* __restrict__ t2) {
-  extern __shared__ double2 lds[];
-  double2* A = lds;
-  double2* B = lds + m;
-  double* row = (double*)B;
-  const double* src = in + (size_t)blockIdx.x * n;
-  for (int t = threadIdx.x; t < n; t += blockDim.x) row[t] = src[t];
-  __syncthreads();
-  const double* y = dct3_row_lds(A, B, n, m, tw, t1, t2);
-  for (int t = threadIdx.x; t < n; t += blockDim.x) out[(size_t)blockIdx.x * n + t] = y[t];
-}
-
-// REDFT01 of mode row j, then psi[1](., j) with Neumann x ghosts and halo
-__device__ __forceinline__ void dct3_psi_body(const double* __restrict__ in, double* __restrict__ psi, int nx,
-                                              int ny, int m, const double2* __restrict__ tw,
-                                              const double2* __restrict__ t1, const double2* __restrict__ t2,
-                                              int j) {
-  extern __shared__ double2 lds[];
-  double2* A = lds;
-  double2* B = lds + m;
-  double* row = (double*)B;
-  const double* src = in + (size_t)(j - 1) * nx;
-  for (int t = threadIdx.x; t < nx; t += blockDim.x) row[t] = src[t];
-  __syncthreads();
-  const double* y = dct3_row_lds(A, B, nx, m, tw, t1, t2);
-  auto idx = [nx](int a, int b) { return L::cc(a, b, nx); };
-  for (int i = threadIdx.x + 1; i <= nx; i += blockDim.x) {
-    const double v = y[i - 1];
-    store_with_halo(psi, i, j, ny, 1, v, idx);
-    if (i == 1) store_with_halo(psi, 0, j, ny, 1, v, idx);
-    if (i == nx) store_with_halo(psi, nx + 1, j, ny, 1, v, idx);
+    const double ve = (e < M) ? zd(Z, 2 * e) : zd(Z, 2 * (N - 1 - e) + 1);
+    const double vo = (o < M) ? zd(Z, 2 * o) : zd(Z, 2 * (N - 1 - o) + 1);
+    return make_double2(ve, vo);
+  };
+  wave_fft<LOG>(Z, lt, tw, ld0);
+  if (rg >= nrows) return;
+  double* dst = out + (size_t)(row0 + rg) * N;
+#pragma unroll
+  for (int i = 0; i <= M / G; i++) {
+    const int k = lt + i * G;
+    if (k > M) break;
+    const int kk = (k == M) ? 0 : k;
+    const int kr = (k == 0) ? 0 : M - k;
+    const double2 za = Z[fpad(kk)], zb = Z[fpad(kr)];
+    const double ar = za.x, ai = za.y;
+    const double br = zb.x, bi = -zb.y;
+    const double er = (ar + br) * 0.5, ei = (ai + bi) * 0.5;
+    const double dr = ar - br, di = ai - bi;
+    const double2 od = make_double2(di * 0.5, -(dr * 0.5));
+    const double2 tt = wmul(od, t1[k]);
+    const double2 vv = make_double2(er + tt.x, ei + tt.y);
+    const double2 w = wmul(vv, t2[k]);
+    dst[k] = 2. * w.x;
+    if (k != 0 && k != M) dst[N - k] = -2. * w.y;
   }
 }
 
-__global__ void k_dct3_psi(const double* __restrict__ in, double* __restrict__ psi, int nx,
-                           int ny, int m, const double2* __restrict__ tw,
-                           const double2* __restrict__ t1, const double2* __restrict__ t2) {
-  dct3_psi_body(in, psi, nx, ny, m, tw, t1, t2, blockIdx.x + 1);
+// where the DCT-III finds mode row jm: the solved buffer, or (L != 0) the
+// partitioned solve's chunk solutions with zero ends plus the two unit
+// spikes weighted by the bounding separators, x = (g + xl w) + xu z, and
+// the separator itself on a chunk's last row (the expansion step of
+// tdm_partition.hip, done as the rows are loaded)
+struct ModeSrc {
+  const double* buf;
+  const double* X;
+  const double* w;
+  const double* z;
+  int nx, L, P;  // L is a power of two
+  // values c = 2h and 2h + 1 of mode row jm
+  __device__ __forceinline__ double2 pair(int jm, int h) const {
+    const size_t at = (size_t)jm * nx + 2 * h;
+    if (L == 0) return *(const double2*)(buf + at);
+    const int ch = jm >> (31 - __clz(L)), k = jm & (L - 1);
+    const double2 xu = *(const double2*)(X + (size_t)ch * nx + 2 * h);
+    if (k == L - 1) return xu;
+    const double2 xl = *(const double2*)(X + (size_t)(ch == 0 ? P - 1 : ch - 1) * nx + 2 * h);
+    const double2 g = *(const double2*)(buf + at);
+    const double2 wk = *(const double2*)(w + (size_t)k * nx + 2 * h);
+    const double2 zk = *(const double2*)(z + (size_t)k * nx + 2 * h);
+    return make_double2((g.x + xl.x * wk.x) + xu.x * zk.x, (g.y + xl.y * wk.y) + xu.y * zk.y);
+  }
+};
+
+// REDFT01 of mode rows row0 .. row0 + R - 1 (of `rows`); st(row, i, y_i)
+template <int LOG, typename Store>
+__device__ __forceinline__ void dct3_wave(const ModeSrc& src, int rows, int row0, const double2* __restrict__ tw,
+                                          const double2* __restrict__ t1, const double2* __restrict__ t2,
+                                          Store st, double2* lds, int lane) {
+  using F = WaveFft<LOG>;
+  constexpr int M = F::M, N = F::N, G = F::G, U = F::U;
+  const int rg = lane / G, lt = lane % G;
+  const int nrows = min(F::R, rows - row0);
+  if (nrows <= 0) return;  // a second unit past the last row (one wavefront, no barrier)
+  rows_to_lds<LOG>(lds, lane, nrows, [&src, row0](int rr, int h) { return src.pair(row0 + rr, h); });
+  F::sync();
+  double2* Z = lds + rg * F::PM;
+  // pre-twiddle in place: the values of k and M - k (0 <= k <= M/2) need
+  // exactly x[k], x[M-k], x[M+k] and x[N-k], so A[k] takes the slots of
+  // x[k] and x[M-k], and A[M-k] those of x[M+k] and x[N-k]; no lane writes
+  // a slot that another lane reads
+  auto pre = [Z, t1, t2](int k, double xk, double xnk, double xmk, double xkm) {
+    // xk = x[k], xnk = x[N-k] (0 for k = 0), xmk = x[M-k], xkm = x[k+M]
+    const double pr = xk, pim = -xnk;
+    const double2 vk = wmul(make_double2(pr, pim), make_double2(t2[k].x, -t2[k].y));
+    const double qr = xkm, qim = -xmk;
+    const double2 vm = wmul(make_double2(qr, qim), make_double2(t2[k + M].x, -t2[k + M].y));
+    const double sr = vk.x + vm.x, si = vk.y + vm.y;
+    const double2 f = wmul(make_double2(vk.x - vm.x, vk.y - vm.y), make_double2(t1[k].x, -t1[k].y));
+    return make_double2(sr - f.y, -(si + f.x));
+  };
+#pragma unroll
+  for (int i = 0; i <= M / 2 / G; i++) {
+    const int k = lt + i * G;
+    if (2 * k > M) break;
+    if (k == 0) {
+      const double x0 = zd(Z, 0), xm = zd(Z, M);
+      const double2 a = pre(0, x0, 0., xm, xm);
+      zd(Z, 0) = a.x;
+      zd(Z, M) = a.y;
+    } else if (2 * k == M) {
+      const double xa = zd(Z, k), xb = zd(Z, M + k);
+      const double2 a = pre(k, xa, xb, xa, xb);
+      zd(Z, k) = a.x;
+      zd(Z, M + k) = a.y;
+    } else {
+      const double xk = zd(Z, k), xmk = zd(Z, M - k), xkm = zd(Z, M + k), xnk = zd(Z, N - k);
+      const double2 a = pre(k, xk, xnk, xmk, xkm);
+      const double2 b = pre(M - k, xmk, xkm, xk, xnk);
+      zd(Z, k) = a.x;
+      zd(Z, M - k) = a.y;
+      zd(Z, M + k) = b.x;
+      zd(Z, N - k) = b.y;
+    }
+  }
+  F::sync();
+  auto ld0 = [Z](int j) {
+    if (j == 0) return make_double2(zd(Z, 0), zd(Z, M));
+    if (2 * j == M) return make_double2(zd(Z, j), zd(Z, M + j));
+    if (2 * j < M) return make_double2(zd(Z, j), zd(Z, M - j));
+    return make_double2(zd(Z, 2 * M - j), zd(Z, M + j));
+  };
+  wave_fft<LOG>(Z, lt, tw, ld0);
+  // y[2x] (x < M) and y[2(N-1-x)+1] (x >= M) take z[x/2].x for even x and
+  // -z[x/2].y for odd x
+  constexpr int OUT = (F::R * N + U - 1) / U;
+#pragma unroll 8
+  for (int o = 0; o < OUT; o++) {
+    const int t = lane + o * U;
+    const int rr = t / N, i = t % N;
+    if (rr >= nrows) break;
+    const int x = (i & 1) ? N - 1 - (i >> 1) : (i >> 1);
+    const double2 zz = lds[rr * F::PM + fpad(x >> 1)];
+    st(row0 + rr, i, (x & 1) ? -zz.y : zz.x);
+  }
 }
+
+// at most 168 VGPRs: LDS bounds these kernels to about 2 wavefronts per SIMD
+// anyway, and the cap keeps the unrolled loops from hoisting every load
+#define SBFS_ROWS_ATTR(LG) __launch_bounds__(wave_fft_u(LG)) __attribute__((amdgpu_waves_per_eu(3)))
+
+template <int LOG>
+__global__ SBFS_ROWS_ATTR(LOG) void k_dct2_rows(const double* __restrict__ in, double* __restrict__ out, int rows,
+                                                const double2* __restrict__ tw, const double2* __restrict__ t1,
+                                                const double2* __restrict__ t2) {
+  extern __shared__ double2 lds[];
+  dct2_wave<LOG>(in, out, rows, (int)blockIdx.x * WaveFft<LOG>::R, tw, t1, t2, lds, (int)threadIdx.x);
+}
+
+template <int LOG>
+__global__ SBFS_ROWS_ATTR(LOG) void k_dct3_rows(const double* __restrict__ in, double* __restrict__ out, int rows,
+                                                const double2* __restrict__ tw, const double2* __restrict__ t1,
+                                                const double2* __restrict__ t2) {
+  constexpr int N = WaveFft<LOG>::N;
+  extern __shared__ double2 lds[];
+  const ModeSrc src{in, nullptr, nullptr, nullptr, N, 0, 0};
+  dct3_wave<LOG>(src, rows, (int)blockIdx.x * WaveFft<LOG>::R, tw, t1, t2,
+                 [out](int r, int i, double v) { out[(size_t)r * N + i] = v; }, lds, (int)threadIdx.x);
+}
+
+// psi[1](i, j) with its Neumann x ghosts and periodic y halo
+// (compute_potential.c:365-380, src/fluid/boundary/psi.c)
+struct PsiStore {
+  double* psi;
+  int nx, ny;
+  __device__ __forceinline__ void operator()(int jm, int i, double v) const {
+    const int n = nx;
+    auto idx = [n](int a, int b) { return L::cc(a, b, n); };
+    const int j = jm + 1, c = i + 1;
+    store_with_halo(psi, c, j, ny, 1, v, idx);
+    if (c == 1) store_with_halo(psi, 0, j, ny, 1, v, idx);
+    if (c == nx) store_with_halo(psi, nx + 1, j, ny, 1, v, idx);
+  }
+};
+
+template <int LOG>
+__global__ SBFS_ROWS_ATTR(LOG) void k_dct3_psi(ModeSrc src, double* __restrict__ psi, int ny,
+                                               const double2* __restrict__ tw, const double2* __restrict__ t1,
+                                               const double2* __restrict__ t2) {
+  extern __shared__ double2 lds[];
+  dct3_wave<LOG>(src, ny, (int)blockIdx.x * WaveFft<LOG>::R, tw, t1, t2, PsiStore{psi, src.nx, ny}, lds,
+                 (int)threadIdx.x);
+}
+
+#define SBFS_LOG_CASES(X) \
+  X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11)
+
+static int ilog2(int m) { return 31 - __builtin_clz((unsigned)m); }
+
+// rows, threads and LDS bytes of one unit of the row transforms for m
+// complex points
+int wave_rows(int m) { return wave_fft_r(ilog2(m)); }
+int wave_unit(int m) { return wave_fft_u(ilog2(m)); }
+size_t wave_lds(int m) { return wave_fft_lds(ilog2(m)); }
 
 // ---------------------------------------------------------------------------
 // tridiagonal solves, reference order: one Thomas + Sherman-Morrison chain
@@ -368,11 +518,25 @@ __global__ void k_tdm_sequential(double* __restrict__ q, double* __restrict__ v,
   tdm_sequential_body(q, v, q1, nx, ny, evals, l, u, pin_mode0, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-// ensembles: one map entry (case, row or 64-mode group) per workgroup
-__global__ void k_dct2_rows_batch(const RowArgs* __restrict__ tab, const uint2* __restrict__ map) {
+// ensembles: one map entry (case, group of rows or of 64 modes) per
+// workgroup; the row transforms switch on the case's transform length,
+// which is uniform over the wavefront
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) void k_dct2_rows_batch(
+    const RowArgs* __restrict__ tab, const uint2* __restrict__ map) {
+  extern __shared__ double2 lds[];
   const uint2 mp = map[blockIdx.x];
   const RowArgs& a = tab[mp.x];
-  dct2_rows_body(a.rhs, a.buf, a.nx, a.m, a.tw, a.t1, a.t2, (int)mp.y);
+  switch (31 - __clz(a.m)) {
+#define SBFS_DCT2_CASE(LG)                                                                              \
+  case LG: {                                                                                            \
+    using F = WaveFft<LG>;                                                                              \
+    const int units = (int)blockDim.x / F::U, u = (int)threadIdx.x / F::U;                              \
+    dct2_wave<LG>(a.rhs, a.buf, a.ny, ((int)mp.y * units + u) * F::R, a.tw, a.t1, a.t2,                   \
+                  lds + (size_t)u * F::R * F::PM, (int)threadIdx.x % F::U);                              \
+  } break;
+    SBFS_LOG_CASES(SBFS_DCT2_CASE)
+#undef SBFS_DCT2_CASE
+  }
 }
 
 __global__ void k_tdm_sequential_batch(const RowArgs* __restrict__ tab, const uint2* __restrict__ map) {
@@ -381,10 +545,25 @@ __global__ void k_tdm_sequential_batch(const RowArgs* __restrict__ tab, const ui
   tdm_sequential_body(a.buf, a.v, a.q1, a.nx, a.ny, a.evals, a.l, a.l, a.pin, (int)mp.y * 64 + threadIdx.x);
 }
 
-__global__ void k_dct3_psi_batch(const RowArgs* __restrict__ tab, const uint2* __restrict__ map) {
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) void k_dct3_psi_batch(
+    const RowArgs* __restrict__ tab, const uint2* __restrict__ map) {
+  extern __shared__ double2 lds[];
   const uint2 mp = map[blockIdx.x];
   const RowArgs& a = tab[mp.x];
-  dct3_psi_body(a.buf, a.psi, a.nx, a.ny, a.m, a.tw, a.t1, a.t2, (int)mp.y + 1);
+  const int K = a.L - 1;
+  const ModeSrc src{a.buf, a.X, a.L ? a.ptab + (size_t)2 * K * a.nx : nullptr,
+                    a.L ? a.ptab + (size_t)3 * K * a.nx : nullptr, a.nx, a.L, a.P};
+  switch (31 - __clz(a.m)) {
+#define SBFS_DCT3_CASE(LG)                                                                              \
+  case LG: {                                                                                            \
+    using F = WaveFft<LG>;                                                                              \
+    const int units = (int)blockDim.x / F::U, u = (int)threadIdx.x / F::U;                              \
+    dct3_wave<LG>(src, a.ny, ((int)mp.y * units + u) * F::R, a.tw, a.t1, a.t2, PsiStore{a.psi, a.nx, a.ny}, \
+                  lds + (size_t)u * F::R * F::PM, (int)threadIdx.x % F::U);                              \
+  } break;
+    SBFS_LOG_CASES(SBFS_DCT3_CASE)
+#undef SBFS_DCT3_CASE
+  }
 }
 
 
@@ -476,7 +655,6 @@ int poisson_solve(const domain_t* d, DomainCtx* c, double dt_old, double dt_new,
 // tridiagonal solves + DCT-III into psi[1]
 static int poisson_tail(const domain_t* d, PoissonPlan* p, fluid_t* f) {
   const int nx = p->nx, ny = p->ny, m = p->m;
-  const size_t lds = sizeof(double2) * 2 * (size_t)(m > 0 ? m : 1);
   // tridiagonal matrix l = u = 1/hy^2 (compute_potential.c:173-176)
   const double l = 1. / d->hy / d->hy;
   const int pin = options().gauge == SBFS_GAUGE_PINNED ? 1 : 0;
@@ -485,19 +663,30 @@ static int poisson_tail(const domain_t* d, PoissonPlan* p, fluid_t* f) {
   int req = options().chunk;
   while (req < 64 && ny / req > 128) req *= 2;
   const int L = partition_chunk(ny, req);
+  ModeSrc src{p->buf, nullptr, nullptr, nullptr, nx, 0, 0};
   if (options().tdm == SBFS_TDM_PARTITIONED && L != 0 && pin && ny / L <= 128) {
     if (p->L != L && partition_setup(p, d, L)) return 1;
+    // reduce + separators; the expansion happens as k_dct3_psi loads its rows
     if (partition_solve(p, pin)) return 1;
+    const int K = L - 1;
+    src = ModeSrc{p->buf, p->X, p->ptab + (size_t)2 * K * nx, p->ptab + (size_t)3 * K * nx, nx, L, p->P};
   } else {
     SBFS_TIMED("k_tdm_sequential",
                hipLaunchKernelGGL(k_tdm_sequential, dim3(cdiv(nx, 64)), dim3(64), 0, stream(),
                                   p->buf, p->v, p->q1, nx, ny, p->evals, l, l, pin));
     if (check_launch("poisson: tridiagonal")) return 1;
   }
-  SBFS_TIMED("k_dct3_psi",
-             hipLaunchKernelGGL(k_dct3_psi, dim3(ny), dim3(row_threads(m)), lds, stream(),
-                                (const double*)p->buf, (double*)f->psi[1].data, nx, ny, m, p->tw,
-                                p->t1, p->t2));
+  double* psi = (double*)f->psi[1].data;
+  switch (ilog2(m)) {
+#define SBFS_TAIL_CASE(LG)                                                                          \
+  case LG:                                                                                          \
+    SBFS_TIMED("k_dct3_psi", hipLaunchKernelGGL(k_dct3_psi<LG>, dim3(cdiv(ny, WaveFft<LG>::R)), dim3(WaveFft<LG>::U), \
+                                                wave_fft_lds(LG), stream(), src, psi, ny, p->tw, p->t1,    \
+                                                p->t2));                                            \
+    break;
+    SBFS_LOG_CASES(SBFS_TAIL_CASE)
+#undef SBFS_TAIL_CASE
+  }
   return check_launch("poisson: DCT-III + psi");
 }
 
@@ -512,11 +701,17 @@ int poisson_transform_fused(const domain_t* d, DomainCtx* c, fluid_t* f, double*
     *rhs_out = p->rhs;
     return 0;
   }
-  const int m = p->m;
-  const size_t lds = sizeof(double2) * 2 * (size_t)(m > 0 ? m : 1);
-  SBFS_TIMED("k_dct2_rows",
-             hipLaunchKernelGGL(k_dct2_rows, dim3(p->ny), dim3(row_threads(m)), lds, stream(),
-                                (const double*)p->rhs, p->buf, p->nx, m, p->tw, p->t1, p->t2));
+  const int ny = p->ny;
+  switch (ilog2(p->m)) {
+#define SBFS_ROWS_CASE(LG)                                                                              \
+  case LG:                                                                                              \
+    SBFS_TIMED("k_dct2_rows", hipLaunchKernelGGL(k_dct2_rows<LG>, dim3(cdiv(ny, WaveFft<LG>::R)), dim3(WaveFft<LG>::U), \
+                                                 wave_fft_lds(LG), stream(), (const double*)p->rhs, p->buf,  \
+                                                 ny, p->tw, p->t1, p->t2));                             \
+    break;
+    SBFS_LOG_CASES(SBFS_ROWS_CASE)
+#undef SBFS_ROWS_CASE
+  }
   if (check_launch("poisson: DCT-II")) return 1;
   return poisson_tail(d, p, f);
 }
@@ -608,13 +803,20 @@ extern "C" int sbfs_dct_rows(int inverse, size_t n, size_t howmany, const double
   SBFS_HIP(hipMemcpy(dtw, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
   SBFS_HIP(hipMemcpy(dt1, t1.data(), sizeof(double) * t1.size(), hipMemcpyHostToDevice));
   SBFS_HIP(hipMemcpy(dt2, t2.data(), sizeof(double) * t2.size(), hipMemcpyHostToDevice));
-  const size_t lds = sizeof(double2) * 2 * (size_t)m;
-  if (inverse)
-    hipLaunchKernelGGL(k_dct3_rows, dim3((unsigned)howmany), dim3(row_threads(m)), lds, stream(), din, dout,
-                       (int)n, m, dtw, dt1, dt2);
-  else
-    hipLaunchKernelGGL(k_dct2_rows, dim3((unsigned)howmany), dim3(row_threads(m)), lds, stream(), din, dout,
-                       (int)n, m, dtw, dt1, dt2);
+  const int rows = (int)howmany;
+  switch (ilog2(m)) {
+#define SBFS_TEST_CASE(LG)                                                                                 \
+  case LG:                                                                                                 \
+    if (inverse)                                                                                           \
+      hipLaunchKernelGGL(k_dct3_rows<LG>, dim3(cdiv(rows, WaveFft<LG>::R)), dim3(WaveFft<LG>::U), wave_fft_lds(LG), stream(), \
+                         din, dout, rows, dtw, dt1, dt2);                                                  \
+    else                                                                                                   \
+      hipLaunchKernelGGL(k_dct2_rows<LG>, dim3(cdiv(rows, WaveFft<LG>::R)), dim3(WaveFft<LG>::U), wave_fft_lds(LG), stream(), \
+                         din, dout, rows, dtw, dt1, dt2);                                                  \
+    break;
+    SBFS_LOG_CASES(SBFS_TEST_CASE)
+#undef SBFS_TEST_CASE
+  }
   int rc = check_launch("sbfs_dct_rows");
   if (!rc) {
     SBFS_HIP(hipStreamSynchronize(stream()));

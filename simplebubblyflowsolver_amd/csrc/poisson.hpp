@@ -54,11 +54,15 @@ __global__ void k_tdm_sequential_batch(const RowArgs* tab, const uint2* map);
 __global__ void k_dct3_psi_batch(const RowArgs* tab, const uint2* map);
 __global__ void k_part_reduce_batch(const RowArgs* tab, const uint2* map);
 __global__ void k_part_conv_batch(const RowArgs* tab, const uint2* map);
-__global__ void k_part_expand_batch(const RowArgs* tab, const uint2* map);
 constexpr int kConvModesBatch = 16;  // modes per workgroup of k_part_conv_batch
 
 int partition_chunk(int n, int requested);
 int partition_setup(PoissonPlan* p, const domain_t* d, int L);
+// row transforms for m complex points: rows, threads and LDS bytes of one
+// unit (the threads that transform one group of rows together)
+int wave_rows(int m);
+int wave_unit(int m);
+size_t wave_lds(int m);
 int partition_solve(PoissonPlan* p, int pin_mode0);
 
 }  // namespace sbfs

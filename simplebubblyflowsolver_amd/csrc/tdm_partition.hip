@@ -5,14 +5,17 @@
 // Nx latency-bound chains.  Here each column is cut into P chunks of L rows
 // whose last row is a separator:
 //   T1  thread (mode, chunk): Thomas on the L-1 interior rows with zero ends
-//       -> first / last interior value                       (reads r once)
+//       -> first / last interior value; the interior solution overwrites
+//       the chunk's rows                                     (reads r once)
 //   T2  the separators satisfy a periodic tridiagonal system of size P
 //       with constant coefficients (a circulant), solved in parallel as a
 //       circular convolution with the first column of its inverse; the
 //       singular mode 0 pins the last separator, i.e. x_{Ny-1} = 0, the
 //       reference's singular branch (src/tdm.c:94)
-//   T3  thread (mode, chunk): recompute the interior solution and add the
-//       two unit spikes weighted by the bounding separators (writes x)
+//   T3  add the two unit spikes weighted by the bounding separators to the
+//       interior solution: T1 leaves that solution in place of the chunk's
+//       rows, and the DCT-III kernel (poisson.hip) forms x = (g + xl w) +
+//       xu z as it loads each row, so the expansion costs no pass of its own
 // The chunk matrices are identical for all chunks of a mode, so the Thomas
 // factors and the spikes are per-(mode, row-in-chunk) tables built once.
 // Data stay in the [j][m] layout: a wavefront handles 64 consecutive modes,
@@ -83,7 +86,7 @@ __device__ __forceinline__ void chunk_thomas(const double* __restrict__ q, int n
 }
 
 template <int L>
-__global__ void k_part_reduce(const double* __restrict__ q, const double* __restrict__ tab,
+__global__ __launch_bounds__(256) void k_part_reduce(double* __restrict__ q, const double* __restrict__ tab,
                               double* __restrict__ gfl, int nx, int P, double a) {
   constexpr int K = L - 1;
   const int mo = blockIdx.x * blockDim.x + threadIdx.x;
@@ -95,6 +98,8 @@ __global__ void k_part_reduce(const double* __restrict__ q, const double* __rest
   chunk_thomas<K>(q, nx, mo, ch * L, inv, vv, a, g);
   gfl[(size_t)(2 * ch) * nx + mo] = g[0];
   gfl[(size_t)(2 * ch + 1) * nx + mo] = g[K - 1];
+#pragma unroll
+  for (int k = 0; k < K; k++) q[(size_t)(ch * L + k) * nx + mo] = g[k];
 }
 
 // Separators of every mode.  The reduced system is periodic tridiagonal with
@@ -161,27 +166,6 @@ __global__ __launch_bounds__(256) void k_part_conv(const double* __restrict__ q,
     const double x = Y[i * kConvModes + tm];
     X[(size_t)i * nx + mo] = pin ? x - xl : x;
   }
-}
-
-template <int L>
-__global__ void k_part_expand(double* __restrict__ q, const double* __restrict__ tab,
-                              const double* __restrict__ X, int nx, int P, double a) {
-  constexpr int K = L - 1;
-  const int mo = blockIdx.x * blockDim.x + threadIdx.x;
-  const int ch = blockIdx.y * blockDim.y + threadIdx.y;
-  if (mo >= nx || ch >= P) return;
-  const double* inv = tab;
-  const double* vv = tab + (size_t)K * nx;
-  const double* w = tab + (size_t)2 * K * nx;
-  const double* z = tab + (size_t)3 * K * nx;
-  double g[K];
-  chunk_thomas<K>(q, nx, mo, ch * L, inv, vv, a, g);
-  const double xl = X[(size_t)((ch + P - 1) % P) * nx + mo];
-  const double xu = X[(size_t)ch * nx + mo];
-#pragma unroll
-  for (int k = 0; k < K; k++)
-    q[(size_t)(ch * L + k) * nx + mo] = (g[k] + xl * w[(size_t)k * nx + mo]) + xu * z[(size_t)k * nx + mo];
-  q[(size_t)(ch * L + K) * nx + mo] = xu;
 }
 
 // per-mode tables (same arithmetic as the oracle's chunk tables)
@@ -293,9 +277,6 @@ static void part_launch(PoissonPlan* p, double a, int pin) {
              hipLaunchKernelGGL(k_part_conv<L>, dim3(cdiv(nx, kConvModes)), dim3(256),
                                 sizeof(double) * 3 * (size_t)P * kConvModes, stream(), p->buf,
                                 p->gfl, p->htab, p->X, nx, P, a, pin));
-  SBFS_TIMED("k_part_expand",
-             hipLaunchKernelGGL(k_part_expand<L>, grd, blk, 0, stream(), p->buf, p->ptab, p->X,
-                                nx, P, a));
 }
 
 int partition_solve(PoissonPlan* p, int pin) {
@@ -326,25 +307,8 @@ __device__ __forceinline__ void part_reduce_body(const RowArgs& a, int mo, int c
   chunk_thomas<K>(a.buf, a.nx, mo, ch * L, inv, vv, a.pa, g);
   a.gfl[(size_t)(2 * ch) * a.nx + mo] = g[0];
   a.gfl[(size_t)(2 * ch + 1) * a.nx + mo] = g[K - 1];
-}
-
-template <int L>
-__device__ __forceinline__ void part_expand_body(const RowArgs& a, int mo, int ch) {
-  constexpr int K = L - 1;
-  const int nx = a.nx, P = a.P;
-  if (mo >= nx || ch >= P) return;
-  const double* inv = a.ptab;
-  const double* vv = a.ptab + (size_t)K * nx;
-  const double* w = a.ptab + (size_t)2 * K * nx;
-  const double* z = a.ptab + (size_t)3 * K * nx;
-  double g[K];
-  chunk_thomas<K>(a.buf, nx, mo, ch * L, inv, vv, a.pa, g);
-  const double xl = a.X[(size_t)((ch + P - 1) % P) * nx + mo];
-  const double xu = a.X[(size_t)ch * nx + mo];
 #pragma unroll
-  for (int k = 0; k < K; k++)
-    a.buf[(size_t)(ch * L + k) * nx + mo] = (g[k] + xl * w[(size_t)k * nx + mo]) + xu * z[(size_t)k * nx + mo];
-  a.buf[(size_t)(ch * L + K) * nx + mo] = xu;
+  for (int k = 0; k < K; k++) a.buf[(size_t)(ch * L + k) * a.nx + mo] = g[k];
 }
 
 #define SBFS_PART_SWITCH(BODY)                  \
@@ -367,14 +331,6 @@ __global__ __launch_bounds__(256) void k_part_reduce_batch(const RowArgs* __rest
   SBFS_PART_SWITCH(part_reduce_body)
 }
 
-__global__ __launch_bounds__(256) void k_part_expand_batch(const RowArgs* __restrict__ tab,
-                                                           const uint2* __restrict__ map) {
-  const uint2 m = map[blockIdx.x];
-  const RowArgs& a = tab[m.x];
-  const int gx = (a.nx + 63) / 64;
-  const int mo = (int)(m.y % gx) * 64 + threadIdx.x, ch = (int)(m.y / gx) * 4 + threadIdx.y;
-  SBFS_PART_SWITCH(part_expand_body)
-}
 #undef SBFS_PART_SWITCH
 
 // map entry (case, group of kConvModes modes); LDS for the largest P
