@@ -332,11 +332,14 @@ struct ModeSrc {
   }
 };
 
-// REDFT01 of mode rows row0 .. row0 + R - 1 (of `rows`); st(row, i, y_i)
-template <int LOG, typename Store>
+// REDFT01 of mode rows row0 .. row0 + R - 1 (of `rows`) into out: with PSI,
+// psi[1] rows j = row + 1 with the Neumann x ghosts and the periodic y halo
+// (compute_potential.c:365-380, src/fluid/boundary/psi.c; out must be
+// 16-byte aligned); otherwise plain rows of N
+template <int LOG, bool PSI>
 __device__ __forceinline__ void dct3_wave(const ModeSrc& src, int rows, int row0, const double2* __restrict__ tw,
                                           const double2* __restrict__ t1, const double2* __restrict__ t2,
-                                          Store st, double2* lds, int lane) {
+                                          double* __restrict__ out, double2* lds, int lane) {
   using F = WaveFft<LOG>;
   constexpr int M = F::M, N = F::N, G = F::G, U = F::U;
   const int rg = lane / G, lt = lane % G;
@@ -393,15 +396,37 @@ __device__ __forceinline__ void dct3_wave(const ModeSrc& src, int rows, int row0
   wave_fft<LOG>(Z, lt, tw, ld0);
   // y[2x] (x < M) and y[2(N-1-x)+1] (x >= M) take z[x/2].x for even x and
   // -z[x/2].y for odd x
-  constexpr int OUT = (F::R * N + U - 1) / U;
-#pragma unroll 8
-  for (int o = 0; o < OUT; o++) {
-    const int t = lane + o * U;
-    const int rr = t / N, i = t % N;
-    if (rr >= nrows) break;
+  if (rg >= nrows) return;
+  auto y = [Z](int i) {
     const int x = (i & 1) ? N - 1 - (i >> 1) : (i >> 1);
-    const double2 zz = lds[rr * F::PM + fpad(x >> 1)];
-    st(row0 + rr, i, (x & 1) ? -zz.y : zz.x);
+    const double2 zz = Z[fpad(x >> 1)];
+    return (x & 1) ? -zz.y : zz.x;
+  };
+  const int r = row0 + rg;
+  if constexpr (PSI) {
+    // the row with both ghosts is nx + 2 = N + 2 doubles from an even
+    // index: pair q holds columns 2q, 2q + 1, i.e. y[2q - 1], y[2q], with
+    // the ghosts y[-1] = y[0] and y[N] = y[N - 1]
+    const int j = r + 1;
+    double2* row = (double2*)(out + L::cc(0, j, N));
+    double2* halo = j == rows ? (double2*)(out + L::cc(0, 0, N))
+                              : (j == 1 ? (double2*)(out + L::cc(0, rows + 1, N)) : nullptr);
+    constexpr int Q = (N / 2 + 1 + G - 1) / G;
+#pragma unroll
+    for (int u = 0; u < Q; u++) {
+      const int q = lt + u * G;
+      if (q > N / 2) break;
+      const double2 v = make_double2(y(max(2 * q - 1, 0)), y(min(2 * q, N - 1)));
+      row[q] = v;
+      if (halo) halo[q] = v;
+    }
+  } else {
+    double* dst = out + (size_t)r * N;
+#pragma unroll
+    for (int u = 0; u < N / G; u++) {
+      const int i = lt + u * G;
+      dst[i] = y(i);
+    }
   }
 }
 
@@ -424,32 +449,17 @@ __global__ SBFS_ROWS_ATTR(LOG) void k_dct3_rows(const double* __restrict__ in, d
   constexpr int N = WaveFft<LOG>::N;
   extern __shared__ double2 lds[];
   const ModeSrc src{in, nullptr, nullptr, nullptr, N, 0, 0};
-  dct3_wave<LOG>(src, rows, (int)blockIdx.x * WaveFft<LOG>::R, tw, t1, t2,
-                 [out](int r, int i, double v) { out[(size_t)r * N + i] = v; }, lds, (int)threadIdx.x);
+  dct3_wave<LOG, false>(src, rows, (int)blockIdx.x * WaveFft<LOG>::R, tw, t1, t2, out, lds, (int)threadIdx.x);
 }
-
-// psi[1](i, j) with its Neumann x ghosts and periodic y halo
-// (compute_potential.c:365-380, src/fluid/boundary/psi.c)
-struct PsiStore {
-  double* psi;
-  int nx, ny;
-  __device__ __forceinline__ void operator()(int jm, int i, double v) const {
-    const int n = nx;
-    auto idx = [n](int a, int b) { return L::cc(a, b, n); };
-    const int j = jm + 1, c = i + 1;
-    store_with_halo(psi, c, j, ny, 1, v, idx);
-    if (c == 1) store_with_halo(psi, 0, j, ny, 1, v, idx);
-    if (c == nx) store_with_halo(psi, nx + 1, j, ny, 1, v, idx);
-  }
-};
 
 template <int LOG>
 __global__ SBFS_ROWS_ATTR(LOG) void k_dct3_psi(ModeSrc src, double* __restrict__ psi, int ny,
                                                const double2* __restrict__ tw, const double2* __restrict__ t1,
                                                const double2* __restrict__ t2) {
   extern __shared__ double2 lds[];
-  dct3_wave<LOG>(src, ny, (int)blockIdx.x * WaveFft<LOG>::R, tw, t1, t2, PsiStore{psi, src.nx, ny}, lds,
-                 (int)threadIdx.x);
+  // XCD-major rows: an XCD's L2 holds the separators of its own chunks
+  const int unit = xcd_group((int)blockIdx.x, (int)gridDim.x);
+  dct3_wave<LOG, true>(src, ny, unit * WaveFft<LOG>::R, tw, t1, t2, psi, lds, (int)threadIdx.x);
 }
 
 #define SBFS_LOG_CASES(X) \
@@ -558,8 +568,8 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) void k
   case LG: {                                                                                            \
     using F = WaveFft<LG>;                                                                              \
     const int units = (int)blockDim.x / F::U, u = (int)threadIdx.x / F::U;                              \
-    dct3_wave<LG>(src, a.ny, ((int)mp.y * units + u) * F::R, a.tw, a.t1, a.t2, PsiStore{a.psi, a.nx, a.ny}, \
-                  lds + (size_t)u * F::R * F::PM, (int)threadIdx.x % F::U);                              \
+    dct3_wave<LG, true>(src, a.ny, ((int)mp.y * units + u) * F::R, a.tw, a.t1, a.t2, a.psi,              \
+                        lds + (size_t)u * F::R * F::PM, (int)threadIdx.x % F::U);                        \
   } break;
     SBFS_LOG_CASES(SBFS_DCT3_CASE)
 #undef SBFS_DCT3_CASE
@@ -677,6 +687,7 @@ static int poisson_tail(const domain_t* d, PoissonPlan* p, fluid_t* f) {
     if (check_launch("poisson: tridiagonal")) return 1;
   }
   double* psi = (double*)f->psi[1].data;
+  SBFS_REQUIRE(((uintptr_t)psi & 15) == 0, "poisson: psi[1] must be 16-byte aligned (hipMalloc'ed)");
   switch (ilog2(m)) {
 #define SBFS_TAIL_CASE(LG)                                                                          \
   case LG:                                                                                          \
@@ -743,6 +754,7 @@ int poisson_row_args(const domain_t* d, DomainCtx* c, double* psi, RowArgs* a) {
   a->buf = p->buf;
   a->v = p->v;
   a->q1 = p->q1;
+  SBFS_REQUIRE(((uintptr_t)psi & 15) == 0, "poisson: psi[1] must be 16-byte aligned (hipMalloc'ed)");
   a->psi = psi;
   a->evals = p->evals;
   a->tw = p->tw;

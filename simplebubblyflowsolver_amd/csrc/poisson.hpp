@@ -7,6 +7,12 @@
 
 namespace sbfs {
 
+// work item of workgroup b among n: consecutive items on one XCD (the
+// hardware deals workgroups to the 8 XCDs in turn) when n is a multiple of 8
+__device__ __forceinline__ int xcd_group(int b, int n) {
+  return (n & 7) ? b : (b & 7) * (n >> 3) + (b >> 3);
+}
+
 struct PoissonPlan {
   int nx = 0, ny = 0, m = 0;
   double lx = 0.;
@@ -54,7 +60,7 @@ __global__ void k_tdm_sequential_batch(const RowArgs* tab, const uint2* map);
 __global__ void k_dct3_psi_batch(const RowArgs* tab, const uint2* map);
 __global__ void k_part_reduce_batch(const RowArgs* tab, const uint2* map);
 __global__ void k_part_conv_batch(const RowArgs* tab, const uint2* map);
-constexpr int kConvModesBatch = 16;  // modes per workgroup of k_part_conv_batch
+constexpr int kConvModesBatch = 4;  // modes per workgroup of k_part_conv_batch
 
 int partition_chunk(int n, int requested);
 int partition_setup(PoissonPlan* p, const domain_t* d, int L);

@@ -108,11 +108,16 @@ __global__ __launch_bounds__(256) void k_part_reduce(double* __restrict__ q, con
 // of the reduced right-hand side R with h = C^-1 e_0 (one column per mode,
 // built on the host with the reference algorithm).  For the singular mode
 // h = C^+ (e_0 - 1/P), and the result is shifted so that X_{P-1} = 0, the
-// reference's singular branch.  A workgroup holds R and h of 16 modes in LDS;
-// each thread forms kConvS consecutive X_i of one mode, reading each R_k once
-// and sliding a register window over h.  The sum runs over ascending k.
-constexpr int kConvModes = 16;
-constexpr int kConvS = 8;
+// reference's singular branch.  A workgroup holds R and h of kConvModes
+// modes in LDS; each thread forms kConvS consecutive X_i of one mode, reading
+// each R_k once and sliding a register window over h.  The sum runs over
+// ascending k.  Few modes per workgroup give nx / 4 workgroups, enough to
+// fill the chip; consecutive groups go to the same XCD (workgroups are
+// dealt to the 8 XCDs in turn), so the 128-byte lines that 4 groups share
+// are fetched into one L2.
+constexpr int kConvModes = 4;
+constexpr int kConvS = 2;
+
 
 template <int L>
 __global__ __launch_bounds__(256) void k_part_conv(const double* __restrict__ q,
@@ -123,11 +128,11 @@ __global__ __launch_bounds__(256) void k_part_conv(const double* __restrict__ q,
   constexpr int K = L - 1;
   constexpr int G = 256 / kConvModes;
   extern __shared__ double lds[];
-  double* R = lds;                               // [P][16] reduced right-hand side
-  double* H = lds + (size_t)P * kConvModes;      // [P][16] inverse columns
-  double* Y = lds + (size_t)2 * P * kConvModes;  // [P][16] result
+  double* R = lds;                               // [P][kConvModes] reduced right-hand side
+  double* H = lds + (size_t)P * kConvModes;      // [P][kConvModes] inverse columns
+  double* Y = lds + (size_t)2 * P * kConvModes;  // [P][kConvModes] result
   const int tm = threadIdx.x % kConvModes, g = threadIdx.x / kConvModes;
-  const int mo = blockIdx.x * kConvModes + tm;
+  const int mo = xcd_group((int)blockIdx.x, (int)gridDim.x) * kConvModes + tm;
   const bool live = mo < nx;
   const int mm = live ? mo : nx - 1;
   for (int c = g; c < P; c += G) {
@@ -334,6 +339,7 @@ __global__ __launch_bounds__(256) void k_part_reduce_batch(const RowArgs* __rest
 #undef SBFS_PART_SWITCH
 
 // map entry (case, group of kConvModes modes); LDS for the largest P
+static_assert(kConvModes == kConvModesBatch, "the ensemble plans groups of kConvModesBatch modes");
 __global__ __launch_bounds__(256) void k_part_conv_batch(const RowArgs* __restrict__ tab,
                                                          const uint2* __restrict__ map) {
   const uint2 mp = map[blockIdx.x];
