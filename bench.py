@@ -281,6 +281,8 @@ def main() -> int:
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-mixed", dest="mixed", action="store_false",
                     help="skip the mixed-precision comparison run")
+    ap.add_argument("--no-learned", dest="learned", action="store_false",
+                    help="skip the learned-curvature run (training and stepping)")
     ap.add_argument("--no-ensemble", dest="ensemble", action="store_false",
                     help="skip the batched-ensemble run")
     ap.add_argument("--ensemble-steps", type=int, default=200)
@@ -313,7 +315,8 @@ def main() -> int:
     mixed = None
     if a.mixed and a.precision == "fp64" and a.path == "fused":
         mixed = mixed_run(be, w, a.warmup, a.steps, t, d0, d1)
-        mixed["learned_curvature"] = learned_run(be, w, a.warmup, a.steps, t, d0, d1)
+        if a.learned:
+            mixed["learned_curvature"] = learned_run(be, w, a.warmup, a.steps, t, d0, d1)
     # per-kernel times: a separate pass with HIP events around every launch
     # (the events keep the step out of graph replay, so it is not the timed loop)
     prof = []

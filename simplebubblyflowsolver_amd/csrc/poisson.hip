@@ -67,6 +67,14 @@ int partition_chunk(int n, int requested) {
   return L >= 2 ? L : 0;
 }
 
+// the chunk of an ny-row solve: the requested length, doubled while the
+// reduced system would exceed kMaxSeparators
+int poisson_chunk(int ny) {
+  int req = options().chunk;
+  while (req < 64 && ny / req > kMaxSeparators) req *= 2;
+  return partition_chunk(ny, req);
+}
+
 // ---------------------------------------------------------------------------
 // FFT in LDS: forward, Stockham autosort, radix-4 stages then one radix-2
 // ---------------------------------------------------------------------------
@@ -668,13 +676,9 @@ static int poisson_tail(const domain_t* d, PoissonPlan* p, fluid_t* f) {
   // tridiagonal matrix l = u = 1/hy^2 (compute_potential.c:173-176)
   const double l = 1. / d->hy / d->hy;
   const int pin = options().gauge == SBFS_GAUGE_PINNED ? 1 : 0;
-  // chunk: the requested length, doubled while the reduced system would
-  // exceed the separator kernel's LDS column (128 separators)
-  int req = options().chunk;
-  while (req < 64 && ny / req > 128) req *= 2;
-  const int L = partition_chunk(ny, req);
+  const int L = poisson_chunk(ny);
   ModeSrc src{p->buf, nullptr, nullptr, nullptr, nx, 0, 0};
-  if (options().tdm == SBFS_TDM_PARTITIONED && L != 0 && pin && ny / L <= 128) {
+  if (options().tdm == SBFS_TDM_PARTITIONED && L != 0 && pin && ny / L <= kMaxSeparators) {
     if (p->L != L && partition_setup(p, d, L)) return 1;
     // reduce + separators; the expansion happens as k_dct3_psi loads its rows
     if (partition_solve(p, pin)) return 1;
@@ -766,15 +770,13 @@ int poisson_row_args(const domain_t* d, DomainCtx* c, double* psi, RowArgs* a) {
   a->m = p->m;
   a->pin = options().gauge == SBFS_GAUGE_PINNED ? 1 : 0;
   // the tridiagonal algorithm poisson_tail would take for this case
-  int req = options().chunk;
-  while (req < 64 && p->ny / req > 128) req *= 2;
-  const int L = partition_chunk(p->ny, req);
+  const int L = poisson_chunk(p->ny);
   a->L = 0;
   a->P = 0;
   a->ptab = a->htab = nullptr;
   a->gfl = a->X = nullptr;
   a->pa = 0.;
-  if (options().tdm == SBFS_TDM_PARTITIONED && L != 0 && a->pin && p->ny / L <= 128) {
+  if (options().tdm == SBFS_TDM_PARTITIONED && L != 0 && a->pin && p->ny / L <= kMaxSeparators) {
     if (p->L != L && partition_setup(p, d, L)) return 1;
     a->L = L;
     a->P = p->P;

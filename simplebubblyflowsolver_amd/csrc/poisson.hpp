@@ -63,6 +63,12 @@ __global__ void k_part_conv_batch(const RowArgs* tab, const uint2* map);
 constexpr int kConvModesBatch = 4;  // modes per workgroup of k_part_conv_batch
 
 int partition_chunk(int n, int requested);
+// largest reduced system of the partitioned solve: the convolution's
+// rounding grows like P^2 for the lowest modes (at 8192 rows, relative
+// error 6.7e-13 with P = 128 against 4.9e-12 with P = 256 and 7.0e-13 for
+// the reference-order sequential solve), so long columns take longer chunks
+constexpr int kMaxSeparators = 128;
+int poisson_chunk(int ny);
 int partition_setup(PoissonPlan* p, const domain_t* d, int L);
 // row transforms for m complex points: rows, threads and LDS bytes of one
 // unit (the threads that transform one group of rows together)

@@ -67,39 +67,60 @@ __device__ void periodic_tdm(double* __restrict__ q, double* __restrict__ v,
   for (int r = 0; r < n - 1; r++) q[r * st] = q[r * st] + xl * q1[r * st];
 }
 
-// Thomas on one chunk interior with the per-mode tables:
-// forward g_k = inv_k (f_k - a g_{k-1}); backward g_k -= v_k g_{k+1}
+// T1 for one (mode, chunk): Thomas on the chunk interior with the per-mode
+// tables, forward g_k = inv_k (f_k - a g_{k-1}), backward g_k -= v_k g_{k+1};
+// the interior solution replaces the chunk's rows, and its first and last
+// values are returned.  Long chunks keep only their last KR forward values in
+// registers: the first K - KR go back to their rows during the forward sweep
+// and are read again by the backward sweep (same operations, half the
+// registers: K = 63 in registers would allow one wavefront per SIMD)
 template <int K>
-__device__ __forceinline__ void chunk_thomas(const double* __restrict__ q, int nx, int mo,
-                                             int row0, const double* __restrict__ inv,
-                                             const double* __restrict__ vv, double a,
-                                             double (&g)[K]) {
+__device__ __forceinline__ void chunk_reduce(double* __restrict__ q, int nx, int mo, int row0,
+                                             const double* __restrict__ inv, const double* __restrict__ vv,
+                                             double a, double& first, double& last) {
+  constexpr int KR = K > 40 ? K / 2 + 1 : K;
+  constexpr int KM = K - KR;
+  double g[KR];
   double prev = 0.;
-#pragma unroll
-  for (int k = 0; k < K; k++) {
+#pragma unroll 8
+  for (int k = 0; k < KM; k++) {
     const double f = q[(size_t)(row0 + k) * nx + mo];
-    g[k] = inv[(size_t)k * nx + mo] * (f - a * prev);
+    const double gk = inv[(size_t)k * nx + mo] * (f - a * prev);
+    q[(size_t)(row0 + k) * nx + mo] = gk;
+    prev = gk;
+  }
+#pragma unroll
+  for (int k = 0; k < KR; k++) {
+    const double f = q[(size_t)(row0 + KM + k) * nx + mo];
+    g[k] = inv[(size_t)(KM + k) * nx + mo] * (f - a * prev);
     prev = g[k];
   }
 #pragma unroll
-  for (int k = K - 2; k >= 0; k--) g[k] = g[k] - vv[(size_t)k * nx + mo] * g[k + 1];
+  for (int k = KR - 2; k >= 0; k--) g[k] = g[k] - vv[(size_t)(KM + k) * nx + mo] * g[k + 1];
+#pragma unroll
+  for (int k = 0; k < KR; k++) q[(size_t)(row0 + KM + k) * nx + mo] = g[k];
+  double next = g[0];
+#pragma unroll 8
+  for (int k = KM - 1; k >= 0; k--) {
+    const double gk = q[(size_t)(row0 + k) * nx + mo] - vv[(size_t)k * nx + mo] * next;
+    q[(size_t)(row0 + k) * nx + mo] = gk;
+    next = gk;
+  }
+  first = next;
+  last = g[KR - 1];
 }
 
 template <int L>
-__global__ __launch_bounds__(256) void k_part_reduce(double* __restrict__ q, const double* __restrict__ tab,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_part_reduce(double* __restrict__ q, const double* __restrict__ tab,
                               double* __restrict__ gfl, int nx, int P, double a) {
   constexpr int K = L - 1;
   const int mo = blockIdx.x * blockDim.x + threadIdx.x;
   const int ch = blockIdx.y * blockDim.y + threadIdx.y;
   if (mo >= nx || ch >= P) return;
-  const double* inv = tab;
-  const double* vv = tab + (size_t)K * nx;
-  double g[K];
-  chunk_thomas<K>(q, nx, mo, ch * L, inv, vv, a, g);
-  gfl[(size_t)(2 * ch) * nx + mo] = g[0];
-  gfl[(size_t)(2 * ch + 1) * nx + mo] = g[K - 1];
-#pragma unroll
-  for (int k = 0; k < K; k++) q[(size_t)(ch * L + k) * nx + mo] = g[k];
+  double first, last;
+  chunk_reduce<K>(q, nx, mo, ch * L, tab, tab + (size_t)K * nx, a, first, last);
+  gfl[(size_t)(2 * ch) * nx + mo] = first;
+  gfl[(size_t)(2 * ch + 1) * nx + mo] = last;
 }
 
 // Separators of every mode.  The reduced system is periodic tridiagonal with
@@ -306,14 +327,10 @@ template <int L>
 __device__ __forceinline__ void part_reduce_body(const RowArgs& a, int mo, int ch) {
   constexpr int K = L - 1;
   if (mo >= a.nx || ch >= a.P) return;
-  const double* inv = a.ptab;
-  const double* vv = a.ptab + (size_t)K * a.nx;
-  double g[K];
-  chunk_thomas<K>(a.buf, a.nx, mo, ch * L, inv, vv, a.pa, g);
-  a.gfl[(size_t)(2 * ch) * a.nx + mo] = g[0];
-  a.gfl[(size_t)(2 * ch + 1) * a.nx + mo] = g[K - 1];
-#pragma unroll
-  for (int k = 0; k < K; k++) a.buf[(size_t)(ch * L + k) * a.nx + mo] = g[k];
+  double first, last;
+  chunk_reduce<K>(a.buf, a.nx, mo, ch * L, a.ptab, a.ptab + (size_t)K * a.nx, a.pa, first, last);
+  a.gfl[(size_t)(2 * ch) * a.nx + mo] = first;
+  a.gfl[(size_t)(2 * ch + 1) * a.nx + mo] = last;
 }
 
 #define SBFS_PART_SWITCH(BODY)                  \
@@ -327,7 +344,7 @@ __device__ __forceinline__ void part_reduce_body(const RowArgs& a, int mo, int c
   }
 
 // map entry (case, tile of 64 modes x 4 chunks)
-__global__ __launch_bounds__(256) void k_part_reduce_batch(const RowArgs* __restrict__ tab,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_part_reduce_batch(const RowArgs* __restrict__ tab,
                                                            const uint2* __restrict__ map) {
   const uint2 m = map[blockIdx.x];
   const RowArgs& a = tab[m.x];

@@ -62,11 +62,13 @@ def test_stages_isolated(oracle_mod, product, nx, ny, seed):
         set_tdm(product, 1)
 
 
-@pytest.mark.parametrize("nx,ny,chunk", [(32, 128, 16), (16, 48, 16), (64, 96, 8), (8, 2048, 16)])
+@pytest.mark.parametrize("nx,ny,chunk", [(32, 128, 16), (16, 48, 16), (64, 96, 8), (8, 2048, 16),
+                                         (8, 4096, 32), (8, 8192, 64)])
 def test_partitioned_tdm_matches_oracle(oracle_mod, product, nx, ny, chunk):
     """GPU partitioned solver vs the oracle's restatement of the same
     algorithm: bitwise; and vs the reference-order solver: 1e-11 relative
-    (reduced systems of 8, 3, 12 and 128 separators)."""
+    (reduced systems of 8, 3, 12, 128, 128 and 128 separators; the last one
+    has the 4096 x 8192 swarm's 8192-row columns and 64-row chunks)."""
     set_tdm(product, 1, chunk)
     oracle_mod.set_poisson(dct=oracle_mod.DCT_FFT, tdm=oracle_mod.TDM_PARTITIONED, chunk=chunk)
     o, g = new_pair(oracle_mod, product, nx, ny)
@@ -85,7 +87,9 @@ def test_partitioned_tdm_matches_oracle(oracle_mod, product, nx, ny, chunk):
         copy_state(g, o)
         o.compute_potential(dt_old, dt_new)
         c = o.get("psi1")
-        assert np.abs(c - b).max() <= 1e-11 * np.abs(c).max()
+        # both orders round; the lowest modes' conditioning grows like ny^2
+        # (tests/test_oracle_pins.py pins each against extended precision)
+        assert np.abs(c - b).max() <= 1e-11 * max(1.0, (ny / 2048) ** 2) * np.abs(c).max()
     finally:
         o.close()
         g.close()
