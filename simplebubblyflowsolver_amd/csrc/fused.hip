@@ -676,6 +676,7 @@ static void fill_march(const domain_t* domain, const DomainCtx* c, const fluid_t
   m.pg_old = rk((k + 1) % 3).g;  // gammas of stage k-1's dt_old / dt_new
   m.pg_new = rk((k + 2) % 3).g;
   m.kappa = nullptr;
+  m.kappa_n = nullptr;
   m.srcb_vof = D(interface->src[1]);
   m.srcb_ux = D(fluid->srcux[1]);
   m.srcb_uy = D(fluid->srcuy[1]);
@@ -813,7 +814,7 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
       MarchArgs m;
       fill_march(domain, c, fluid, interface, cf, rhs, band, k, corr, us, vs, g_scr.vof, g_scr.p, dtp, m);
       if (learned_h > 0.) {
-        if (curv_mlp_launch(a.vof, g_scr.kappa, nx, ny, learned_h)) return 1;
+        if (curv_mlp_launch(a.vof, g_scr.kappa, nx, ny, learned_h, &m.kappa_n)) return 1;
         m.kappa = g_scr.kappa;
       }
       const dim3 grid(cdiv(nx, kStrip), cdiv(ny, band)), block(64);
@@ -914,6 +915,7 @@ static StepKey step_key(const DomainCtx* c, fluid_t* f, interface_t* s, const Co
                 ((uint64_t)options().chunk << 16) | ((uint64_t)options().precision << 48) |
                 ((uint64_t)options().curvature << 56));
   key_add(k, g_scr.kappa);
+  k.v.push_back(options().curvature == SBFS_CURVATURE_LEARNED ? curv_buffer_epoch() : 0u);
   return k;
 }
 

@@ -302,7 +302,9 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
   double np = ld_cc(a.p, s_first), nps = ld_cc(a.psi0, CORR ? s_first + 1 : s_first);
   double npo = CORR ? ld_cc(a.psi_old, s_first + 1) : 0.;
   // learned curvature of row s, prefetched a row ahead (0 where absent)
-  auto ld_kap = [&](int j) { return (a.kappa && c_in) ? lane_at(ci_row(a.kappa, wrap_r(j, ny)), loff >> 1) : 0.f; };
+  // (no listed segment: every value is NaN, so none is read)
+  const float* kap = (a.kappa && *a.kappa_n != 0u) ? a.kappa : nullptr;
+  auto ld_kap = [&](int j) { return (kap && c_in) ? lane_at(ci_row(kap, wrap_r(j, ny)), loff >> 1) : 0.f; };
   float nkap = ld_kap(s_first);
   for (int s = s_first; s <= je + 1; s++) {
     // -- new input rows -----------------------------------------------------

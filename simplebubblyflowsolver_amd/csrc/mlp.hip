@@ -545,6 +545,7 @@ __global__ __launch_bounds__(64) void k_curv_mlp(Weights w, const double* __rest
   __shared__ double tile[5][68];  // rows j-2..j+2, columns i0-2..i0+65
   const int l = threadIdx.x, lg = l >> 4;
   const unsigned n = *count;
+  if (blockIdx.x >= n) return;  // no segment for this workgroup (all of them when none is listed)
   const size_t wc = (size_t)nx + 2;
   AOps a;
   load_aops(w, a);
@@ -705,7 +706,10 @@ static int grad_pass(const Samples& s, long first, long stride, int tiles, float
 // the learned curvature of a stage's input vof: the gate, then the network
 // on the listed segments (list and counter kept per process, grown with the
 // grid)
-int curv_mlp_launch(const double* vof, float* kappa, int nx, int ny, double h) {
+static unsigned g_curv_epoch = 0;  // bumped whenever the segment buffer moves
+unsigned curv_buffer_epoch() { return g_curv_epoch; }
+
+int curv_mlp_launch(const double* vof, float* kappa, int nx, int ny, double h, const unsigned** count) {
   using namespace mlp;
   SBFS_REQUIRE(g_net.trained, "learned curvature: no trained network (sbfs_mlp_train or sbfs_mlp_set_params)");
   static unsigned* buf = nullptr;  // [0] counter, then the segment list
@@ -718,6 +722,7 @@ int curv_mlp_launch(const double* vof, float* kappa, int nx, int ny, double h) {
     buf = nullptr;
     SBFS_HIP(hipMalloc(&buf, sizeof(unsigned) * (segs + 1)));
     cap = segs + 1;
+    g_curv_epoch++;  // captured steps hold the old address: their keys no longer match
   }
   hipLaunchKernelGGL(k_curv_reset, dim3(1), dim3(64), 0, stream(), buf);
   SBFS_TIMED("k_curv_gate", hipLaunchKernelGGL(k_curv_gate, dim3(segs_x, cdiv(ny, 4 * kGateRows)), dim3(256), 0,
@@ -725,6 +730,7 @@ int curv_mlp_launch(const double* vof, float* kappa, int nx, int ny, double h) {
   SBFS_TIMED("k_curv_mlp", hipLaunchKernelGGL(k_curv_mlp, dim3(2048), dim3(64), 0, stream(), weights(), vof, kappa,
                                               nx, ny, (float)(1. / h), (const unsigned*)buf,
                                               (const unsigned*)(buf + 1), segs_x));
+  *count = buf;
   return check_launch("k_curv_mlp");
 }
 
@@ -934,7 +940,8 @@ extern "C" int sbfs_mlp_coverage(const domain_t* domain, const interface_t* inte
   SBFS_HIP(hipMalloc(&k, sizeof(float) * n));
   SBFS_HIP(hipMalloc(&cnt, sizeof(unsigned long long)));
   SBFS_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), stream()));
-  int rc = curv_mlp_launch((const double*)interface->vof.data, k, nx, ny, domain->hy);
+  const unsigned* listed = nullptr;
+  int rc = curv_mlp_launch((const double*)interface->vof.data, k, nx, ny, domain->hy, &listed);
   unsigned long long h = 0;
   if (!rc) {
     hipLaunchKernelGGL(k_count_learned, dim3(1024), dim3(256), 0, stream(), (const float*)k, n, cnt);

@@ -150,8 +150,12 @@ int npy_write(const char* dir, const char* name, const char* descr,
 int load_dt_coefs();
 
 // learned curvature (mlp.hip): kappa of the interior cells of `vof`, NaN
-// where the plain curvature stays; h is the (uniform, square) cell size
-int curv_mlp_launch(const double* vof, float* kappa, int nx, int ny, double h);
+// where the plain curvature stays; h is the (uniform, square) cell size;
+// *count receives the device address of the number of listed row segments
+int curv_mlp_launch(const double* vof, float* kappa, int nx, int ny, double h, const unsigned** count);
+// changes whenever curv_mlp_launch moves its segment buffer (part of the key
+// of a captured step)
+unsigned curv_buffer_epoch();
 
 // kernel timing (profiler.hip): events around launches while profiling
 bool prof_active();  // kernel timing on (sbfs_profile_start)

@@ -120,3 +120,32 @@ def test_wide_profile_keeps_plain_curvature(product, trained):
     b = drop(product, net, True, vof, 200)
     print(f"wide-profile static drop, max |u| after 200 steps: {np.abs(a[0]).max():.3e}", flush=True)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_learned_steps_after_a_larger_case(product, trained):
+    """A larger case grows the inference's segment buffer; steps of a
+    smaller case captured before it must not replay with the old buffer
+    (the step graphs are keyed on the buffer), so the small case repeats
+    bitwise."""
+    net, _ = trained
+
+    def run(n, steps):
+        w = configs.static_drop(n, steps, radius_cells=n / 8)
+        assert product.lib.sbfs_set_precision(1) == 0
+        net.use(True)
+        g = solver.Case(product, w.nx, w.ny, w.lx, w.ly, w.Re, w.Fr, w.denr, w.visr, w.We)
+        try:
+            g.load_initial_condition(sharp_drop_vof(n, 1 / 8))
+            for _ in range(steps):
+                g.integrate()
+            return g.get("ux"), g.get("uy"), g.get("vof")
+        finally:
+            g.close()
+            net.use(False)
+            product.lib.sbfs_set_precision(0)
+
+    first = run(64, 12)
+    run(256, 4)
+    again = run(64, 12)
+    for a, b in zip(first, again):
+        assert np.array_equal(a, b)
