@@ -500,6 +500,7 @@ static FusedScratch g_scr;
 static int scratch(const domain_t* d, const interface_t* s, const fluid_t* f) {
   if (g_scr.nx == d->mysizes[0] && g_scr.ny == d->mysizes[1]) return 0;
   (void)hipStreamSynchronize(stream());
+  graph_epoch_bump();
   for (double* q : {g_scr.vof, g_scr.ux, g_scr.uy, g_scr.ux2, g_scr.uy2, g_scr.p}) (void)hipFree(q);
   (void)hipFree(g_scr.kappa);
   g_scr = FusedScratch();
@@ -583,6 +584,7 @@ static int coef_tables(const domain_t* d, const DomainCtx* c, double Re, Coef* o
     return 0;
   }
   (void)hipStreamSynchronize(stream());
+  graph_epoch_bump();
   (void)hipFree(g_coef.buf);
   g_coef.buf = nullptr;
   if (coef_build(d, Re, &g_coef.buf, &g_coef.cf)) return 1;
@@ -915,7 +917,7 @@ static StepKey step_key(const DomainCtx* c, fluid_t* f, interface_t* s, const Co
                 ((uint64_t)options().chunk << 16) | ((uint64_t)options().precision << 48) |
                 ((uint64_t)options().curvature << 56));
   key_add(k, g_scr.kappa);
-  k.v.push_back(options().curvature == SBFS_CURVATURE_LEARNED ? curv_buffer_epoch() : 0u);
+  k.v.push_back(graph_epoch());
   return k;
 }
 

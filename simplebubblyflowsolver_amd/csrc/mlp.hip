@@ -706,9 +706,6 @@ static int grad_pass(const Samples& s, long first, long stride, int tiles, float
 // the learned curvature of a stage's input vof: the gate, then the network
 // on the listed segments (list and counter kept per process, grown with the
 // grid)
-static unsigned g_curv_epoch = 0;  // bumped whenever the segment buffer moves
-unsigned curv_buffer_epoch() { return g_curv_epoch; }
-
 int curv_mlp_launch(const double* vof, float* kappa, int nx, int ny, double h, const unsigned** count) {
   using namespace mlp;
   SBFS_REQUIRE(g_net.trained, "learned curvature: no trained network (sbfs_mlp_train or sbfs_mlp_set_params)");
@@ -722,7 +719,7 @@ int curv_mlp_launch(const double* vof, float* kappa, int nx, int ny, double h, c
     buf = nullptr;
     SBFS_HIP(hipMalloc(&buf, sizeof(unsigned) * (segs + 1)));
     cap = segs + 1;
-    g_curv_epoch++;  // captured steps hold the old address: their keys no longer match
+    graph_epoch_bump();  // captured steps hold the old address
   }
   hipLaunchKernelGGL(k_curv_reset, dim3(1), dim3(64), 0, stream(), buf);
   SBFS_TIMED("k_curv_gate", hipLaunchKernelGGL(k_curv_gate, dim3(segs_x, cdiv(ny, 4 * kGateRows)), dim3(256), 0,

@@ -37,6 +37,7 @@ static const double kPiPoisson = 3.14159265358979324;  // compute_potential.c:21
 
 void poisson_destroy(PoissonPlan* p) {
   if (!p) return;
+  graph_epoch_bump();
   (void)hipFree(p->buf);
   (void)hipFree(p->v);
   (void)hipFree(p->q1);

@@ -152,9 +152,14 @@ static uint64_t domain_hash(const domain_t* d) {
 
 void poisson_destroy(PoissonPlan* p);  // poisson.hip
 
+static uint64_t g_graph_epoch = 0;
+void graph_epoch_bump() { g_graph_epoch++; }
+uint64_t graph_epoch() { return g_graph_epoch; }
+
 static void ctx_free(DomainCtx* c) {
   if (!c) return;
   (void)hipStreamSynchronize(g_stream);
+  graph_epoch_bump();
   (void)hipFree(c->hxxf);
   (void)hipFree(c->hxxc);
   (void)hipFree(c->jdxf);

@@ -153,9 +153,13 @@ int load_dt_coefs();
 // where the plain curvature stays; h is the (uniform, square) cell size;
 // *count receives the device address of the number of listed row segments
 int curv_mlp_launch(const double* vof, float* kappa, int nx, int ny, double h, const unsigned** count);
-// changes whenever curv_mlp_launch moves its segment buffer (part of the key
-// of a captured step)
-unsigned curv_buffer_epoch();
+// Captured step graphs hold device addresses that their keys do not list
+// (Poisson plan buffers, domain metrics, the inference's segment list).
+// Whatever frees such memory bumps this epoch, which is part of every key,
+// so no graph replays against freed memory, even when a new allocation
+// reuses the old addresses.
+void graph_epoch_bump();
+uint64_t graph_epoch();
 
 // kernel timing (profiler.hip): events around launches while profiling
 bool prof_active();  // kernel timing on (sbfs_profile_start)

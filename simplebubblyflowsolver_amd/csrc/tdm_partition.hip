@@ -236,6 +236,7 @@ static void host_periodic_tdm(int n, double l, double c, double u, bool pin, dou
 
 int partition_setup(PoissonPlan* p, const domain_t* d, int L) {
   const int nx = p->nx, ny = p->ny, K = L - 1, P = ny / L;
+  graph_epoch_bump();
   (void)hipFree(p->ptab);
   (void)hipFree(p->gfl);
   (void)hipFree(p->X);
