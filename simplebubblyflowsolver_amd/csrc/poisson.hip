@@ -281,6 +281,7 @@ __device__ __forceinline__ void dct2_wave(const double* __restrict__ in, double*
   const int rg = lane / G, lt = lane % G;
   const int nrows = min(F::R, rows - row0);
   if (nrows <= 0) return;  // a second unit past the last row (one wavefront, no barrier)
+  const auto tw0 = pass_tw<LOG, 0>(lt, tw);
   const double2* src = (const double2*)(in + (size_t)row0 * N);
   rows_to_lds<LOG>(lds, lane, nrows, [src](int rr, int h) { return src[rr * (N / 2) + h]; });
   F::sync();
@@ -292,7 +293,7 @@ __device__ __forceinline__ void dct2_wave(const double* __restrict__ in, double*
     const double vo = (o < M) ? zd(Z, 2 * o) : zd(Z, 2 * (N - 1 - o) + 1);
     return make_double2(ve, vo);
   };
-  wave_fft<LOG>(Z, lt, tw, ld0);
+  wave_fft<LOG>(Z, lt, tw, ld0, tw0);
   if (rg >= nrows) return;
   double* dst = out + (size_t)(row0 + rg) * N;
 #pragma unroll
@@ -354,6 +355,7 @@ __device__ __forceinline__ void dct3_wave(const ModeSrc& src, int rows, int row0
   const int rg = lane / G, lt = lane % G;
   const int nrows = min(F::R, rows - row0);
   if (nrows <= 0) return;  // a second unit past the last row (one wavefront, no barrier)
+  const auto tw0 = pass_tw<LOG, 0>(lt, tw);
   rows_to_lds<LOG>(lds, lane, nrows, [&src, row0](int rr, int h) { return src.pair(row0 + rr, h); });
   F::sync();
   double2* Z = lds + rg * F::PM;
@@ -402,7 +404,7 @@ __device__ __forceinline__ void dct3_wave(const ModeSrc& src, int rows, int row0
     if (2 * j < M) return make_double2(zd(Z, j), zd(Z, M - j));
     return make_double2(zd(Z, 2 * M - j), zd(Z, M + j));
   };
-  wave_fft<LOG>(Z, lt, tw, ld0);
+  wave_fft<LOG>(Z, lt, tw, ld0, tw0);
   // y[2x] (x < M) and y[2(N-1-x)+1] (x >= M) take z[x/2].x for even x and
   // -z[x/2].y for odd x
   if (rg >= nrows) return;
@@ -439,9 +441,11 @@ __device__ __forceinline__ void dct3_wave(const ModeSrc& src, int rows, int row0
   }
 }
 
-// at most 168 VGPRs: LDS bounds these kernels to about 2 wavefronts per SIMD
-// anyway, and the cap keeps the unrolled loops from hoisting every load
-#define SBFS_ROWS_ATTR(LG) __launch_bounds__(wave_fft_u(LG)) __attribute__((amdgpu_waves_per_eu(3)))
+// two wavefronts' worth of VGPRs (256): the twiddles of the next pass are
+// in flight beside the current one, and LDS bounds these kernels to about 2
+// wavefronts per SIMD anyway; the cap keeps the unrolled loops from
+// hoisting every load
+#define SBFS_ROWS_ATTR(LG) __launch_bounds__(wave_fft_u(LG)) __attribute__((amdgpu_waves_per_eu(2)))
 
 template <int LOG>
 __global__ SBFS_ROWS_ATTR(LOG) void k_dct2_rows(const double* __restrict__ in, double* __restrict__ out, int rows,
@@ -540,7 +544,7 @@ __global__ void k_tdm_sequential(double* __restrict__ q, double* __restrict__ v,
 // ensembles: one map entry (case, group of rows or of 64 modes) per
 // workgroup; the row transforms switch on the case's transform length,
 // which is uniform over the wavefront
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) void k_dct2_rows_batch(
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void k_dct2_rows_batch(
     const RowArgs* __restrict__ tab, const uint2* __restrict__ map) {
   extern __shared__ double2 lds[];
   const uint2 mp = map[blockIdx.x];
@@ -564,7 +568,7 @@ __global__ void k_tdm_sequential_batch(const RowArgs* __restrict__ tab, const ui
   tdm_sequential_body(a.buf, a.v, a.q1, a.nx, a.ny, a.evals, a.l, a.l, a.pin, (int)mp.y * 64 + threadIdx.x);
 }
 
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) void k_dct3_psi_batch(
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void k_dct3_psi_batch(
     const RowArgs* __restrict__ tab, const uint2* __restrict__ map) {
   extern __shared__ double2 lds[];
   const uint2 mp = map[blockIdx.x];

@@ -81,19 +81,66 @@ struct ZLoad {
   __device__ __forceinline__ double2 operator()(int i) const { return z[fpad(i)]; }
 };
 
-// stages T and T+1 (s = 4^T): group gi = (p2, q) runs the stage-T
-// butterflies p = p2 + k2 Q (k2 = 0..3) at q, whose outputs k are the inputs
-// of the stage-(T+1) butterfly (p2, q + s k)
+// The twiddles of a pass, in the order the pass uses them.  They depend on
+// the lane only, so each pass's set is loaded one pass ahead (the first
+// before the row arrives): the loads are in flight while the previous pass
+// runs, since a wavefront-level fence does not wait for them.
+struct NoTw {};
+template <int N>
+struct TwSet {
+  double2 w[N];
+};
+
+template <int LOG, int T>
+__device__ __forceinline__ auto pass_tw(int lt, const double2* __restrict__ tw) {
+  using F = WaveFft<LOG>;
+  constexpr int M = F::M, NR4 = F::NR4;
+  if constexpr (T + 1 < NR4) {  // stages T, T+1 (one group of 4 + 4 butterflies per lane)
+    static_assert((M / 16) / F::G == 1, "one group per lane");
+    constexpr int s = 1 << (2 * T), Q = M / (16 * s);
+    TwSet<15> t;
+    const int p2 = lt >> (2 * T);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; k2++) {
+      const int p = p2 + k2 * Q;
+      t.w[3 * k2] = tw[p * s];
+      t.w[3 * k2 + 1] = tw[2 * p * s];
+      t.w[3 * k2 + 2] = tw[3 * p * s];
+    }
+    t.w[12] = tw[p2 * 4 * s];
+    t.w[13] = tw[2 * p2 * 4 * s];
+    t.w[14] = tw[3 * p2 * 4 * s];
+    return t;
+  } else if constexpr (T < NR4) {  // the last radix-4 stage alone
+    constexpr int s = 1 << (2 * T), PER = (M / 4) / F::G;
+    TwSet<3 * PER> t;
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int p = (lt + i * F::G) >> (2 * T);
+      t.w[3 * i] = tw[p * s];
+      t.w[3 * i + 1] = tw[2 * p * s];
+      t.w[3 * i + 2] = tw[3 * p * s];
+    }
+    return t;
+  } else if constexpr (LOG & 1) {  // the final radix-2 stage
+    TwSet<1> t;
+    t.w[0] = tw[0];
+    return t;
+  } else {
+    return NoTw{};
+  }
+}
+
+// stages T and T+1 (s = 4^T): group (p2, q) runs the stage-T butterflies
+// p = p2 + k2 Q (k2 = 0..3) at q, whose outputs k are the inputs of the
+// stage-(T+1) butterfly (p2, q + s k)
 template <int LOG, int T, typename Ld>
-__device__ __forceinline__ void pass_pair(double2* Z, int lt, const double2* __restrict__ tw, Ld ld) {
+__device__ __forceinline__ void pass_pair(double2* Z, int lt, const TwSet<15>& t, Ld ld) {
   using F = WaveFft<LOG>;
   constexpr int M = F::M, s = 1 << (2 * T), q4 = M / (4 * s), Q = q4 / 4;
-  constexpr int GP = (M / 16) / F::G;
-  double2 r[GP][16];
-#pragma unroll
-  for (int i = 0; i < GP; i++) {
-    const int gi = lt + i * F::G;
-    const int q = gi & (s - 1), p2 = gi >> (2 * T);
+  const int q = lt & (s - 1), p2 = lt >> (2 * T);
+  double2 r[16];
+  {
     double2 v[4][4];
 #pragma unroll
     for (int k2 = 0; k2 < 4; k2++)
@@ -101,40 +148,33 @@ __device__ __forceinline__ void pass_pair(double2* Z, int lt, const double2* __r
       for (int k = 0; k < 4; k++) v[k2][k] = ld(q + s * (p2 + k2 * Q + k * q4));
     double2 o[4][4];
 #pragma unroll
-    for (int k2 = 0; k2 < 4; k2++) {
-      const int p = p2 + k2 * Q;
-      bfly4(v[k2][0], v[k2][1], v[k2][2], v[k2][3], tw[p * s], tw[2 * p * s], tw[3 * p * s], o[k2]);
-    }
-    const double2 w1 = tw[p2 * 4 * s], w2 = tw[2 * p2 * 4 * s], w3 = tw[3 * p2 * 4 * s];
+    for (int k2 = 0; k2 < 4; k2++)
+      bfly4(v[k2][0], v[k2][1], v[k2][2], v[k2][3], t.w[3 * k2], t.w[3 * k2 + 1], t.w[3 * k2 + 2], o[k2]);
 #pragma unroll
-    for (int k = 0; k < 4; k++) bfly4(o[0][k], o[1][k], o[2][k], o[3][k], w1, w2, w3, &r[i][4 * k]);
+    for (int k = 0; k < 4; k++) bfly4(o[0][k], o[1][k], o[2][k], o[3][k], t.w[12], t.w[13], t.w[14], &r[4 * k]);
   }
   F::sync();
 #pragma unroll
-  for (int i = 0; i < GP; i++) {
-    const int gi = lt + i * F::G;
-    const int q = gi & (s - 1), p2 = gi >> (2 * T);
+  for (int k = 0; k < 4; k++)
 #pragma unroll
-    for (int k = 0; k < 4; k++)
-#pragma unroll
-      for (int kk = 0; kk < 4; kk++) Z[fpad(q + s * k + 4 * s * (4 * p2 + kk))] = r[i][4 * k + kk];
-  }
+    for (int kk = 0; kk < 4; kk++) Z[fpad(q + s * k + 4 * s * (4 * p2 + kk))] = r[4 * k + kk];
   F::sync();
 }
 
 // one radix-4 stage T (the last one when their count is odd)
-template <int LOG, int T, typename Ld>
-__device__ __forceinline__ void pass_r4(double2* Z, int lt, const double2* __restrict__ tw, Ld ld) {
+template <int LOG, int T, typename Ld, int N>
+__device__ __forceinline__ void pass_r4(double2* Z, int lt, const TwSet<N>& t, Ld ld) {
   using F = WaveFft<LOG>;
   constexpr int M = F::M, s = 1 << (2 * T), q4 = M / (4 * s);
   constexpr int PER = (M / 4) / F::G;
+  static_assert(N == 3 * PER, "twiddles of the stage");
   double2 r[PER][4];
 #pragma unroll
   for (int i = 0; i < PER; i++) {
     const int b = lt + i * F::G;
     const int p = b >> (2 * T), q = b & (s - 1);
-    bfly4(ld(q + s * p), ld(q + s * (p + q4)), ld(q + s * (p + 2 * q4)), ld(q + s * (p + 3 * q4)), tw[p * s],
-          tw[2 * p * s], tw[3 * p * s], r[i]);
+    bfly4(ld(q + s * p), ld(q + s * (p + q4)), ld(q + s * (p + 2 * q4)), ld(q + s * (p + 3 * q4)), t.w[3 * i],
+          t.w[3 * i + 1], t.w[3 * i + 2], r[i]);
   }
   F::sync();
 #pragma unroll
@@ -149,12 +189,12 @@ __device__ __forceinline__ void pass_r4(double2* Z, int lt, const double2* __res
 
 // the final radix-2 stage (s = M / 2)
 template <int LOG, typename Ld>
-__device__ __forceinline__ void pass_r2(double2* Z, int lt, const double2* __restrict__ tw, Ld ld) {
+__device__ __forceinline__ void pass_r2(double2* Z, int lt, const TwSet<1>& t, Ld ld) {
   using F = WaveFft<LOG>;
   constexpr int s = F::M / 2;
   constexpr int PER = s / F::G;
   double2 r0[PER], r1[PER];
-  const double2 w = tw[0];
+  const double2 w = t.w[0];
 #pragma unroll
   for (int i = 0; i < PER; i++) {
     const int q = lt + i * F::G;
@@ -172,31 +212,35 @@ __device__ __forceinline__ void pass_r2(double2* Z, int lt, const double2* __res
   F::sync();
 }
 
-// the stages from T on; the first pass reads its input through `ld`
-template <int LOG, int T, typename Ld>
-__device__ __forceinline__ void fft_from(double2* Z, int lt, const double2* __restrict__ tw, Ld ld) {
+// the stages from T on, with the twiddles of the pass at T already loaded;
+// the first pass reads its input through `ld`
+template <int LOG, int T, typename Ld, typename Tw>
+__device__ __forceinline__ void fft_from(double2* Z, int lt, const double2* __restrict__ tw, Ld ld, const Tw& cur) {
   constexpr int NR4 = WaveFft<LOG>::NR4;
   if constexpr (T + 1 < NR4) {
-    pass_pair<LOG, T>(Z, lt, tw, ld);
-    fft_from<LOG, T + 2>(Z, lt, tw, ZLoad{Z});
+    const auto nxt = pass_tw<LOG, T + 2>(lt, tw);
+    pass_pair<LOG, T>(Z, lt, cur, ld);
+    fft_from<LOG, T + 2>(Z, lt, tw, ZLoad{Z}, nxt);
   } else if constexpr (T < NR4) {
-    pass_r4<LOG, T>(Z, lt, tw, ld);
-    fft_from<LOG, T + 1>(Z, lt, tw, ZLoad{Z});
+    const auto nxt = pass_tw<LOG, T + 1>(lt, tw);
+    pass_r4<LOG, T>(Z, lt, cur, ld);
+    fft_from<LOG, T + 1>(Z, lt, tw, ZLoad{Z}, nxt);
   } else if constexpr (LOG & 1) {
-    pass_r2<LOG>(Z, lt, tw, ld);
+    pass_r2<LOG>(Z, lt, cur, ld);
   }
 }
 
-// forward M-point FFT of the points ld(0..M-1) into Z (natural order)
-template <int LOG, typename Ld>
-__device__ __forceinline__ void wave_fft(double2* Z, int lt, const double2* __restrict__ tw, Ld ld) {
+// forward M-point FFT of the points ld(0..M-1) into Z (natural order);
+// tw0 = pass_tw<LOG, 0>(lt, tw), loaded by the caller as early as it likes
+template <int LOG, typename Ld, typename Tw>
+__device__ __forceinline__ void wave_fft(double2* Z, int lt, const double2* __restrict__ tw, Ld ld, const Tw& tw0) {
   if constexpr (LOG == 0) {
     const double2 v = ld(0);
     wave_sync();
     Z[0] = v;
     wave_sync();
   } else {
-    fft_from<LOG, 0>(Z, lt, tw, ld);
+    fft_from<LOG, 0>(Z, lt, tw, ld, tw0);
   }
 }
 
