@@ -34,10 +34,11 @@ __device__ __forceinline__ void dt_partial_body(const double* __restrict__ ux, c
   const int j0 = 1 + by * rows;
   if (i <= nx) {
     const double hx = hxxf[i - 1];
+    // rows past ny repeat row ny (a repeated value leaves a min unchanged),
+    // so the loads carry no branch and are all in flight together
 #pragma unroll 16
     for (int r = 0; r < rows; r++) {
-      const int j = j0 + r;
-      if (j > ny) break;
+      const int j = min(j0 + r, ny);
       if (i >= 2) m = fmin(m, hx / (fabs(ux[L::xf(i, j, nx)]) + small));
       m = fmin(m, hy / (fabs(uy[L::cc(i, j, nx)]) + small));
     }
