@@ -1,7 +1,8 @@
 """Throughput of the two-phase time step on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--path fused|staged]
-                    [--workload swarm|rising] [--no-cpu-baseline]
+                    [--workload swarm|rising|large] [--no-cpu-baseline]
+                    [--no-mixed] [--no-learned] [--no-ensemble] [--no-profile]
 
 Workload (default): plain FP64 bubble swarm, 2048 x 4096 cells on a 1 x 2
 domain (walls in x, periodic y), 64 bubbles of diameter 24-48 cells placed by
@@ -21,7 +22,8 @@ restatement, OpenMP) on a bounded sample of the same workload.
 Beside the headline: `mixed` reruns the workload in mixed precision, and
 `mixed.learned_curvature` with the learned curvature after training its
 network on the device; `ensemble` steps 256 independent rising bubbles of
-uneven grids as one batched ensemble.
+uneven grids as one batched ensemble.  `--workload large` runs the
+4096 x 8192 swarm with 1024 bubbles (Re 100, We 5) instead.
 """
 from __future__ import annotations
 
