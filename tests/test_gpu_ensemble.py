@@ -87,3 +87,29 @@ def test_autotune_restores_state(product):
         _, f1 = alone(product, w, 4)
         for n in FIELDS:
             assert np.array_equal(got[k][n], f1[n]), (k, n)
+
+
+def test_ensemble_with_a_wide_case(product):
+    """A 4096-column case (its row transforms take two wavefronts per row)
+    beside narrow ones: the batched transforms then run 128-thread
+    workgroups, two one-wavefront units per workgroup for the narrow cases.
+    Every case still gives the bits it gives alone."""
+    ws = configs.ensemble(3, seed=5)
+    ws[0].nx, ws[0].ny, ws[0].lx, ws[0].ly = 4096, 16, 1.0, 16 / 4096
+    ws[1].nx, ws[1].ny = 64, 128
+    ws[2].nx, ws[2].ny = 128, 256
+    ws[0].bubbles = [(0.5, 0.5 * 16 / 4096, 0.001)]
+    steps = 4
+    e = solver.Ensemble(product, specs_of(ws))
+    try:
+        for c, w in zip(e.cases, ws):
+            c.load_initial_condition(w.vof())
+        dts = np.array([e.step() for _ in range(steps)])
+        got = [{n: c.get(n) for n in FIELDS} for c in e.cases]
+    finally:
+        e.close()
+    for k, w in enumerate(ws):
+        dt1, f1 = alone(product, w, steps)
+        assert np.array_equal(dts[:, k], np.array(dt1)), k
+        for n in FIELDS:
+            assert np.array_equal(got[k][n], f1[n]), (k, n)
