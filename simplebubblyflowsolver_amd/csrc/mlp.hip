@@ -5,9 +5,10 @@
 //                     of known curvature (16 x 16 sub-cell sampling, random
 //                     sub-cell offsets, 8-fold symmetry) and from the solver's
 //                     own diffuse tanh initial profile;
-//   * k_mlp_grad      forward + backward of 64 samples per wavefront on BF16
-//                     MFMA with FP32 accumulation; writes one gradient partial
-//                     per wavefront;
+//   * k_mlp_grad      forward + backward of kGradS (16) samples per wavefront
+//                     on BF16 MFMA with FP32 accumulation; writes one gradient
+//                     partial per wavefront (a 4096-sample batch is 256
+//                     wavefronts);
 //   * k_mlp_adam      sums the partials in a fixed order and takes an Adam
 //                     step on the FP32 master weights, refreshing the BF16
 //                     copies the MFMA operands read;
@@ -263,7 +264,7 @@ __device__ inline float forward16(const Weights& w, const AOps& a, s4 x0, s4 x1,
   return s + w.p[OB3];
 }
 
-// B operands of subtile nt from 64 samples [sample][NIP] bf16 in LDS
+// B operands of subtile nt (16 samples) from [sample][NIP] bf16 in LDS
 __device__ inline void load_x(const unsigned short* X, int nt, s4& x0, s4& x1) {
   const int l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
   x0 = ld4(X + (16 * nt + lr) * NIP + 4 * lg);
@@ -280,30 +281,38 @@ __device__ inline float sum16(float v) {
 }
 
 // ---------------------------------------------------------------------------
-// training: one wavefront = one tile of 64 samples; part[tile][NPART].
+// training: one wavefront = one tile of S samples; part[tile][NPART].
 // Forward and the deltas run per 16-sample subtile; the weight gradients,
 // which sum over the samples, read the staged activations and deltas from
 // LDS as [hidden][sample].
 // ---------------------------------------------------------------------------
+// S samples per wavefront (a multiple of 16, at most 64): 16 keeps a
+// 4096-sample batch at 256 wavefronts; the weight gradients contract over
+// the S samples in S / 16 MFMA k-steps.
+constexpr int kGradS = 16;
+
+template <int S>
 __global__ __launch_bounds__(64) void k_mlp_grad(Weights w, const unsigned short* __restrict__ X,
                                                  const float* __restrict__ T, long n, long first,
                                                  long stride, float scale, float* __restrict__ part) {
-  __shared__ __align__(16) unsigned short sx[64 * NIP];   // [sample][feature]
-  __shared__ __align__(16) unsigned short sxt[NIP * 64];  // [feature][sample]
-  __shared__ __align__(16) unsigned short sa1[NH * 64];   // gelu(h1) [hidden][sample]
-  __shared__ __align__(16) unsigned short sd2[NH * 64];   // dh2 [hidden][sample]
-  __shared__ __align__(16) unsigned short sd1[NH * 64];   // dh1 [hidden][sample]
-  __shared__ float st[64];
+  static_assert(S % 16 == 0 && S <= 64, "samples per wavefront");
+  constexpr int NT = S / 16;
+  __shared__ __align__(16) unsigned short sx[S * NIP];   // [sample][feature]
+  __shared__ __align__(16) unsigned short sxt[NIP * S];  // [feature][sample]
+  __shared__ __align__(16) unsigned short sa1[NH * S];   // gelu(h1) [hidden][sample]
+  __shared__ __align__(16) unsigned short sd2[NH * S];   // dh2 [hidden][sample]
+  __shared__ __align__(16) unsigned short sd1[NH * S];   // dh1 [hidden][sample]
+  __shared__ float st[S];
   const int l = threadIdx.x, lr = l & 15, lg = l >> 4;
   const long tile = blockIdx.x;
-  // sample of this lane: a stride permutation of the data set
-  const long idx = (first + (tile * 64 + l) * stride) % n;
-  {
+  if (l < S) {
+    // sample of this lane: a stride permutation of the data set
+    const long idx = (first + (tile * S + l) * stride) % n;
     const unsigned short* src = X + (size_t)idx * NIP;
     for (int k = 0; k < NIP; k++) {
       const unsigned short v = src[k];
       sx[l * NIP + k] = v;
-      sxt[k * 64 + l] = v;
+      sxt[k * S + l] = v;
     }
     st[l] = T[idx];
   }
@@ -317,7 +326,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(Weights w, const unsigned short
 #pragma unroll
     for (int r = 0; r < 4; r++) gw3[mt][r] = gb2[mt][r] = gb1[mt][r] = 0.f;
 #pragma unroll 1
-  for (int nt = 0; nt < 4; nt++) {
+  for (int nt = 0; nt < NT; nt++) {
     s4 x0, x1;
     load_x(sx, nt, x0, x1);
     f4 h1[4], h2[4];
@@ -338,11 +347,11 @@ __global__ __launch_bounds__(64) void k_mlp_grad(Weights w, const unsigned short
         gw3[mt][r] += gelu(h) * dy;
         d[r] = w.p[OW3 + hid] * dy * dgelu(h);
         gb2[mt][r] += d[r];
-        sa1[hid * 64 + 16 * nt + lr] = f2bf(gelu(h1[mt][r]));
+        sa1[hid * S + 16 * nt + lr] = f2bf(gelu(h1[mt][r]));
       }
       d2b[mt] = pack4(d[0], d[1], d[2], d[3]);
 #pragma unroll
-      for (int r = 0; r < 4; r++) sd2[(16 * mt + 4 * lg + r) * 64 + 16 * nt + lr] = (unsigned short)d2b[mt][r];
+      for (int r = 0; r < 4; r++) sd2[(16 * mt + 4 * lg + r) * S + 16 * nt + lr] = (unsigned short)d2b[mt][r];
     }
     // dh1 = gelu'(h1) * (W2^T dh2)
 #pragma unroll
@@ -354,7 +363,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(Weights w, const unsigned short
       for (int r = 0; r < 4; r++) {
         const float d = c[r] * dgelu(h1[mt][r]);
         gb1[mt][r] += d;
-        sd1[(16 * mt + 4 * lg + r) * 64 + 16 * nt + lr] = f2bf(d);
+        sd1[(16 * mt + 4 * lg + r) * S + 16 * nt + lr] = f2bf(d);
       }
     }
   }
@@ -385,8 +394,8 @@ __global__ __launch_bounds__(64) void k_mlp_grad(Weights w, const unsigned short
     for (int nt = 0; nt < 4; nt++) {
       f4 c = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int ks = 0; ks < 4; ks++)
-        c = mma(ld4(sd2 + (16 * mt + lr) * 64 + 16 * ks + 4 * lg), ld4(sa1 + (16 * nt + lr) * 64 + 16 * ks + 4 * lg), c);
+      for (int ks = 0; ks < NT; ks++)
+        c = mma(ld4(sd2 + (16 * mt + lr) * S + 16 * ks + 4 * lg), ld4(sa1 + (16 * nt + lr) * S + 16 * ks + 4 * lg), c);
 #pragma unroll
       for (int r = 0; r < 4; r++) out[OW2 + (16 * mt + 4 * lg + r) * NH + 16 * nt + lr] = c[r];
     }
@@ -398,8 +407,8 @@ __global__ __launch_bounds__(64) void k_mlp_grad(Weights w, const unsigned short
     for (int nt = 0; nt < 2; nt++) {
       f4 c = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int ks = 0; ks < 4; ks++)
-        c = mma(ld4(sd1 + (16 * mt + lr) * 64 + 16 * ks + 4 * lg), ld4(sxt + (16 * nt + lr) * 64 + 16 * ks + 4 * lg), c);
+      for (int ks = 0; ks < NT; ks++)
+        c = mma(ld4(sd1 + (16 * mt + lr) * S + 16 * ks + 4 * lg), ld4(sxt + (16 * nt + lr) * S + 16 * ks + 4 * lg), c);
 #pragma unroll
       for (int r = 0; r < 4; r++) out[OW1 + (16 * mt + 4 * lg + r) * NIP + 16 * nt + lr] = c[r];
     }
@@ -417,12 +426,35 @@ struct AdamState {
   unsigned short* w2tb;
 };
 
-__global__ void k_mlp_adam(AdamState s, const float* __restrict__ part, int tiles, int adam, float lr,
-                           float b1t, float b2t, float* __restrict__ grad, float* __restrict__ loss) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k > NP) return;
+// A workgroup of 256 threads takes 32 parameters, 8 threads each: thread
+// `sl` sums the tiles [sl tiles / 8, (sl + 1) tiles / 8) in order, 16 loads
+// in flight at a time, and the first thread adds the 8 slice sums in order.
+constexpr int kAdamSlices = 8;
+__global__ __launch_bounds__(256) void k_mlp_adam(AdamState s, const float* __restrict__ part, int tiles, int adam,
+                                                  float lr, float b1t, float b2t, float* __restrict__ grad,
+                                                  float* __restrict__ loss) {
+  __shared__ float slice[256];
+  const int sl = (int)threadIdx.x % kAdamSlices;
+  const int k = (int)blockIdx.x * (256 / kAdamSlices) + (int)threadIdx.x / kAdamSlices;
+  const int t1 = (int)(((long)tiles * (sl + 1)) / kAdamSlices);
+  int t = (int)(((long)tiles * sl) / kAdamSlices);
+  float gs = 0.f;
+  if (k <= NP) {
+    for (; t + 16 <= t1; t += 16) {
+      float v[16];
+#pragma unroll
+      for (int u = 0; u < 16; u++) v[u] = part[(size_t)(t + u) * NPART + k];
+#pragma unroll
+      for (int u = 0; u < 16; u++) gs += v[u];
+    }
+    for (; t < t1; t++) gs += part[(size_t)t * NPART + k];
+  }
+  slice[threadIdx.x] = gs;
+  __syncthreads();
+  if (sl != 0 || k > NP) return;
   float g = 0.f;
-  for (int t = 0; t < tiles; t++) g += part[(size_t)t * NPART + k];
+#pragma unroll
+  for (int u = 0; u < kAdamSlices; u++) g += slice[threadIdx.x + u];
   if (grad) grad[k] = g;
   if (k == NP) {
     if (loss) *loss = g;
@@ -696,8 +728,8 @@ static void free_samples(Samples& s) {
 
 // one pass of `tiles` tiles starting at permuted position `first`
 static int grad_pass(const Samples& s, long first, long stride, int tiles, float scale, float* part) {
-  SBFS_TIMED("k_mlp_grad", hipLaunchKernelGGL(k_mlp_grad, dim3(tiles), dim3(64), 0, stream(), weights(), s.x, s.t,
-                                              s.n, first, stride, scale, part));
+  SBFS_TIMED("k_mlp_grad", hipLaunchKernelGGL(k_mlp_grad<kGradS>, dim3(tiles), dim3(64), 0, stream(), weights(), s.x,
+                                              s.t, s.n, first, stride, scale, part));
   return check_launch("mlp: gradients");
 }
 
@@ -787,7 +819,7 @@ extern "C" int sbfs_mlp_get_params(float* params, size_t n) {
 // parameters; grad[NP], *loss = the mean squared error
 extern "C" int sbfs_mlp_gradient(const float* features, const float* labels, size_t n, float* grad,
                                  double* loss) {
-  SBFS_REQUIRE(features && labels && grad && loss && n > 0 && n % 64 == 0, "sbfs_mlp_gradient: bad arguments");
+  SBFS_REQUIRE(features && labels && grad && loss && n > 0 && n % kGradS == 0, "sbfs_mlp_gradient: bad arguments");
   SBFS_REQUIRE(g_net.p, "sbfs_mlp_gradient: no parameters set");
   std::vector<unsigned short> hx(NIP * n);
   for (size_t k = 0; k < NIP * n; k++) hx[k] = f2bf(features[k]);
@@ -795,7 +827,7 @@ extern "C" int sbfs_mlp_gradient(const float* features, const float* labels, siz
   SBFS_HIP(hipMalloc(&s.x, sizeof(unsigned short) * NIP * n));
   SBFS_HIP(hipMalloc(&s.t, sizeof(float) * n));
   s.n = (long)n;
-  const int tiles = (int)(n / 64);
+  const int tiles = (int)(n / kGradS);
   float *part = nullptr, *g = nullptr;
   SBFS_HIP(hipMalloc(&part, sizeof(float) * NPART * tiles));
   SBFS_HIP(hipMalloc(&g, sizeof(float) * NPART));
@@ -803,7 +835,7 @@ extern "C" int sbfs_mlp_gradient(const float* features, const float* labels, siz
   SBFS_HIP(hipMemcpyAsync(s.t, labels, sizeof(float) * n, hipMemcpyHostToDevice, stream()));
   int rc = grad_pass(s, 0, 1, tiles, 2.f / (float)n, part);
   if (!rc) {
-    hipLaunchKernelGGL(k_mlp_adam, dim3(cdiv(NPART, 256)), dim3(256), 0, stream(), adam_state(), (const float*)part,
+    hipLaunchKernelGGL(k_mlp_adam, dim3(cdiv(NPART * kAdamSlices, 256)), dim3(256), 0, stream(), adam_state(), (const float*)part,
                        tiles, 0, 0.f, 1.f, 1.f, g, (float*)nullptr);
     rc = check_launch("sbfs_mlp_gradient");
   }
@@ -854,14 +886,14 @@ extern "C" int sbfs_mlp_predict(const double* stencils, size_t n, double* out) {
 // error of epoch e (may be null)
 extern "C" int sbfs_mlp_train(uint64_t seed, size_t nsamples, int epochs, int batch, double lr,
                               double* history) {
-  SBFS_REQUIRE(nsamples > 0 && epochs > 0 && batch > 0 && batch % 64 == 0 && nsamples % (size_t)batch == 0,
-               "sbfs_mlp_train: nsamples must be a multiple of batch, batch a multiple of 64");
+  SBFS_REQUIRE(nsamples > 0 && epochs > 0 && batch > 0 && batch % kGradS == 0 && nsamples % (size_t)batch == 0,
+               "sbfs_mlp_train: nsamples must be a multiple of batch, batch a multiple of 16");
   std::vector<float> p0;
   init_params(seed, p0);
   if (set_params(p0.data())) return 1;
   Samples s;
   if (make_samples(seed, (long)nsamples, s, nullptr)) return 1;
-  const int tiles = batch / 64;
+  const int tiles = batch / kGradS;
   const long steps = (long)(nsamples / (size_t)batch);
   float *part = nullptr, *loss = nullptr;
   SBFS_HIP(hipMalloc(&part, sizeof(float) * NPART * tiles));
@@ -888,7 +920,7 @@ extern "C" int sbfs_mlp_train(uint64_t seed, size_t nsamples, int epochs, int ba
       const float b1t = 1.f - powf(0.9f, (float)g_net.step), b2t = 1.f - powf(0.999f, (float)g_net.step);
       rc = grad_pass(s, offset + k * batch * stride, stride, tiles, 2.f / (float)batch, part);
       if (!rc) {
-        SBFS_TIMED("k_mlp_adam", hipLaunchKernelGGL(k_mlp_adam, dim3(cdiv(NPART, 256)), dim3(256), 0, stream(),
+        SBFS_TIMED("k_mlp_adam", hipLaunchKernelGGL(k_mlp_adam, dim3(cdiv(NPART * kAdamSlices, 256)), dim3(256), 0, stream(),
                                                     adam_state(), (const float*)part, tiles, 1, lrk, b1t, b2t,
                                                     (float*)nullptr, loss + e * steps + k));
         rc = check_launch("sbfs_mlp_train");
