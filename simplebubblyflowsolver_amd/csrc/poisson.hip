@@ -470,8 +470,19 @@ __global__ SBFS_ROWS_ATTR(LOG) void k_dct3_psi(ModeSrc src, double* __restrict__
                                                const double2* __restrict__ tw, const double2* __restrict__ t1,
                                                const double2* __restrict__ t2) {
   extern __shared__ double2 lds[];
-  // XCD-major rows: an XCD's L2 holds the separators of its own chunks
-  const int unit = xcd_group((int)blockIdx.x, (int)gridDim.x);
+  // Rows in XCD order.  One row per unit with a partitioned solve: XCD x
+  // takes chunk half x / 4 and quarter x % 4 of the rows within a chunk, so
+  // its L2 holds half the separators and a quarter of the spike tables (3 MB
+  // of 8 at 4096 x 8192).  Otherwise XCD-major rows: an XCD's L2 holds the
+  // separators of its own chunks.
+  int unit;
+  const int n = (int)gridDim.x, b = (int)blockIdx.x;
+  if (WaveFft<LOG>::R == 1 && src.L >= 4 && (src.P & 1) == 0 && (n & 7) == 0) {
+    const int x = b & 7, i = b >> 3, kq = src.L / 4;
+    unit = ((x >> 2) * (src.P / 2) + i / kq) * src.L + (x & 3) * kq + i % kq;
+  } else {
+    unit = xcd_group(b, n);
+  }
   dct3_wave<LOG, true>(src, ny, unit * WaveFft<LOG>::R, tw, t1, t2, psi, lds, (int)threadIdx.x);
 }
 
