@@ -751,6 +751,22 @@ static void fill_post(const DomainCtx* c, const fluid_t* fluid, const interface_
 static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, interface_t* interface,
                       const Coef& cf, double* rhs, int band, bool use_tiles, double learned_h) {
   const int nx = (int)c->nx, ny = (int)c->ny;
+  // learned curvature of each stage's input vof: on the side stream, forked
+  // after the kernel that wrote that vof and joined before the stage kernel
+  // that reads kappa, so it overlaps the dt reduction (stage 0) or the
+  // previous stage's Poisson solve; with per-launch timing it stays on the
+  // library stream, right before the stage kernel
+  const bool learned = learned_h > 0. && !use_tiles;
+  const bool overlap = learned && !prof_active();
+  const unsigned* kappa_n = nullptr;
+  auto fork_curv = [&](int k, const double* vof) -> int {
+    SBFS_HIP(hipEventRecord(fork_event(k), stream()));
+    SBFS_HIP(hipStreamWaitEvent(side_stream(), fork_event(k), 0));
+    if (curv_mlp_launch(vof, g_scr.kappa, nx, ny, learned_h, &kappa_n, side_stream())) return 1;
+    SBFS_HIP(hipEventRecord(join_event(k), side_stream()));
+    return 0;
+  };
+  if (overlap && fork_curv(0, D(interface->vof))) return 1;
   if (compute_dt_device(domain, c, fluid, interface)) return 1;
   const double* dtp = c->dt_dev;
   const bool fold = !use_tiles && march_fold();
@@ -815,8 +831,13 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
     } else {
       MarchArgs m;
       fill_march(domain, c, fluid, interface, cf, rhs, band, k, corr, us, vs, g_scr.vof, g_scr.p, dtp, m);
-      if (learned_h > 0.) {
-        if (curv_mlp_launch(a.vof, g_scr.kappa, nx, ny, learned_h, &m.kappa_n)) return 1;
+      if (learned) {
+        if (overlap) {
+          SBFS_HIP(hipStreamWaitEvent(stream(), join_event(k), 0));
+          m.kappa_n = kappa_n;
+        } else if (curv_mlp_launch(a.vof, g_scr.kappa, nx, ny, learned_h, &m.kappa_n, stream())) {
+          return 1;
+        }
         m.kappa = g_scr.kappa;
       }
       const dim3 grid(cdiv(nx, kStrip), cdiv(ny, band)), block(64);
@@ -843,6 +864,9 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
       interface->vof.data = g_scr.vof;
       g_scr.vof = (double*)t;
     }
+    // the next stage's curvature from the new vof, beside this stage's
+    // Poisson solve (the stage kernel that read kappa has been launched)
+    if (overlap && k < 2 && fork_curv(k + 1, D(interface->vof))) return 1;
     if (use_tiles) {
       if (poisson_solve_fused(domain, c, dtp, g_old, g_new, g_scr.ux, g_scr.uy, D(interface->vof), fluid))
         return 1;

@@ -738,7 +738,8 @@ static int grad_pass(const Samples& s, long first, long stride, int tiles, float
 // the learned curvature of a stage's input vof: the gate, then the network
 // on the listed segments (list and counter kept per process, grown with the
 // grid)
-int curv_mlp_launch(const double* vof, float* kappa, int nx, int ny, double h, const unsigned** count) {
+int curv_mlp_launch(const double* vof, float* kappa, int nx, int ny, double h, const unsigned** count,
+                    hipStream_t st) {
   using namespace mlp;
   SBFS_REQUIRE(g_net.trained, "learned curvature: no trained network (sbfs_mlp_train or sbfs_mlp_set_params)");
   static unsigned* buf = nullptr;  // [0] counter, then the segment list
@@ -753,10 +754,10 @@ int curv_mlp_launch(const double* vof, float* kappa, int nx, int ny, double h, c
     cap = segs + 1;
     graph_epoch_bump();  // captured steps hold the old address
   }
-  hipLaunchKernelGGL(k_curv_reset, dim3(1), dim3(64), 0, stream(), buf);
+  hipLaunchKernelGGL(k_curv_reset, dim3(1), dim3(64), 0, st, buf);
   SBFS_TIMED("k_curv_gate", hipLaunchKernelGGL(k_curv_gate, dim3(segs_x, cdiv(ny, 4 * kGateRows)), dim3(256), 0,
-                                               stream(), vof, kappa, nx, ny, buf, buf + 1));
-  SBFS_TIMED("k_curv_mlp", hipLaunchKernelGGL(k_curv_mlp, dim3(2048), dim3(64), 0, stream(), weights(), vof, kappa,
+                                               st, vof, kappa, nx, ny, buf, buf + 1));
+  SBFS_TIMED("k_curv_mlp", hipLaunchKernelGGL(k_curv_mlp, dim3(2048), dim3(64), 0, st, weights(), vof, kappa,
                                               nx, ny, (float)(1. / h), (const unsigned*)buf,
                                               (const unsigned*)(buf + 1), segs_x));
   *count = buf;
@@ -970,7 +971,7 @@ extern "C" int sbfs_mlp_coverage(const domain_t* domain, const interface_t* inte
   SBFS_HIP(hipMalloc(&cnt, sizeof(unsigned long long)));
   SBFS_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), stream()));
   const unsigned* listed = nullptr;
-  int rc = curv_mlp_launch((const double*)interface->vof.data, k, nx, ny, domain->hy, &listed);
+  int rc = curv_mlp_launch((const double*)interface->vof.data, k, nx, ny, domain->hy, &listed, stream());
   unsigned long long h = 0;
   if (!rc) {
     hipLaunchKernelGGL(k_count_learned, dim3(1024), dim3(256), 0, stream(), (const float*)k, n, cnt);

@@ -104,12 +104,17 @@ RK rk(size_t stage) {
 // ---------------------------------------------------------------------------
 static Options g_opt;
 static hipStream_t g_stream = nullptr;
+static hipStream_t g_side = nullptr;
+static hipEvent_t g_fork[3] = {}, g_join[3] = {};
 static bool g_ready = false;
 static std::mutex g_mutex;
 static std::vector<std::pair<const domain_t*, DomainCtx*>> g_domains;
 
 Options& options() { return g_opt; }
 hipStream_t stream() { return g_stream; }
+hipStream_t side_stream() { return g_side; }
+hipEvent_t fork_event(int k) { return g_fork[k]; }
+hipEvent_t join_event(int k) { return g_join[k]; }
 
 int init_runtime() {
   if (g_ready) return 0;
@@ -118,8 +123,15 @@ int init_runtime() {
   int n = 0;
   SBFS_HIP(hipGetDeviceCount(&n));
   SBFS_REQUIRE(n > 0, "no HIP device visible");
-  // default-priority, non-blocking stream
+  // default-priority, non-blocking streams: the library's, and a side
+  // stream for work that overlaps it within a step (forked and joined with
+  // the events below, which graph capture records as dependencies)
   SBFS_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  SBFS_HIP(hipStreamCreateWithFlags(&g_side, hipStreamNonBlocking));
+  for (int k = 0; k < 3; k++) {
+    SBFS_HIP(hipEventCreateWithFlags(&g_fork[k], hipEventDisableTiming));
+    SBFS_HIP(hipEventCreateWithFlags(&g_join[k], hipEventDisableTiming));
+  }
   const char* p = getenv("SBFS_PATH");
   if (p) g_opt.path = atoi(p);
   g_ready = true;

@@ -139,6 +139,11 @@ struct Options {
 
 Options& options();
 hipStream_t stream();
+// a second default-priority stream for work that overlaps the library
+// stream within a step, and fork / join events (k = 0..2)
+hipStream_t side_stream();
+hipEvent_t fork_event(int k);
+hipEvent_t join_event(int k);
 int init_runtime();
 // device context of a host domain (created / refreshed on demand)
 DomainCtx* domain_ctx(const domain_t* d);
@@ -152,7 +157,8 @@ int load_dt_coefs();
 // learned curvature (mlp.hip): kappa of the interior cells of `vof`, NaN
 // where the plain curvature stays; h is the (uniform, square) cell size;
 // *count receives the device address of the number of listed row segments
-int curv_mlp_launch(const double* vof, float* kappa, int nx, int ny, double h, const unsigned** count);
+int curv_mlp_launch(const double* vof, float* kappa, int nx, int ny, double h, const unsigned** count,
+                    hipStream_t st);
 // Captured step graphs hold device addresses that their keys do not list
 // (Poisson plan buffers, domain metrics, the inference's segment list).
 // Whatever frees such memory bumps this epoch, which is part of every key,
