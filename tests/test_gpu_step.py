@@ -77,31 +77,6 @@ def test_swarm_invariants(product):
         g.close()
 
 
-def test_large_swarm_matches_oracle(oracle_mod, product):
-    """4096 x 8192 with 1024 bubbles (4096-point DCT rows, 64-row chunks of
-    the partitioned solve), 2 steps on the GPU against the oracle on the
-    host: velocity and vof within 1e-10 relative, p and psi after removing
-    their means."""
-    w = configs.large_swarm()
-    print(f"{w.name}: oracle and GPU, 2 steps", flush=True)
-    oracle_mod.set_threads(16)
-    oracle_mod.set_poisson(dct=oracle_mod.DCT_FFT, tdm=oracle_mod.TDM_SEQUENTIAL)
-    o, g = new_pair(oracle_mod, product, w.nx, w.ny, phys=dict(Re=w.Re, We=w.We))
-    try:
-        vof = w.vof()
-        o.load_initial_condition(vof)
-        g.load_initial_condition(vof)
-        for _ in range(2):
-            assert g.integrate() == pytest.approx(o.integrate(), rel=1e-12)
-        for n in ("ux", "uy", "vof"):
-            assert rel(g.get(n), o.get(n)) <= 1e-10, n
-        for n in ("p", "psi0"):
-            assert rel(demean(g.get(n)), demean(o.get(n))) <= 1e-10, n
-    finally:
-        o.close()
-        g.close()
-
-
 @pytest.mark.parametrize("nx,ny", [(64, 128), (32, 66)])
 def test_fused_path_matches_staged_bitwise(oracle_mod, product, nx, ny):
     """The fused kernels recompute every intermediate with the staged
