@@ -251,9 +251,10 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
     if constexpr (kPinned) return lane_at(row, loff);
     else return row[lane];
   };
-  auto ld_vof = [&](int j) { return c_vof ? AT(vof_row(a.vof, wrap_r(j, ny))) : 0.; };
-  auto ld_ux = [&](int j) { return c_fx ? AT(xf_row(a.ux, wrap_r(j, ny))) : 0.; };
-  auto ld_cc = [&](const double* f, int j) { return c_vof ? AT(cc_row(f, wrap_r(j, ny))) : 0.; };
+  // loads of row jw, already wrapped into 1..ny
+  auto ld_vof = [&](int jw) { return c_vof ? AT(vof_row(a.vof, jw)) : 0.; };
+  auto ld_ux = [&](int jw) { return c_fx ? AT(xf_row(a.ux, jw)) : 0.; };
+  auto ld_cc = [&](const double* f, int jw) { return c_vof ? AT(cc_row(f, jw)) : 0.; };
   auto DEN = [&](double v) { return mix_clamp(a.denr, a.dlo, a.dhi, v); };
   // viscosity at x face c from the vof on its two sides (viscosity.c:37-53)
   auto VISUX = [&](int col, R vl, R vr) {
@@ -296,16 +297,46 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
     if (LR) rings[k][lane] = v;
     else reg = v;
   };
+  // Every input of step s+1 is loaded during step s, unconditionally (the
+  // rows wrap, so the last step's loads stay in bounds and go unused), and
+  // is first read by the fence below, after the bulk of the step's
+  // arithmetic and before its first store.  The vector-memory counter
+  // retires loads and stores in issue order, so a wait placed after the
+  // step's stores would also wait for them; at the fence only the step's
+  // loads and the previous step's stores are outstanding.
+  // w1..w3: rows s+1..s+3 wrapped into 1..ny, advanced with s
+  int w1 = wrap_r(s_first + 1, ny), w2 = wrap_r(s_first + 2, ny), w3 = wrap_r(s_first + 3, ny);
   // prefetched inputs of the next step (the folded projection needs psi one
   // row further ahead)
-  double nv = ld_vof(s_first + 2), nux = ld_ux(s_first + 1), nuy = ld_cc(a.uy, s_first + 1);
-  double np = ld_cc(a.p, s_first), nps = ld_cc(a.psi0, CORR ? s_first + 1 : s_first);
-  double npo = CORR ? ld_cc(a.psi_old, s_first + 1) : 0.;
+  double nv = ld_vof(w2), nux = ld_ux(w1), nuy = ld_cc(a.uy, w1);
+  double np = ld_cc(a.p, wrap_r(s_first, ny)), nps = ld_cc(a.psi0, CORR ? w1 : wrap_r(s_first, ny));
+  double npo = CORR ? ld_cc(a.psi_old, w1) : 0.;
   // learned curvature of row s, prefetched a row ahead (0 where absent)
   // (no listed segment: every value is NaN, so none is read)
   const float* kap = (a.kappa && *a.kappa_n != 0u) ? a.kappa : nullptr;
-  auto ld_kap = [&](int j) { return (kap && c_in) ? lane_at(ci_row(kap, wrap_r(j, ny)), loff >> 1) : 0.f; };
-  float nkap = ld_kap(s_first);
+  auto ld_kap = [&](int jw) { return (kap && c_in) ? lane_at(ci_row(kap, jw), loff >> 1) : 0.f; };
+  float nkap = ld_kap(wrap_r(s_first, ny));
+  // beta sources of row s (stages 1, 2): the vof's for every marched row,
+  // uy's from the first output row on (the predicted uy of row je+1 enters
+  // the Poisson rhs of row je), ux's on output rows
+  auto ld_sb = [&](int s_, int jw, double& bv, double& by, double& bx) {
+    bv = by = bx = 0.;
+    if (has_b && c_in) {
+      bv = AT(ci_row(a.srcb_vof, jw));
+      if (s_ >= jb) by = AT(ci_row(a.srcb_uy, jw));
+      if (s_ >= jb && s_ <= je && c >= 2) bx = AT(fx_row(a.srcb_ux, jw));
+    }
+  };
+  double nsbv, nsby, nsbx;
+  ld_sb(s_first, wrap_r(s_first, ny), nsbv, nsby, nsbx);
+  // the fence below, for the first step's inputs: without it the loop entry
+  // would carry outstanding loads into every step's first reads
+  auto fence = [&]() {
+    asm volatile("" : "+v"(nv), "+v"(nux), "+v"(nuy), "+v"(np), "+v"(nps), "+v"(nkap), "+v"(nsbv), "+v"(nsby),
+                 "+v"(nsbx));
+    if constexpr (CORR) asm volatile("" : "+v"(npo));
+  };
+  fence();
   for (int s = s_first; s <= je + 1; s++) {
     // -- new input rows -----------------------------------------------------
     rvm1 = R(v0); v0 = vp1; vp1 = vp2; vp2 = nv;
@@ -321,24 +352,18 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
     }
     const bool outrow = s >= jb && s <= je;
     const float kap = nkap;
-    double sbv = 0., sbx = 0., sby = 0.;
-    if (has_b && c_in) {
-      // the new vof of every marched row needs its beta source; the
-      // predicted uy of row je+1 enters the Poisson rhs of row je
-      const int sw = wrap_r(s, ny);
-      sbv = AT(ci_row(a.srcb_vof, sw));
-      if (s >= jb) sby = AT(ci_row(a.srcb_uy, sw));
-      if (outrow && c >= 2) sbx = AT(fx_row(a.srcb_ux, s));
-    }
-    if (s + 1 <= je + 1) {
-      nv = ld_vof(s + 3);
-      nux = ld_ux(s + 2);
-      nuy = ld_cc(a.uy, s + 2);
-      np = ld_cc(a.p, s + 1);
-      nps = ld_cc(a.psi0, CORR ? s + 2 : s + 1);
-      if (CORR) npo = ld_cc(a.psi_old, s + 2);
-      nkap = ld_kap(s + 1);
-    }
+    const double sbv = nsbv, sby = nsby, sbx = nsbx;
+    nv = ld_vof(w3);
+    nux = ld_ux(w2);
+    nuy = ld_cc(a.uy, w2);
+    np = ld_cc(a.p, w1);
+    nps = ld_cc(a.psi0, CORR ? w2 : w1);
+    if (CORR) npo = ld_cc(a.psi_old, w2);
+    nkap = ld_kap(w1);
+    ld_sb(s + 1, w1, nsbv, nsby, nsbx);
+    w1 = w2;
+    w2 = w3;
+    w3 = w3 == ny ? 1 : w3 + 1;
     const R rux0 = R(ux0), ruxp1 = R(uxp1), ruy0 = R(uy0), ruyp1 = R(uyp1);
     const R rv0 = R(v0), rvp1 = R(vp1);
     // -- corner gradients, row s+2 (curvature_tensor.c:16-49) ----------------
@@ -515,6 +540,10 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
     const R qy_uy_new = (SB(T_ECY) * mfy0 + SB(T_ECY) * mfyp1) * (R(0.5) * ruy0 + R(0.5) * ruyp1);
     const R cyf_txy1 = SB(T_CYF) * txy1;
     const R cyc_tyy0 = SB(T_CYC) * tyy0;
+    // The fence: an empty statement that reads and redefines the next step's
+    // inputs, so the compiler waits for their loads here, ahead of this
+    // step's first store, rather than where the loop carries them over.
+    fence();
     // -- predicted ux, row s (predict/ux.c, force.c) ---------------------------
     double uxs = 0.;
     if (c >= 2 && c <= nx) {
@@ -637,9 +666,10 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
       rings[RG_PM1][lane] = p0;
       rings[RG_DNM1][lane] = dn0;
     }
-    if (CORR && s + 1 <= je + 1) {
+    if (CORR) {
       // The previous stage's projection of the rows prefetched for the next
-      // step, formed here where the stencil's temporaries are dead: ux of row
+      // step (unused after the last step), formed here where the stencil's
+      // temporaries are dead: ux of row
       // s+2, uy of face s+2 (correct_velocity/ux.c:26-45, uy.c:26-43) and the
       // pressure of row s+1 (update_pressure.c:13-24, written with the
       // outputs of that row), with k_post's expressions and the lane's own
