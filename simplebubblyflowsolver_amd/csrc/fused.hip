@@ -631,10 +631,16 @@ static bool march_fold() {
 
 // Rows per band of k_march.  Every wavefront marches its band plus kWarm + 2
 // rows of ring fill, and a wavefront lives for the whole band, so the grid
-// is sized to one round of the device: as many bands as there are resident
-// wavefront slots per strip (occupancy x CUs), each as short as that
-// allows.  A partial second round would leave most SIMDs idle while it
-// runs.  SBFS_MARCH_BAND overrides the choice.
+// is sized to whole rounds of the device: a multiple of the resident
+// wavefront slots per strip (occupancy x CUs), each band as short as that
+// allows.  A partial last round would leave most SIMDs idle while it runs.
+// The FP64 bodies take about twice as long over a row with interface cells
+// (the THINC flux's tanh and log), so in a single round the bands a bubble
+// crosses finish last, 1.5x after the mean wavefront; three rounds let the
+// dispatcher balance them for 3x the ring fill (measured on the 2048 x 4096
+// swarm: k_march 397 -> 363 us, k_march_corr 522 -> 494 us).  The FP32
+// bodies' cheaper fluxes leave less to balance, and one round is fastest.
+// SBFS_MARCH_BAND overrides the choice.
 static int march_band(int nx, int ny, bool mixed) {
   static long slots[2] = {0, 0};
   long& sl = slots[mixed ? 1 : 0];
@@ -653,7 +659,8 @@ static int march_band(int nx, int ny, bool mixed) {
     if (b >= 1) return b < ny ? b : ny;
   }
   const long strips = (nx + kStrip - 1) / kStrip;
-  long bands = slots_now / strips;
+  const long rounds = mixed ? 1 : 3;
+  long bands = rounds * slots_now / strips;
   if (bands < 1) bands = 1;
   long band = (ny + bands - 1) / bands;
   if (band < 4) band = 4;
