@@ -494,6 +494,10 @@ struct FusedScratch {
   double* uy2 = nullptr;
   double* p = nullptr;    // pressure written by a stage kernel with the folded projection
   float* kappa = nullptr;  // learned curvature of a stage's input vof (interior cells)
+  // the stage kernels' dispatch order and unit weights (march.hpp)
+  unsigned* weight = nullptr;
+  unsigned* order = nullptr;
+  int units = 0;
 };
 static FusedScratch g_scr;
 
@@ -503,6 +507,8 @@ static int scratch(const domain_t* d, const interface_t* s, const fluid_t* f) {
   graph_epoch_bump();
   for (double* q : {g_scr.vof, g_scr.ux, g_scr.uy, g_scr.ux2, g_scr.uy2, g_scr.p}) (void)hipFree(q);
   (void)hipFree(g_scr.kappa);
+  (void)hipFree(g_scr.weight);
+  (void)hipFree(g_scr.order);
   g_scr = FusedScratch();
   const std::pair<double**, size_t> bufs[] = {{&g_scr.vof, s->vof.datasize}, {&g_scr.ux, f->ux.datasize},
                                               {&g_scr.uy, f->uy.datasize},   {&g_scr.ux2, f->ux.datasize},
@@ -514,6 +520,22 @@ static int scratch(const domain_t* d, const interface_t* s, const fluid_t* f) {
   SBFS_HIP(hipMalloc(&g_scr.kappa, sizeof(float) * d->mysizes[0] * d->mysizes[1]));
   g_scr.nx = d->mysizes[0];
   g_scr.ny = d->mysizes[1];
+  return 0;
+}
+
+// weight and order buffers for n units (zero weights at first: any order)
+static int order_buffers(int n) {
+  if (g_scr.units >= n) return 0;
+  (void)hipStreamSynchronize(stream());
+  graph_epoch_bump();
+  (void)hipFree(g_scr.weight);
+  (void)hipFree(g_scr.order);
+  g_scr.weight = g_scr.order = nullptr;
+  g_scr.units = 0;
+  SBFS_HIP(hipMalloc(&g_scr.weight, sizeof(unsigned) * (size_t)n));
+  SBFS_HIP(hipMalloc(&g_scr.order, sizeof(unsigned) * (size_t)n));
+  SBFS_HIP(hipMemsetAsync(g_scr.weight, 0, sizeof(unsigned) * (size_t)n, stream()));
+  g_scr.units = n;
   return 0;
 }
 
@@ -629,6 +651,16 @@ static bool march_fold() {
   return on == 1;
 }
 
+// Whether the stage kernels run their units heaviest first (k_march_order):
+// with several rounds of bands, the FP64 default.  SBFS_MARCH_ORDER=0 keeps
+// the grid's own order, =1 orders the FP32 kernels too.
+static bool march_ordered(bool mixed) {
+  const char* e = getenv("SBFS_MARCH_ORDER");
+  if (e && e[0] == '0') return false;
+  if (e && e[0] == '1') return true;
+  return !mixed;
+}
+
 // Rows per band of k_march.  Every wavefront marches its band plus kWarm + 2
 // rows of ring fill, and a wavefront lives for the whole band, so the grid
 // is sized to whole rounds of the device: a multiple of the resident
@@ -636,11 +668,13 @@ static bool march_fold() {
 // allows.  A partial last round would leave most SIMDs idle while it runs.
 // The FP64 bodies take about twice as long over a row with interface cells
 // (the THINC flux's tanh and log), so in a single round the bands a bubble
-// crosses finish last, 1.5x after the mean wavefront; three rounds let the
-// dispatcher balance them for 3x the ring fill (measured on the 2048 x 4096
-// swarm: k_march 397 -> 363 us, k_march_corr 522 -> 494 us).  The FP32
-// bodies' cheaper fluxes leave less to balance, and one round is fastest.
-// SBFS_MARCH_BAND overrides the choice.
+// crosses finish last, 1.5x after the mean wavefront.  Several rounds let
+// the dispatcher balance them, for more ring fill: two rounds when the units
+// run heaviest first (k_march_order), three in the grid's own order
+// (measured on the 2048 x 4096 swarm: k_march 397 / 363 / 305 us and
+// k_march_corr 522 / 510 / 444 us for one round, three unordered and two
+// ordered).  The FP32 bodies' cheaper fluxes leave less to balance, and one
+// round is fastest for them.  SBFS_MARCH_BAND overrides the choice.
 static int march_band(int nx, int ny, bool mixed) {
   static long slots[2] = {0, 0};
   long& sl = slots[mixed ? 1 : 0];
@@ -659,7 +693,7 @@ static int march_band(int nx, int ny, bool mixed) {
     if (b >= 1) return b < ny ? b : ny;
   }
   const long strips = (nx + kStrip - 1) / kStrip;
-  const long rounds = mixed ? 1 : 3;
+  const long rounds = mixed ? 1 : (march_ordered(mixed) ? 2 : 3);
   long bands = rounds * slots_now / strips;
   if (bands < 1) bands = 1;
   long band = (ny + bands - 1) / bands;
@@ -686,6 +720,8 @@ static void fill_march(const domain_t* domain, const DomainCtx* c, const fluid_t
   m.pg_new = rk((k + 2) % 3).g;
   m.kappa = nullptr;
   m.kappa_n = nullptr;
+  m.order = nullptr;
+  m.weight = nullptr;
   m.srcb_vof = D(interface->src[1]);
   m.srcb_ux = D(fluid->srcux[1]);
   m.srcb_uy = D(fluid->srcuy[1]);
@@ -777,6 +813,19 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
   if (compute_dt_device(domain, c, fluid, interface)) return 1;
   const double* dtp = c->dt_dev;
   const bool fold = !use_tiles && march_fold();
+  // the units in order of the interface rows the last stage kernel found in
+  // them (the previous step's; the bubbles move a fraction of a cell per
+  // step), for all three stages of this step
+  const bool mixed = options().precision == SBFS_PRECISION_MIXED;
+  const bool ordered = !use_tiles && march_ordered(mixed);
+  const int units = march_units(nx, ny, band);
+  if (ordered) {
+    SBFS_REQUIRE(g_scr.units >= units, "integrate: no dispatch-order buffers for %d units", units);
+    SBFS_TIMED("k_march_order", hipLaunchKernelGGL(k_march_order, dim3(1), dim3(1024), 0, stream(),
+                                                   (const unsigned*)g_scr.weight, g_scr.order, units,
+                                                   march_max_weight(band)));
+    if (check_launch("integrate: k_march_order")) return 1;
+  }
   // predicted velocity of each stage: with the fold, stage 1 writes the
   // second pair while it reads the first
   double* const us[3] = {g_scr.ux, fold ? g_scr.ux2 : g_scr.ux, g_scr.ux};
@@ -847,8 +896,11 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
         }
         m.kappa = g_scr.kappa;
       }
-      const dim3 grid(cdiv(nx, kStrip), cdiv(ny, band)), block(64);
-      const bool mixed = options().precision == SBFS_PRECISION_MIXED;
+      if (ordered) {
+        m.order = g_scr.order;
+        m.weight = g_scr.weight;
+      }
+      const dim3 grid = ordered ? dim3(units) : dim3(cdiv(nx, kStrip), cdiv(ny, band)), block(64);
       if (mixed && corr)
         SBFS_TIMED("k_march_mixed_corr", hipLaunchKernelGGL(k_march_mixed_corr, grid, block, 0, stream(), m));
       else if (mixed)
@@ -1040,6 +1092,9 @@ int integrate_fused(const domain_t* domain, fluid_t* fluid, interface_t* interfa
   if (!use_tiles && poisson_transform_fused(domain, c, fluid, &rhs)) return 1;
   const int band = march_band(nx, ny, options().precision == SBFS_PRECISION_MIXED);
   if (band <= 0) return 1;
+  if (!use_tiles && march_ordered(options().precision == SBFS_PRECISION_MIXED) &&
+      order_buffers(march_units(nx, ny, band)))
+    return 1;
   if (load_dt_coefs()) return 1;
   double learned_h = 0.;
   if (options().curvature == SBFS_CURVATURE_LEARNED) {

@@ -337,6 +337,7 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
     if constexpr (CORR) asm volatile("" : "+v"(npo));
   };
   fence();
+  unsigned weight = 0;  // marched rows with an interface cell
   for (int s = s_first; s <= je + 1; s++) {
     // -- new input rows -----------------------------------------------------
     rvm1 = R(v0); v0 = vp1; vp1 = vp2; vp2 = nv;
@@ -351,6 +352,7 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
       psm1 = ps0; ps0 = nps;
     }
     const bool outrow = s >= jb && s <= je;
+    if (a.weight) weight += __ballot(c_vof && in_band(v0)) != 0ull ? 1u : 0u;
     const float kap = nkap;
     const double sbv = nsbv, sby = nsby, sbx = nsbx;
     nv = ld_vof(w3);
@@ -703,6 +705,55 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
       if (c_in) np = np + psp1;
     }
   }
+  if (a.weight && lane == 0) a.weight[by * march_strips(nx) + bx] = weight;
+}
+
+__global__ __launch_bounds__(1024) void k_march_order(const unsigned* __restrict__ weight,
+                                                      unsigned* __restrict__ order, int n, int max_weight) {
+  constexpr int kB = 32, kPer = 8, kT = 1024;
+  __shared__ unsigned cnt[kB], next[kB];
+  const int t = threadIdx.x;
+  const unsigned mw = max_weight > 0 ? (unsigned)max_weight : 1u;
+  // the buckets of kPer units per thread, their loads issued together
+  // (kB: no unit)
+  auto buckets = [&](int base, int* b) {
+    unsigned w[kPer];
+#pragma unroll
+    for (int i = 0; i < kPer; i++) {
+      const int u = base + i * kT + t;
+      w[i] = u < n ? weight[u] : 0u;
+    }
+#pragma unroll
+    for (int i = 0; i < kPer; i++) {
+      const int u = base + i * kT + t;
+      b[i] = u < n ? (int)((unsigned long long)min(w[i], mw) * (kB - 1) / mw) : kB;
+    }
+  };
+  if (t < kB) cnt[t] = 0;
+  __syncthreads();
+  for (int base = 0; base < n; base += kPer * kT) {
+    int b[kPer];
+    buckets(base, b);
+#pragma unroll
+    for (int i = 0; i < kPer; i++)
+      if (b[i] < kB) atomicAdd(&cnt[b[i]], 1u);
+  }
+  __syncthreads();
+  if (t == 0) {
+    unsigned s = 0;
+    for (int b = kB - 1; b >= 0; b--) {
+      next[b] = s;
+      s += cnt[b];
+    }
+  }
+  __syncthreads();
+  for (int base = 0; base < n; base += kPer * kT) {
+    int b[kPer];
+    buckets(base, b);
+#pragma unroll
+    for (int i = 0; i < kPer; i++)
+      if (b[i] < kB) order[atomicAdd(&next[b[i]], 1u)] = (unsigned)(base + i * kT + t);
+  }
 }
 
 // The FP64 bodies need about 230-250 VGPRs, two wavefronts per SIMD (three
@@ -712,7 +763,15 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
 #define SBFS_MARCH_KERNEL(name, R, CORR, WAVES)                                            \
   __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES))) void name( \
       MarchArgs a) {                                                                      \
-    march_body<R, CORR>(a, blockIdx.x, blockIdx.y);                                       \
+    int bx = blockIdx.x, by = blockIdx.y;                                                 \
+    if (a.order) {                                                                        \
+      const unsigned u = a.order[blockIdx.x];                                             \
+      if (u >= (unsigned)march_units(a.nx, a.ny, a.band)) return;                         \
+      const int strips = march_strips(a.nx);                                              \
+      bx = (int)u % strips;                                                               \
+      by = (int)u / strips;                                                               \
+    }                                                                                     \
+    march_body<R, CORR>(a, bx, by);                                                       \
   }                                                                                       \
   __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES))) void name##_batch( \
       const MarchArgs* __restrict__ tab, const uint2* __restrict__ map) {                \

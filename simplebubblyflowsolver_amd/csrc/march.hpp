@@ -73,9 +73,26 @@ struct MarchArgs {
   // and the kernel skips the kappa reads)
   const float* kappa;
   const unsigned* kappa_n;
+  // dispatch order of the (strip, band) units, unit = band * strips + strip
+  // (null: the 2D grid's own order), and each unit's count of marched rows
+  // with interface cells, written for the next order (null: not counted)
+  const unsigned* order;
+  unsigned* weight;
   int stage;
   int nx, ny, band;
 };
+
+// units of a stage kernel launch, and the largest weight one can have
+__host__ __device__ inline int march_strips(int nx) { return (nx + kStrip - 1) / kStrip; }
+__host__ __device__ inline int march_units(int nx, int ny, int band) {
+  return march_strips(nx) * ((ny + band - 1) / band);
+}
+__host__ __device__ inline int march_max_weight(int band) { return band + kWarmCorr + 2; }
+// Orders the units by weight, heaviest first (a counting sort over 32
+// weight buckets, one workgroup of 1024 threads): the dispatcher starts the
+// units with the most interface rows first and fills in behind them.  Any
+// weights give a permutation of 0..n-1.
+__global__ void k_march_order(const unsigned* weight, unsigned* order, int n, int max_weight);
 
 // projection of u and the pressure update after the last stage (k_post)
 struct PostArgs {
