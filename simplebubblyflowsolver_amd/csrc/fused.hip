@@ -1000,6 +1000,8 @@ static StepKey step_key(const DomainCtx* c, fluid_t* f, interface_t* s, const Co
                 ((uint64_t)options().chunk << 16) | ((uint64_t)options().precision << 48) |
                 ((uint64_t)options().curvature << 56));
   key_add(k, g_scr.kappa);
+  k.v.push_back(march_ordered(options().precision == SBFS_PRECISION_MIXED) ? 1 : 0);
+  key_add(k, g_scr.order);
   k.v.push_back(graph_epoch());
   return k;
 }
