@@ -729,14 +729,33 @@ __global__ __launch_bounds__(1024) void k_march_order(const unsigned* __restrict
       b[i] = u < n ? (int)((unsigned long long)min(w[i], mw) * (kB - 1) / mw) : kB;
     }
   };
+  // Adds the wavefront's lanes with a unit to their buckets of h and gives
+  // each such lane a distinct slot: one atomic per distinct bucket in the
+  // wavefront (most units share the bucket of weight 0, and lane-by-lane
+  // atomics on one LDS word serialise).  Called in uniform control flow.
+  const int lane = t & 63;
+  auto add = [&](unsigned* h, int b) -> unsigned {
+    unsigned slot = 0;
+    unsigned long long left = __ballot(b < kB);
+    while (left) {
+      const int lead = __ffsll((long long)left) - 1;
+      const int bl = __shfl(b, lead);
+      const unsigned long long m = __ballot(b == bl);
+      unsigned first = 0;
+      if (lane == lead) first = atomicAdd(&h[bl], (unsigned)__popcll(m));
+      first = __shfl(first, lead);
+      if ((m >> lane) & 1ull) slot = first + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+      left &= ~m;
+    }
+    return slot;
+  };
   if (t < kB) cnt[t] = 0;
   __syncthreads();
   for (int base = 0; base < n; base += kPer * kT) {
     int b[kPer];
     buckets(base, b);
 #pragma unroll
-    for (int i = 0; i < kPer; i++)
-      if (b[i] < kB) atomicAdd(&cnt[b[i]], 1u);
+    for (int i = 0; i < kPer; i++) add(cnt, b[i]);
   }
   __syncthreads();
   if (t == 0) {
@@ -751,8 +770,10 @@ __global__ __launch_bounds__(1024) void k_march_order(const unsigned* __restrict
     int b[kPer];
     buckets(base, b);
 #pragma unroll
-    for (int i = 0; i < kPer; i++)
-      if (b[i] < kB) order[atomicAdd(&next[b[i]], 1u)] = (unsigned)(base + i * kT + t);
+    for (int i = 0; i < kPer; i++) {
+      const unsigned slot = add(next, b[i]);
+      if (b[i] < kB) order[slot] = (unsigned)(base + i * kT + t);
+    }
   }
 }
 
