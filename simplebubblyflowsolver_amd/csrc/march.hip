@@ -465,7 +465,7 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
       fyp1 = f;
       mfyp1 = mf;
     }
-    // -- viscosities and stress tensor (viscosity.c, predict/txx.c, tyy.c, txy.c)
+    // -- viscosities (viscosity.c) ------------------------------------------------
     // (every DPP shift runs in uniform control flow: an inactive source lane
     // would not deliver its value)
     const double v0l = from_left(v0);
@@ -476,28 +476,6 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
     visx1 = VISUX(c, rvp1l, rvp1);   // x face c, row s+1
     visy0 = visy1;
     visy1 = VISUY(rv0, rvp1);        // y face s+1
-    const R visx0r = A::right(visx0), visy1l = A::left(visy1);
-    R txx0 = 0;
-    if (c_in) {
-      const R vx = R(0.5) * visx0 + R(0.5) * visx0r;
-      const R l = SB(T_RHC) * (-rux0 + rux0r);
-      txx0 = vx * l + vx * l;
-    }
-    R tyy0 = 0;
-    if (c_in) {
-      const R vy = R(0.5) * visy0 + R(0.5) * visy1;
-      const R l = rhyr * (-ruy0 + ruyp1);
-      tyy0 = vy * l + vy * l;
-    }
-    const R txy_prev = txy0;   // corner row s
-    R txy1 = 0;
-    if (c_fx) {
-      const R vx = R(0.5) * visx0 + R(0.5) * visx1;
-      const R vy = R(0.5) * visy1l + R(0.5) * visy1;
-      const R lxy = rhyr * (-rux0 + ruxp1);
-      const R lyx = SB(T_RHF) * (-ruyp1l + ruyp1);
-      txy1 = vx * lxy + vy * lyx;
-    }
     // -- new vof, row s (update/main.c:54-120), FP64 ---------------------------
     double src = 0.;
     const double fx0d = double(fx0);
@@ -522,121 +500,162 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
     const double dfy = 0.5 * d_m1 + 0.5 * d_0;    // y face s, old vof
     const double d1x = 0.5 * dn0l + 0.5 * dn0;    // x face c, row s, new vof
     d1y = 0.5 * ring_ld(RG_DNM1, dnm1) + 0.5 * dn0;  // y face s, new vof
-    // -- momentum fluxes shared between neighbouring faces ---------------------
-    const R mfx0r = A::right(mfx0), mfyp1l = A::left(mfyp1);
-    const R ruy0l = A::left(ruy0);
-    // ux, x direction: cell-centre flux (mp * up) of the cell right of face c
-    const R qx = (SB(T_EQF) * mfx0 + SR(T_EQF) * mfx0r) * (R(0.5) * rux0 + R(0.5) * rux0r);
-    // uy, x direction: corner flux (mm * um) at face c
-    const R qm = (SB(T_EQF) * mfxm1 + SB(T_EQF) * mfx0) * (R(0.5) * ruy0l + R(0.5) * ruy0);
-    const R qxl = A::left(qx), qmr = A::right(qm);
-    const R qt = SB(T_QC) * txx0;
-    const R qtl = A::left(qt);
-    const R qf_txy0 = SB(T_QF) * txy_prev;
-    const R qf_txy0r = A::right(qf_txy0);
-    const double p0l = from_left(p0);
-    const R k0l = A::left(k0);
-    // ux, y direction: corner flux (mp * up) at face s+1
-    const R qy_ux_new = (SL(T_ECY) * mfyp1l + SB(T_ECY) * mfyp1) * (R(0.5) * rux0 + R(0.5) * ruxp1);
-    // uy, y direction: cell-centre flux (mp * up) of row s
-    const R qy_uy_new = (SB(T_ECY) * mfy0 + SB(T_ECY) * mfyp1) * (R(0.5) * ruy0 + R(0.5) * ruyp1);
-    const R cyf_txy1 = SB(T_CYF) * txy1;
-    const R cyc_tyy0 = SB(T_CYC) * tyy0;
+    // Rows before jb - 1 only fill the interface and density rings: their
+    // stresses, momentum fluxes, predicted velocity and Poisson rhs reach no
+    // output, since every ring that row jb's predicted velocity reads is
+    // refilled by row jb - 1, and the rhs of row jb is formed at step jb + 1
+    // from the rings of step jb.  So the stresses and momentum fluxes start
+    // at row jb - 1, the predicted velocity and the rhs at row jb.  (The
+    // flags are uniform: the DPP shifts inside stay in uniform control flow.)
+    // Measured on the 2048 x 4096 swarm: the folded body gains (k_march_corr
+    // 452 -> 414 us), the plain one loses (k_march 312 -> 361 us, its code
+    // laid out worse), so only the folded body skips.
+    constexpr bool kSkip = CORR;
+    const bool mom = !kSkip || s >= jb - 1, pred = !kSkip || s >= jb;
+    const R txy_prev = txy0;   // corner row s
+    R txx0 = 0, tyy0 = 0, txy1 = 0;
+    R qx = 0, qm = 0, qxl = 0, qmr = 0, qt = 0, qtl = 0, qf_txy0 = 0, qf_txy0r = 0;
+    R qy_ux_new = 0, qy_uy_new = 0, cyf_txy1 = 0, cyc_tyy0 = 0;
+    if (mom) {
+      // -- stress tensor (predict/txx.c, tyy.c, txy.c) --------------------------
+      const R visx0r = A::right(visx0), visy1l = A::left(visy1);
+      if (c_in) {
+        const R vx = R(0.5) * visx0 + R(0.5) * visx0r;
+        const R l = SB(T_RHC) * (-rux0 + rux0r);
+        txx0 = vx * l + vx * l;
+      }
+      if (c_in) {
+        const R vy = R(0.5) * visy0 + R(0.5) * visy1;
+        const R l = rhyr * (-ruy0 + ruyp1);
+        tyy0 = vy * l + vy * l;
+      }
+      if (c_fx) {
+        const R vx = R(0.5) * visx0 + R(0.5) * visx1;
+        const R vy = R(0.5) * visy1l + R(0.5) * visy1;
+        const R lxy = rhyr * (-rux0 + ruxp1);
+        const R lyx = SB(T_RHF) * (-ruyp1l + ruyp1);
+        txy1 = vx * lxy + vy * lyx;
+      }
+      // -- momentum fluxes shared between neighbouring faces -------------------
+      const R mfx0r = A::right(mfx0), mfyp1l = A::left(mfyp1);
+      const R ruy0l = A::left(ruy0);
+      // ux, x direction: cell-centre flux (mp * up) of the cell right of face c
+      qx = (SB(T_EQF) * mfx0 + SR(T_EQF) * mfx0r) * (R(0.5) * rux0 + R(0.5) * rux0r);
+      // uy, x direction: corner flux (mm * um) at face c
+      qm = (SB(T_EQF) * mfxm1 + SB(T_EQF) * mfx0) * (R(0.5) * ruy0l + R(0.5) * ruy0);
+      qxl = A::left(qx);
+      qmr = A::right(qm);
+      qt = SB(T_QC) * txx0;
+      qtl = A::left(qt);
+      qf_txy0 = SB(T_QF) * txy_prev;
+      qf_txy0r = A::right(qf_txy0);
+      // ux, y direction: corner flux (mp * up) at face s+1
+      qy_ux_new = (SL(T_ECY) * mfyp1l + SB(T_ECY) * mfyp1) * (R(0.5) * rux0 + R(0.5) * ruxp1);
+      // uy, y direction: cell-centre flux (mp * up) of row s
+      qy_uy_new = (SB(T_ECY) * mfy0 + SB(T_ECY) * mfyp1) * (R(0.5) * ruy0 + R(0.5) * ruyp1);
+      cyf_txy1 = SB(T_CYF) * txy1;
+      cyc_tyy0 = SB(T_CYC) * tyy0;
+    }
     // The fence: an empty statement that reads and redefines the next step's
     // inputs, so the compiler waits for their loads here, ahead of this
     // step's first store, rather than where the loop carries them over.
     fence();
-    // -- predicted ux, row s (predict/ux.c, force.c) ---------------------------
-    double uxs = 0.;
-    if (c >= 2 && c <= nx) {
-      R s_ = 0;
-      s_ -= SB(T_RJF) * (-qxl + qx);
-      s_ -= SB(T_RJF) * (-qy_ux + qy_ux_new);
-      s_ += SB(T_DFX) * (-qtl + qt);
-      s_ += SB(T_DFX) * (-cyf_txy0 + cyf_txy1);
-      double g = 0.;
-      g -= TB(T_RHF) * (-p0l + p0);
-      const R rdfx = R(dfx);
-      {
-        const R k = R(0.5) * k0l + R(0.5) * k0;
-        const R delta = SB(T_RHF) * (-rv0l + rv0);
-        s_ += rdfx * refdeninv * tension * k * delta;
+    double uxs = 0., uys = 0.;
+    if (mom) {
+      if (pred) {
+        const double p0l = from_left(p0);
+        const R k0l = A::left(k0);
+        // -- predicted ux, row s (predict/ux.c, force.c) -----------------------
+        if (c >= 2 && c <= nx) {
+          R s_ = 0;
+          s_ -= SB(T_RJF) * (-qxl + qx);
+          s_ -= SB(T_RJF) * (-qy_ux + qy_ux_new);
+          s_ += SB(T_DFX) * (-qtl + qt);
+          s_ += SB(T_DFX) * (-cyf_txy0 + cyf_txy1);
+          double g = 0.;
+          g -= TB(T_RHF) * (-p0l + p0);
+          const R rdfx = R(dfx);
+          {
+            const R k = R(0.5) * k0l + R(0.5) * k0;
+            const R delta = SB(T_RHF) * (-rv0l + rv0);
+            s_ += rdfx * refdeninv * tension * k * delta;
+          }
+          s_ += grav * rdfx;
+          const double sd = double(s_);
+          if (a.srca_ux && outrow && c_out) AT(fx_row(a.srca_ux, s)) = sd;
+          uxs = dfx * ux0 + dtca * sd + dtcb * sbx + dtcg * g;
+          uxs /= d1x;
+        }
+        // -- predicted uy, row s (predict/uy.c, force.c) -----------------------
+        if (c_in) {
+          R s_ = 0;
+          s_ -= SB(T_RJC) * (-qm + qmr);
+          s_ -= SB(T_RJC) * (-qy_uy + qy_uy_new);
+          s_ += SB(T_DCX) * (-qf_txy0 + qf_txy0r);
+          s_ += SB(T_DCX) * (-cyc_tyym1 + cyc_tyy0);
+          double g = 0.;
+          g -= rhy * (-ring_ld(RG_PM1, pm1) + p0);
+          {
+            const R k = R(0.5) * km1 + R(0.5) * k0;
+            const R delta = rhyr * (-rvm1 + rv0);
+            s_ += R(dfy) * refdeninv * tension * k * delta;
+          }
+          const double sd = double(s_);
+          if (a.srca_uy && outrow && c_out) AT(ci_row(a.srca_uy, s)) = sd;
+          uys = dfy * uy0 + dtca * sd + dtcb * sby + dtcg * g;
+          uys /= d1y;
+        }
       }
-      s_ += grav * rdfx;
-      const double sd = double(s_);
-      if (a.srca_ux && outrow && c_out) AT(fx_row(a.srca_ux, s)) = sd;
-      uxs = dfx * ux0 + dtca * sd + dtcb * sbx + dtcg * g;
-      uxs /= d1x;
-    }
-    // -- predicted uy, row s (predict/uy.c, force.c) ---------------------------
-    double uys = 0.;
-    if (c_in) {
-      R s_ = 0;
-      s_ -= SB(T_RJC) * (-qm + qmr);
-      s_ -= SB(T_RJC) * (-qy_uy + qy_uy_new);
-      s_ += SB(T_DCX) * (-qf_txy0 + qf_txy0r);
-      s_ += SB(T_DCX) * (-cyc_tyym1 + cyc_tyy0);
-      double g = 0.;
-      g -= rhy * (-ring_ld(RG_PM1, pm1) + p0);
-      {
-        const R k = R(0.5) * km1 + R(0.5) * k0;
-        const R delta = rhyr * (-rvm1 + rv0);
-        s_ += R(dfy) * refdeninv * tension * k * delta;
-      }
-      const double sd = double(s_);
-      if (a.srca_uy && outrow && c_out) AT(ci_row(a.srca_uy, s)) = sd;
-      uys = dfy * uy0 + dtca * sd + dtcb * sby + dtcg * g;
-      uys /= d1y;
-    }
-    qy_ux = qy_ux_new;
-    qy_uy = qy_uy_new;
-    cyf_txy0 = cyf_txy1;
-    cyc_tyym1 = cyc_tyy0;
-    txy0 = txy1;
-    // -- outputs of row s -------------------------------------------------------
-    const double txy0r = double(A::right(txy_prev));
-    if (outrow && c_out) {
-      auto put = [&](double* f, int nh, auto row, double v) {
-        AT(row(f, s)) = v;
-        if (s > ny - nh) AT(row(f, s - ny)) = v;
-        if (s <= nh) AT(row(f, s + ny)) = v;
-      };
-      auto put_at = [&](double* f, int nh, auto row, int off, double v) {
-        AT(row(f, s) + off) = v;
-        if (s > ny - nh) AT(row(f, s - ny) + off) = v;
-        if (s <= nh) AT(row(f, s + ny) + off) = v;
-      };
-      auto rv = [&](double* f, int j) { return vof_row(f, j); };
-      auto rc = [&](double* f, int j) { return cc_row(f, j); };
-      auto rx = [&](double* f, int j) { return xf_row(f, j); };
-      put(a.vofn, 2, rv, vn0);
-      if (c == 1) put_at(a.vofn, 2, rv, -1, 0.);
-      if (c == nx) put_at(a.vofn, 2, rv, 1, 0.);
-      if (a.srca_vof) AT(ci_row(a.srca_vof, s)) = src;
-      put(a.uxs, 1, rx, c == 1 ? 0. : uxs);
-      if (c == nx) put_at(a.uxs, 1, rx, 1, 0.);
-      put(a.uys, 1, rc, uys);
-      if (c == 1) put_at(a.uys, 1, rc, -1, kUyXm);
-      if (c == nx) put_at(a.uys, 1, rc, 1, kUyXp);
-      if (a.den1) {
-        const double dg = DEN(0.);
-        put(a.den1, 1, rc, dn0);
-        if (c == 1) put_at(a.den1, 1, rc, -1, dg);
-        if (c == nx) put_at(a.den1, 1, rc, 1, dg);
-      }
-      if (a.txy) {
-        put(a.txy, 1, rx, double(txy_prev));
-        if (c == nx) put_at(a.txy, 1, rx, 1, txy0r);
-      }
-      if (CORR) {
-        // the previous stage's updated pressure, with its Neumann ghosts
-        put(a.p_out, 1, rc, p0);
-        if (c == 1) put_at(a.p_out, 1, rc, -1, p0);
-        if (c == nx) put_at(a.p_out, 1, rc, 1, p0);
+      qy_ux = qy_ux_new;
+      qy_uy = qy_uy_new;
+      cyf_txy0 = cyf_txy1;
+      cyc_tyym1 = cyc_tyy0;
+      txy0 = txy1;
+      // -- outputs of row s -----------------------------------------------------
+      const double txy0r = double(A::right(txy_prev));
+      if (outrow && c_out) {
+        auto put = [&](double* f, int nh, auto row, double v) {
+          AT(row(f, s)) = v;
+          if (s > ny - nh) AT(row(f, s - ny)) = v;
+          if (s <= nh) AT(row(f, s + ny)) = v;
+        };
+        auto put_at = [&](double* f, int nh, auto row, int off, double v) {
+          AT(row(f, s) + off) = v;
+          if (s > ny - nh) AT(row(f, s - ny) + off) = v;
+          if (s <= nh) AT(row(f, s + ny) + off) = v;
+        };
+        auto rv = [&](double* f, int j) { return vof_row(f, j); };
+        auto rc = [&](double* f, int j) { return cc_row(f, j); };
+        auto rx = [&](double* f, int j) { return xf_row(f, j); };
+        put(a.vofn, 2, rv, vn0);
+        if (c == 1) put_at(a.vofn, 2, rv, -1, 0.);
+        if (c == nx) put_at(a.vofn, 2, rv, 1, 0.);
+        if (a.srca_vof) AT(ci_row(a.srca_vof, s)) = src;
+        put(a.uxs, 1, rx, c == 1 ? 0. : uxs);
+        if (c == nx) put_at(a.uxs, 1, rx, 1, 0.);
+        put(a.uys, 1, rc, uys);
+        if (c == 1) put_at(a.uys, 1, rc, -1, kUyXm);
+        if (c == nx) put_at(a.uys, 1, rc, 1, kUyXp);
+        if (a.den1) {
+          const double dg = DEN(0.);
+          put(a.den1, 1, rc, dn0);
+          if (c == 1) put_at(a.den1, 1, rc, -1, dg);
+          if (c == nx) put_at(a.den1, 1, rc, 1, dg);
+        }
+        if (a.txy) {
+          put(a.txy, 1, rx, double(txy_prev));
+          if (c == nx) put_at(a.txy, 1, rx, 1, txy0r);
+        }
+        if (CORR) {
+          // the previous stage's updated pressure, with its Neumann ghosts
+          put(a.p_out, 1, rc, p0);
+          if (c == 1) put_at(a.p_out, 1, rc, -1, p0);
+          if (c == nx) put_at(a.p_out, 1, rc, 1, p0);
+        }
       }
     }
     // -- Poisson right-hand side of row s-1 (compute_potential.c:315-360), FP64
-    {
+    if (pred) {
       // u* divergence: the x-face terms of cell c+1 come from lane c+1
       const double q_uxs = ring_ld(RG_QUXS, q_uxs_m1);     // QF * ux* at face c, row s-1
       const double q_uxsr = from_right(q_uxs);
