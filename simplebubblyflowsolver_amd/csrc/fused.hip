@@ -663,18 +663,21 @@ static bool march_ordered(bool mixed) {
 
 // Rows per band of k_march.  Every wavefront marches its band plus kWarm + 2
 // rows of ring fill, and a wavefront lives for the whole band, so the grid
-// is sized to whole rounds of the device: a multiple of the resident
-// wavefront slots per strip (occupancy x CUs), each band as short as that
-// allows.  A partial last round would leave most SIMDs idle while it runs.
+// is sized in rounds of the device: multiples of the resident wavefront
+// slots per strip (occupancy x CUs), each band as short as that allows.  In
+// the grid's own order a partial last round would leave most SIMDs idle
+// while it runs, so the rounds are whole.
 // The FP64 bodies take about twice as long over a row with interface cells
 // (the THINC flux's tanh and log), so in a single round the bands a bubble
 // crosses finish last, 1.5x after the mean wavefront.  Several rounds let
-// the dispatcher balance them, for more ring fill: two rounds when the units
-// run heaviest first (k_march_order), three in the grid's own order
-// (measured on the 2048 x 4096 swarm: k_march 397 / 363 / 305 us and
-// k_march_corr 522 / 510 / 444 us for one round, three unordered and two
-// ordered).  The FP32 bodies' cheaper fluxes leave less to balance, and one
-// round is fastest for them.  SBFS_MARCH_BAND overrides the choice.
+// the dispatcher balance them, for more ring fill: 1.75 rounds when the
+// units run heaviest first (k_march_order; the partial last round holds the
+// lightest units), three in the grid's own order.  Measured on the 2048 x
+// 4096 swarm, k_march / k_march_corr: 397 / 522 us for one round, 363 / 510
+// for three unordered, 304 / 404 for two ordered and 289 / 398 for 1.75
+// ordered (band 41; 2.4 and 1.5 rounds were slower).  The FP32 bodies'
+// cheaper fluxes leave less to balance, and one round is fastest for them.
+// SBFS_MARCH_BAND overrides the choice.
 static int march_band(int nx, int ny, bool mixed) {
   static long slots[2] = {0, 0};
   long& sl = slots[mixed ? 1 : 0];
@@ -693,8 +696,9 @@ static int march_band(int nx, int ny, bool mixed) {
     if (b >= 1) return b < ny ? b : ny;
   }
   const long strips = (nx + kStrip - 1) / kStrip;
-  const long rounds = mixed ? 1 : (march_ordered(mixed) ? 2 : 3);
-  long bands = rounds * slots_now / strips;
+  // rounds of the device, in quarters
+  const long quarters = mixed ? 4 : (march_ordered(mixed) ? 7 : 12);
+  long bands = quarters * slots_now / (4 * strips);
   if (bands < 1) bands = 1;
   long band = (ny + bands - 1) / bands;
   if (band < 4) band = 4;
