@@ -507,11 +507,12 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
     // from the rings of step jb.  So the stresses and momentum fluxes start
     // at row jb - 1, the predicted velocity and the rhs at row jb.  (The
     // flags are uniform: the DPP shifts inside stay in uniform control flow.)
-    // Measured on the 2048 x 4096 swarm: the folded body gains (k_march_corr
-    // 452 -> 414 us), the plain one loses (k_march 312 -> 361 us, its code
-    // laid out worse), so only the folded body skips.
-    constexpr bool kSkip = CORR;
-    const bool mom = !kSkip || s >= jb - 1, pred = !kSkip || s >= jb;
+    // Measured on the 2048 x 4096 swarm: the folded body gains from both
+    // (k_march_corr 452 -> 414 us); the plain one gains from the later start
+    // of the predicted velocity and the rhs (k_march 296 -> 288 us) but loses
+    // with the stresses and fluxes skipped too (361 us, its code laid out
+    // worse), so it computes those on every row.
+    const bool mom = !CORR || s >= jb - 1, pred = s >= jb;
     const R txy_prev = txy0;   // corner row s
     R txx0 = 0, tyy0 = 0, txy1 = 0;
     R qx = 0, qm = 0, qxl = 0, qmr = 0, qt = 0, qtl = 0, qf_txy0 = 0, qf_txy0r = 0;
