@@ -336,6 +336,11 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
                  "+v"(nsbx));
     if constexpr (CORR) asm volatile("" : "+v"(npo));
   };
+  // the vof ring holds rows s_first and s_first + 1 before the first step,
+  // so the first step's corner gradients are already valid
+  vp1 = ld_vof(wrap_r(s_first, ny));
+  vp2 = ld_vof(w1);
+  asm volatile("" : "+v"(vp1), "+v"(vp2));
   fence();
   unsigned weight = 0;  // marched rows with an interface cell
   for (int s = s_first; s <= je + 1; s++) {

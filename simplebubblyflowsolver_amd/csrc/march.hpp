@@ -11,17 +11,17 @@ namespace sbfs {
 // predicted ux of the face to its right, whose mass flux upwinds into the
 // next cell, whose normal needs the corner gradient one column further.
 constexpr int kStrip = 58;
-// Rows marched before the first output row jb of a band.  The vof ring holds
-// rows s+1 and s+2 only from the second marched row on, so the first valid
-// corner gradients are those of row s_first + 3, and the first valid normals
-// those of row s_first + 3.  The y flux into row jb - 1 needs the normals of
-// row jb - 2, hence s_first <= jb - 5.  With the folded projection, ux is
-// valid from row s_first + 2 and uy from face s_first + 3, which the first
-// output row needs from face jb - 1 on: s_first <= jb - 4, so the interface
-// chain decides both.  (With 4 rows the fused path no longer matches the
-// staged one bitwise.)
-constexpr int kWarm = 5;
-constexpr int kWarmCorr = 5;
+// Rows marched before the first output row jb of a band.  The vof ring
+// holds rows s_first .. s_first + 2 from the first step on (two rows are
+// loaded before the loop), so the first valid corner gradients are those of
+// row s_first + 2 and the first valid normals those of row s_first + 2.  The
+// y flux into row jb - 1 needs the normals of row jb - 2, hence s_first <=
+// jb - 4.  With the folded projection, ux and uy are valid from row and face
+// s_first + 2, which the first output row needs from jb - 1 on.  (Without
+// the two rows loaded ahead, 4 rows of fill broke the fused path's bitwise
+// match with the staged one.)
+constexpr int kWarm = 4;
+constexpr int kWarmCorr = 4;
 
 // per-column quotients of the metrics (see fused.hip)
 struct Coef {
