@@ -24,31 +24,84 @@ GAUGE_PINNED, GAUGE_REFERENCE = 0, 1
 _lib = None
 
 
+# the flags of oracle/Makefile; -ffp-contract=off keeps every a*b+c as two
+# rounded operations, as the reference's -std=c99 build does
+CFLAGS = ["-std=c99", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-Wall", "-Wextra"]
+
+
 def build(force: bool = False) -> Path:
     if force or not LIB.exists():
         subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
     return LIB
 
 
+def compile_library(dest: Path, source: str | None = None, opt=("-O2",)) -> Path:
+    """gcc the oracle into `dest`: from oracle.c, or from `source`, the
+    text of a modified copy (the mutation check plants mistakes this way).
+    `opt` replaces the Makefile's -O2 (the CPU baseline builds -O3
+    -march=native on the machine that times it)."""
+    dest = Path(dest)
+    src = HERE / "oracle.c"
+    if source is not None:
+        src = dest.with_suffix(".c")
+        src.write_text(source)
+    cmd = ["gcc"] + CFLAGS + list(opt) + [f"-I{HERE}", f"-I{HERE.parent / 'include'}", "-shared",
+                                          "-o", str(dest), str(src), "-lm"]
+    subprocess.run(cmd, check=True, capture_output=True)
+    return dest
+
+
+def _bind(lib: C.CDLL) -> C.CDLL:
+    lib.oracle_set_poisson.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+    lib.oracle_set_threads.argtypes = [C.c_int]
+    lib.oracle_dct2.argtypes = [C.c_int, C.c_size_t, C.c_size_t, C.c_void_p, C.c_void_p]
+    lib.oracle_dct3.argtypes = [C.c_int, C.c_size_t, C.c_size_t, C.c_void_p, C.c_void_p]
+    lib.oracle_fft.argtypes = [C.c_size_t, C.c_void_p, C.c_void_p]
+    lib.oracle_tdm_periodic.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_int]
+    lib.oracle_tdm_partitioned.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_void_p]
+    lib.oracle_poisson_evals.argtypes = [C.c_void_p, C.c_void_p]
+    lib.oracle_rkcoefs.argtypes = [C.c_void_p]
+    return lib
+
+
 def lib() -> C.CDLL:
     global _lib
     if _lib is None:
         build()
-        _lib = C.CDLL(str(LIB))
-        _lib.oracle_set_poisson.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
-        _lib.oracle_set_threads.argtypes = [C.c_int]
-        _lib.oracle_dct2.argtypes = [C.c_int, C.c_size_t, C.c_size_t, C.c_void_p, C.c_void_p]
-        _lib.oracle_dct3.argtypes = [C.c_int, C.c_size_t, C.c_size_t, C.c_void_p, C.c_void_p]
-        _lib.oracle_fft.argtypes = [C.c_size_t, C.c_void_p, C.c_void_p]
-        _lib.oracle_tdm_periodic.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_int]
-        _lib.oracle_tdm_partitioned.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_void_p]
-        _lib.oracle_poisson_evals.argtypes = [C.c_void_p, C.c_void_p]
-        _lib.oracle_rkcoefs.argtypes = [C.c_void_p]
+        _lib = _bind(C.CDLL(str(LIB)))
     return _lib
 
 
 def backend() -> solver.Backend:
     return solver.Backend(lib(), "oracle_", "oracle_", False)
+
+
+class Build:
+    """An oracle library built separately (compile_library), with the
+    module's case API: `new_case`, `set_poisson`, `set_threads`."""
+
+    DCT_DIRECT, DCT_FFT = DCT_DIRECT, DCT_FFT
+    TDM_SEQUENTIAL, TDM_PARTITIONED = TDM_SEQUENTIAL, TDM_PARTITIONED
+    GAUGE_PINNED, GAUGE_REFERENCE = GAUGE_PINNED, GAUGE_REFERENCE
+
+    def __init__(self, path: Path):
+        self.lib = _bind(C.CDLL(str(path)))
+
+    def backend(self) -> solver.Backend:
+        return solver.Backend(self.lib, "oracle_", "oracle_", False)
+
+    def set_poisson(self, dct: int = DCT_FFT, tdm: int = TDM_SEQUENTIAL, chunk: int = 32,
+                    gauge: int = GAUGE_PINNED) -> None:
+        self.lib.oracle_set_poisson(dct, tdm, chunk, gauge)
+
+    def set_threads(self, n: int) -> None:
+        self.lib.oracle_set_threads(n)
+
+    def new_case(self, nx, ny, lx=1.0, ly=2.0, Re=35.0, Fr=1.0, denr=1e-3, visr=1e-2, We=10.0,
+                 dt_coefs=(0.2, 0.5, 0.95)) -> solver.Case:
+        be = self.backend()
+        be.set_dt_coefs(*dt_coefs)
+        return solver.Case(be, nx, ny, lx, ly, Re, Fr, denr, visr, We)
 
 
 def set_poisson(dct: int = DCT_FFT, tdm: int = TDM_SEQUENTIAL, chunk: int = 32,
