@@ -90,7 +90,7 @@ def check_ux_advection_in_x(om):
     ux at face i is -(a^2 / h)(xc_i^2 - xc_{i-1}^2) = -2 a^2 xf_i exactly;
     stress divergence vanishes for a linear field and every other source is
     off."""
-    nx, ny, a = 32, 8, 0.7
+    nx, ny, a = 32, 32, 0.7
     c = _case(om, nx, ny)
     try:
         xf, xc, hy, h = _grid(c)
@@ -141,7 +141,7 @@ def check_uy_advection_and_diffusion_in_x(om):
     -U B cos(pi xc) sin(pi h) / h and its x diffusion
     (1/Re) B sin(pi xc) 2 (cos(pi h) - 1) / h^2 for cells away from the
     walls."""
-    nx, ny, B, U, Re = 32, 8, 0.5, -0.25, 15.0
+    nx, ny, B, U, Re = 32, 32, 0.5, -0.25, 15.0
     c = _case(om, nx, ny, Re=Re)
     try:
         xf, xc, hy, h = _grid(c)
@@ -170,7 +170,7 @@ def check_uy_advection_and_diffusion_in_y(om):
     -(up^2 - um^2) / hy, i.e.
     -A cos(k yf) sin(k hy) (2 V + A sin(k yf)(1 + cos(k hy))) / hy, and the
     normal-stress diffusion (2/Re) A sin(k yf) 2 (cos(k hy) - 1) / hy^2."""
-    nx, ny, A, V, Re = 8, 32, 0.3, 0.2, 25.0
+    nx, ny, A, V, Re = 16, 32, 0.3, 0.2, 25.0
     c = _case(om, nx, ny, Re=Re)
     try:
         xf, xc, hy, h = _grid(c)
@@ -203,7 +203,7 @@ def check_variable_viscosity_diffusion(om):
     ux = 0.  The flux-form x diffusion of uy is
     (1/Re) d/dx(mu duy/dx) = (2 b / Re)(mu(xc) + xc mu'), which the
     discrete operator reproduces exactly for these polynomials."""
-    nx, ny, b, Re, visr = 32, 8, 0.9, 12.0, 0.3
+    nx, ny, b, Re, visr = 32, 32, 0.9, 12.0, 0.3
     c0, c1 = 0.2, 0.6
     c = _case(om, nx, ny, Re=Re, visr=visr)
     try:
@@ -350,7 +350,7 @@ def check_stratified_column_at_rest(om, denr=0.1, steps=80):
     to hydrostatic balance dp/dx = -rho_face / Fr^2 on every face whose
     stencil does not reach the gas-filled top wall (whose zero ghost vof
     gives a wall curvature and a surface force in the reference's scheme)."""
-    nx, ny, Fr = 32, 8, 1.0
+    nx, ny, Fr = 32, 32, 1.0
     c = _case(om, nx, ny, Re=35.0, Fr=Fr, denr=denr, visr=1e-2, We=10.0)
     try:
         xf, xc, hy, h = _grid(c)
@@ -400,7 +400,7 @@ def check_decaying_shear_wave(om, steps=12, rtol=1e-13):
     P(dt L) = I + Z + Z^2/2 + Z^3/6 per step, so after N steps
     uy = prod P(dt_n L) uy_0 exactly (the dt of each step is what
     integrate returns)."""
-    nx, ny, Re = 32, 4, 20.0
+    nx, ny, Re = 32, 32, 20.0
     c = _case(om, nx, ny, Re=Re)
     try:
         xf, xc, hy, h = _grid(c)
