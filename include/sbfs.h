@@ -62,42 +62,51 @@ extern int sbfs_set_precision(int precision);
 
 /* curvature of the fused step's stage kernel (the staged path always uses
  * the plain one).  LEARNED evaluates the trained network (below) on the
- * stage's input vof and takes its value at sharp interfaces: in cells whose
- * 3 x 3 block holds a cell with vof in [0.02, 0.98], whose 5 x 5 stencil
- * holds cells below 1e-3 and above 1 - 1e-3 and avoids the wall ghosts, and
- * whose prediction |kappa h| lies in [1/96, 3/4].  The plain curvature
- * (src/interface/curvature_tensor.c:100-135) stays everywhere else, which
- * includes the wide tanh profile of the initial condition.  Needs uniform
- * square cells and a trained or loaded network. */
+ * stage's input vof and takes its value in the interface band: in cells
+ * whose 3 x 3 block holds a cell with vof in [0.02, 0.98], whose 5 x 5
+ * stencil avoids the wall ghosts and is not a sharp interface (cells below
+ * 1e-3 and above 1 - 1e-3 both present), and whose learned |kappa h| lies
+ * in [1/96, 1/4].  The plain curvature (src/interface/curvature_tensor.c:
+ * 100-135) stays everywhere else.  Needs uniform square cells and a trained
+ * or loaded network. */
 #define SBFS_CURVATURE_PLAIN   0
 #define SBFS_CURVATURE_LEARNED 1
 extern int sbfs_set_curvature(int mode);
 
 /*
- * The curvature network: a 25 -> 64 -> 64 -> 1 MLP with GELU, BF16 weights
- * and FP32 accumulation, mapping a 5 x 5 vof stencil to kappa h of its
- * centre cell.  Inputs are the differences to the centre value and the
- * centre value minus 1/2.  One network per process.
+ * The curvature network: a 26 -> 64 -> 64 -> 1 MLP with GELU, BF16 weights
+ * and FP32 accumulation, a learned correction of the plain curvature: its
+ * inputs are a 5 x 5 vof stencil (the differences to the centre value and
+ * the centre value minus 1/2) and the plain curvature kp of the centre cell
+ * (the reference's corner-normal divergence), and kappa h = kp + y of its
+ * output y.  One network per process.
  */
-/* number of FP32 parameters (W1 64x32 with columns 25.. zero, b1, W2 64x64,
+/* number of FP32 parameters (W1 64x32 with columns 26.. zero, b1, W2 64x64,
  * b2, w3, b3) */
 extern int sbfs_mlp_num_params(void);
 /* train from seeded Glorot weights with Adam on `nsamples` stencils the
- * device rasterises (circles and ellipses by 16x16 sub-cell sampling,
- * diffuse tanh circles); history[e] = mean squared error of epoch e */
+ * device rasterises (the initial condition's tanh profile around circles
+ * of radius 6 .. 64 cells, slopes 1/2 .. 1 per cell), minimising
+ * mean(w (y - t)^2) with target t = label - kp and weight w = 1 / label^2;
+ * history[e] = that loss over epoch e */
 extern int sbfs_mlp_train(uint64_t seed, size_t nsamples, int epochs, int batch,
                           double lr, double * history);
 extern int sbfs_mlp_set_params(const float * params, size_t n);
 extern int sbfs_mlp_get_params(float * params, size_t n);
 /* training samples first .. first+n-1 of a seed: features [n][32] (BF16
- * values), labels [n] (kappa h), and optionally the raw stencils [n][25] */
+ * values), labels [n] (kappa h), and optionally the targets [n], the
+ * weights [n] and the raw stencils [n][25] */
 extern int sbfs_mlp_samples(uint64_t seed, size_t first, size_t n,
-                            float * features, float * labels, float * stencils);
-/* gradient of the mean squared error over n (a multiple of 64) samples at
- * the current parameters, by the training kernels; *loss = the error */
-extern int sbfs_mlp_gradient(const float * features, const float * labels,
-                             size_t n, float * grad, double * loss);
-/* kappa h predicted for n stencils of 25 vof values (row-major, centre 12) */
+                            float * features, float * labels, float * targets,
+                            float * weights, float * stencils);
+/* gradient of mean(w (y - t)^2) over n (a multiple of 16) samples at the
+ * current parameters, by the training kernels (weights null: all 1);
+ * *loss = that mean */
+extern int sbfs_mlp_gradient(const float * features, const float * targets,
+                             const float * weights, size_t n, float * grad,
+                             double * loss);
+/* the learned kappa h = kp + y for n stencils of 25 vof values (row-major,
+ * centre 12) */
 extern int sbfs_mlp_predict(const double * stencils, size_t n, double * out);
 /* fraction of the interior cells of the current vof where LEARNED takes
  * the network's value */

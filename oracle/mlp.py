@@ -8,19 +8,22 @@ convention (kappa > 0 with the gas, vof = 1, inside) the labels follow.
 
 * `uniform`: the counter-based generator (splitmix64 finaliser) the
   device uses.
-* `samples`: the training stencils, restated from their definition. Sharp
-  cells are 16 x 16 sub-cell samples of circles and rotated ellipses; diffuse
-  cells are the initial-condition profile 1/2 (1 + tanh(s (R - d)))
-  (initial_condition/main.py:40-41). Half the samples are complemented
-  (1 - vof, label -kappa h); sharp stencils are centred on the cut cell or
-  one of its 8 neighbours. The tests pin this by closed forms:
-  the label of a circle is 1/R, and of an ellipse at its vertex a/b^2; the
-  diffuse value is the formula; and the sharp value is the area fraction.
+* `samples`: the training stencils, restated from their definition: the
+  initial-condition profile 1/2 (1 + tanh(s (R - d))) around circles
+  (initial_condition/main.py:40-41) at random sub-cell offsets, slopes and
+  normal offsets, in the 8 symmetries; half are complemented (1 - vof, label
+  -kappa h). The tests pin this by closed forms: the label is +-1/R, and the
+  values are the formula at the cell centres.
+* The network corrects the plain curvature: its features are the stencil
+  in canonical orientation (`canonical`: of its 8 images under the grid's
+  mirrors and transpose, the one whose core gradient has gx >= gy >= 0) and
+  kp = `plain_curvature` of it; its output y is added to kp, and training
+  minimises mean(w (y - t)^2) with t = label - kp, w = 1 / label^2.
 * `plain_curvature`: the reference's curvature of the centre cell of a
   stencil with h = 1 (curvature_tensor.c:16-49 corner gradients, :100-135
   divergence). The tests pin it against the C oracle on a full grid.
-* `loss_grad64`: forward and backward in float64 with no rounding, pinned
-  by central finite differences.
+* `loss_grad64`: forward and backward of the weighted loss in float64 with
+  no rounding, pinned by central finite differences.
 * `loss_grad_bf16`: the same with the device kernels' BF16 roundings (inputs,
   weights, layer-1 activations and the deltas that feed MFMA operands). This
   is the checker of the device gradients.
@@ -30,7 +33,7 @@ from __future__ import annotations
 import numpy as np
 from scipy.special import erf
 
-NI, NIP, NH = 25, 32, 64
+NI, NF, NIP, NH = 25, 26, 32, 64  # stencil values, features used, padded, hidden
 OW1 = 0
 OB1 = OW1 + NH * NIP
 OW2 = OB1 + NH
@@ -38,7 +41,6 @@ OB2 = OW2 + NH * NH
 OW3 = OB2 + NH
 OB3 = OW3 + NH
 NP = OB3 + 1
-KSUB = 16
 
 _M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
 
@@ -65,65 +67,79 @@ def bf16(x) -> np.ndarray:
     return (u.astype(np.uint32) << np.uint32(16)).view(np.float32)
 
 
+def canonical(st: np.ndarray) -> np.ndarray:
+    """Each [25] stencil in the orientation, among its 8 images under the
+    grid's mirrors and transpose, whose 3 x 3 core gradient sums (gx, gy)
+    satisfy gx >= gy >= 0."""
+    v = np.asarray(st, dtype=np.float64).reshape(-1, 25)
+    gx = np.zeros(v.shape[0])
+    gy = np.zeros(v.shape[0])
+    for r in (1, 2, 3):
+        gx = gx + (v[:, r * 5 + 3] - v[:, r * 5 + 1])
+    for c in (1, 2, 3):
+        gy = gy + (v[:, 3 * 5 + c] - v[:, 1 * 5 + c])
+    sx = np.where(gx < 0, -1, 1)
+    sy = np.where(gy < 0, -1, 1)
+    tr = np.abs(gy) > np.abs(gx)
+    out = np.empty_like(v)
+    for dj in range(-2, 3):
+        for di in range(-2, 3):
+            si = np.where(tr, sx * dj, sx * di)
+            sj = np.where(tr, sy * di, sy * dj)
+            out[:, (dj + 2) * 5 + di + 2] = v[np.arange(v.shape[0]), (sj + 2) * 5 + si + 2]
+    return out
+
+
 def features(st: np.ndarray) -> np.ndarray:
-    """[n][25] stencils -> [n][32] float32 features before BF16 rounding."""
-    st = np.asarray(st, dtype=np.float64)
+    """[n][25] stencils -> [n][32] float32 features before BF16 rounding, of
+    the canonical stencil: the differences to the centre, the centre - 1/2,
+    and kp."""
+    st = canonical(st)
     f = st - st[:, 12:13]
     f[:, 12] = st[:, 12] - 0.5
     out = np.zeros((st.shape[0], NIP), np.float32)
     out[:, :NI] = f.astype(np.float32)
+    out[:, NI] = plain_curvature(st).astype(np.float32)
     return out
 
 
-def samples(seed: int, first: int, n: int):
-    """(features [n][32] BF16 values, labels [n], stencils [n][25])."""
+def targets(labels64: np.ndarray, st: np.ndarray):
+    """Training targets label - kp (kp of the canonical stencil) and weights
+    1 / label^2 (float32) from float64 labels."""
+    lab = np.asarray(labels64, dtype=np.float64)
+    return (lab - plain_curvature(canonical(st))).astype(np.float32), (1.0 / (lab * lab)).astype(np.float32)
+
+
+def samples(seed: int, first: int, n: int, with_targets: bool = False):
+    """(features [n][32] BF16 values, labels [n], stencils [n][25]), and the
+    targets and weights if `with_targets`."""
     i = np.arange(first, first + n, dtype=np.uint64)
-    u = [uniform(seed, i, k) for k in range(13)]
-    ellipse = u[0] >= 0.5
-    b = 2.0 + 62.0 * u[1]
-    q = np.where(ellipse, 1.0 + 2.0 * u[2], 1.0)
-    a = q * b
-    phi, t = 2.0 * np.pi * u[3], 2.0 * np.pi * u[4]
+    u = {k: uniform(seed, i, k) for k in (1, 4, 5, 6, 8, 9, 10, 11)}
+    b = 6.0 + 58.0 * u[1]
+    t = 2.0 * np.pi * u[4]
     ox, oy = u[5], u[6]
-    diffuse = ~ellipse & (u[7] < 0.5)
-    slope = 0.5 * 4.0 ** u[8]
+    slope = 0.5 * 2.0 ** u[8]
     sym = (u[9] * 8.0).astype(np.int64)
-    cph, sph = np.cos(phi), np.sin(phi)
-    ex, ey = a * np.cos(t), b * np.sin(t)
-    px, py = cph * ex - sph * ey, sph * ex + cph * ey
     off = (2.0 * u[10] - 1.0) * 2.0 / slope
-    px = np.where(diffuse, px + off * px / b, px)
-    py = np.where(diffuse, py + off * py / b, py)
-    st_, ct_ = np.sin(t), np.cos(t)
+    px, py = (b + off) * np.cos(t), (b + off) * np.sin(t)
     comp = u[11] < 0.5
-    label = np.where(comp, -1.0, 1.0) * a * b / (a * a * st_ * st_ + b * b * ct_ * ct_) ** 1.5
+    label = np.where(comp, -1.0, 1.0) / b
     ic, jc = np.floor(px - ox), np.floor(py - oy)
-    m = ((u[12] - 0.5) * 16.0).astype(np.int64)
-    sh = np.where(m < 4, m, m + 1)
-    shift = ~diffuse & (u[12] >= 0.5)
-    ic = np.where(shift, ic + sh % 3 - 1, ic)
-    jc = np.where(shift, jc + sh // 3 - 1, jc)
-    sub = (np.arange(KSUB) + 0.5) / KSUB
     v = np.zeros((n, NI))
     for dj in range(-2, 3):
         for di in range(-2, 3):
-            x0, y0 = ic + di + ox, jc + dj + oy
-            cx, cy = x0 + 0.5, y0 + 0.5
-            dif = 0.5 * (1.0 + np.tanh(slope * (b - np.sqrt(cx * cx + cy * cy))))
-            xs = x0[:, None, None] + sub[None, None, :]
-            ys = y0[:, None, None] + sub[None, :, None]
-            xe = cph[:, None, None] * xs + sph[:, None, None] * ys
-            ye = -sph[:, None, None] * xs + cph[:, None, None] * ys
-            inside = (xe / a[:, None, None]) ** 2 + (ye / b[:, None, None]) ** 2 < 1.0
-            sharp = inside.sum(axis=(1, 2)) / float(KSUB * KSUB)
-            val = np.where(diffuse, dif, sharp)
+            cx, cy = ic + di + ox + 0.5, jc + dj + oy + 0.5
+            val = 0.5 * (1.0 + np.tanh(slope * (b - np.sqrt(cx * cx + cy * cy))))
             ti, tj = np.full(n, di), np.full(n, dj)
             t1 = (sym & 1) != 0
             ti, tj = np.where(t1, tj, ti), np.where(t1, ti, tj)
             ti = np.where((sym & 2) != 0, -ti, ti)
             tj = np.where((sym & 4) != 0, -tj, tj)
             v[np.arange(n), (tj + 2) * 5 + (ti + 2)] = np.where(comp, 1.0 - val, val)
-    return bf16(features(v)), label.astype(np.float32), v
+    out = (bf16(features(v)), label.astype(np.float32), v)
+    if with_targets:
+        out = out + targets(label, v)
+    return out
 
 
 def plain_curvature(st: np.ndarray) -> np.ndarray:
@@ -150,8 +166,8 @@ def plain_curvature(st: np.ndarray) -> np.ndarray:
 def glorot(seed: int) -> np.ndarray:
     """The device's initial parameters."""
     p = np.zeros(NP, np.float32)
-    r1, r2, r3 = np.sqrt(6.0 / (NI + NH)), np.sqrt(6.0 / (NH + NH)), np.sqrt(6.0 / (NH + 1))
-    o, k = np.meshgrid(np.arange(NH), np.arange(NI), indexing="ij")
+    r1, r2, r3 = np.sqrt(6.0 / (NF + NH)), np.sqrt(6.0 / (NH + NH)), np.sqrt(6.0 / (NH + 1))
+    o, k = np.meshgrid(np.arange(NH), np.arange(NF), indexing="ij")
     idx = (1000000000 + o * NIP + k).astype(np.uint64)
     p[OW1 + o * NIP + k] = (r1 * (2.0 * uniform(seed, idx, 0) - 1.0)).astype(np.float32)
     p[OW2:OW2 + NH * NH] = (r2 * (2.0 * uniform(seed, 2000000000 + np.arange(NH * NH), 0) - 1.0)).astype(np.float32)
@@ -192,24 +208,26 @@ def predict64(p, x):
     return w3 @ _gelu(h2) + b3
 
 
-def loss_grad64(p, x, t):
-    """Mean squared error and its gradient, float64, no rounding."""
+def loss_grad64(p, x, t, wt=None):
+    """Weighted mean squared error mean(wt (y - t)^2) and its gradient,
+    float64, no rounding (wt None: all 1)."""
     w1, b1, w2, b2, w3, b3 = (np.asarray(a, np.float64) for a in _unpack(p))
     x, t = np.asarray(x, np.float64), np.asarray(t, np.float64)
     n = x.shape[0]
+    wt = np.ones(n) if wt is None else np.asarray(wt, np.float64)
     h1 = w1 @ x.T + b1[:, None]
     a1 = _gelu(h1)
     h2 = w2 @ a1 + b2[:, None]
     a2 = _gelu(h2)
     y = w3 @ a2 + b3
     e = y - t
-    dy = 2.0 / n * e
+    dy = 2.0 / n * wt * e
     d2 = w3[:, None] * dy[None, :] * _dgelu(h2)
     d1 = (w2.T @ d2) * _dgelu(h1)
-    return float(np.mean(e * e)), _pack(d1 @ x, d1.sum(1), d2 @ a1.T, d2.sum(1), a2 @ dy, dy.sum())
+    return float(np.mean(wt * e * e)), _pack(d1 @ x, d1.sum(1), d2 @ a1.T, d2.sum(1), a2 @ dy, dy.sum())
 
 
-def loss_grad_bf16(p, x, t):
+def loss_grad_bf16(p, x, t, wt=None):
     """The same with the device's BF16 operand roundings (x must already be
     BF16 values); sums in float64, activations in float32."""
     w1, b1, w2, b2, w3, b3 = _unpack(np.asarray(p, np.float32))
@@ -217,16 +235,18 @@ def loss_grad_bf16(p, x, t):
     x = np.asarray(x, np.float32).astype(np.float64)
     t = np.asarray(t, np.float64)
     n = x.shape[0]
+    wt = np.ones(n, np.float32) if wt is None else np.asarray(wt, np.float32)
     f32 = lambda a: np.asarray(a, np.float64).astype(np.float32).astype(np.float64)  # noqa: E731
     h1 = f32(w1b @ x.T + b1[:, None])
     a1b = bf16(f32(_gelu(h1))).astype(np.float64)
     h2 = f32(w2b @ a1b + b2[:, None])
     a2 = f32(_gelu(h2))
     y = f32(w3.astype(np.float64) @ a2 + b3)
-    e = y - t
-    dy = f32(2.0 / n * e)
+    e = f32(y - t)
+    we = f32(wt.astype(np.float64) * e)
+    dy = f32(2.0 / n * we)
     d2 = f32(w3[:, None] * dy[None, :] * f32(_dgelu(h2)))
     d2b = bf16(d2).astype(np.float64)
     d1 = f32(f32(w2b.T @ d2b) * f32(_dgelu(h1)))
     d1b = bf16(d1).astype(np.float64)
-    return float(np.mean(e * e)), _pack(d1b @ x, d1.sum(1), d2b @ a1b.T, d2.sum(1), a2 @ dy, dy.sum())
+    return float(np.mean(we * e)), _pack(d1b @ x, d1.sum(1), d2b @ a1b.T, d2.sum(1), a2 @ dy, dy.sum())

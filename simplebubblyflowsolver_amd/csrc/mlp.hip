@@ -1,10 +1,20 @@
-// Learned curvature: a 25 -> 64 -> 64 -> 1 MLP (GELU) that maps a 5 x 5
+// Learned curvature: a 26 -> 64 -> 64 -> 1 MLP (GELU) that maps a 5 x 5
 // volume-fraction stencil to the grid-scaled curvature kappa * h of the
-// centre cell.  Everything runs on the device:
-//   * k_mlp_samples   rasterises training stencils from circles and ellipses
-//                     of known curvature (16 x 16 sub-cell sampling, random
-//                     sub-cell offsets, 8-fold symmetry) and from the solver's
-//                     own diffuse tanh initial profile;
+// centre cell, as a learned correction of the plain curvature: its inputs
+// are the stencil and the plain curvature of the centre cell, kp (the
+// reference's corner-normal divergence on the stencil), and its output y is
+// added to kp, so kappa h = kp + y.  On the wide tanh profile of the
+// solver's initial condition the plain value is the curvature of the level
+// set through the cell centre, 1/r; the network learns the curvature of the
+// interface itself, 1/R, the same on every cell of the band (a force
+// kappa grad(vof) with one kappa across the band is a discrete gradient,
+// which the pressure balances).  Everything runs on the device:
+//   * k_mlp_samples   rasterises training stencils of the solver's own
+//                     initial profile, tanh of the distance to a circle of
+//                     known radius, at random sub-cell offsets of the grid,
+//                     slopes and normal offsets, in the 8 symmetries; the
+//                     target is the label minus kp, weighted by 1 / label^2
+//                     (a relative squared error);
 //   * k_mlp_grad      forward + backward of kGradS (16) samples per wavefront
 //                     on BF16 MFMA with FP32 accumulation; writes one gradient
 //                     partial per wavefront (a 4096-sample batch is 256
@@ -12,10 +22,10 @@
 //   * k_mlp_adam      sums the partials in a fixed order and takes an Adam
 //                     step on the FP32 master weights, refreshing the BF16
 //                     copies the MFMA operands read;
-//   * k_curv_mlp      inference on the solver's vof, one 64-cell row segment
-//                     per wavefront, skipped where no cell is in the THINC
-//                     band; writes kappa (NaN where the plain curvature
-//                     stays).
+//   * k_curv_gate     lists the cells of the solver's vof in the interface
+//                     band;
+//   * k_curv_mlp      inference on those cells, 64 per wavefront; writes
+//                     kappa (NaN where the plain curvature stays).
 //
 // MFMA: v_mfma_f32_16x16x16bf16_1k.  Lane l holds A[l%16][4(l/16)+r],
 // B[4(l/16)+r][l%16] and D[4(l/16)+r][l%16] (r = 0..3).  Activations are
@@ -24,11 +34,13 @@
 // lane for lane.
 //
 // The reference computes the curvature from the normalised vof gradient
-// (src/interface/curvature_tensor.c:100-135); this network replaces it at
-// sharp interfaces (cut cells and their neighbours) whose stencil lies
-// inside the domain and whose prediction falls inside the training range.
+// (src/interface/curvature_tensor.c:100-135); this network replaces it in
+// the interface band (cut cells and their neighbours) of diffuse interfaces,
+// where the stencil lies inside the domain and the prediction falls inside
+// the training range.
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -39,9 +51,9 @@
 namespace sbfs {
 namespace mlp {
 
-constexpr int NI = 25, NIP = 32, NH = 64;
+constexpr int NI = 25, NF = 26, NIP = 32, NH = 64;  // stencil values, features used, padded
 // offsets of the FP32 parameter block
-constexpr int OW1 = 0;               // W1 [NH][NIP] (columns NI.. stay zero)
+constexpr int OW1 = 0;               // W1 [NH][NIP] (columns NF.. stay zero)
 constexpr int OB1 = OW1 + NH * NIP;  // b1 [NH]
 constexpr int OW2 = OB1 + NH;        // W2 [NH][NH] (out x in)
 constexpr int OB2 = OW2 + NH * NH;   // b2 [NH]
@@ -49,16 +61,14 @@ constexpr int OW3 = OB2 + NH;        // w3 [NH]
 constexpr int OB3 = OW3 + NH;        // b3
 constexpr int NP = OB3 + 1;          // 6337 parameters
 constexpr int NPART = NP + 1;        // + the loss sum of the tile
-constexpr int kSub = 16;             // sub-cell samples per direction
 constexpr double kPi = 3.14159265358979323846;
 // A cell gets the learned value if its 3 x 3 block holds an interface cell
-// (vof in [kUseLo, kUseHi]), its 5 x 5 stencil holds near-pure cells of
-// both phases (vof < kPure and > 1 - kPure: a sharp interface; on the wide
-// tanh profile of the initial condition the plain curvature is the more
-// accurate one) and avoids the wall ghosts, and |prediction| lies in
-// [kKhLo, kKhHi] (kappa h; the training circles span 1/64 .. 1/2).
+// (vof in [kUseLo, kUseHi]), its 5 x 5 stencil avoids the wall ghosts and is
+// not a sharp interface (near-pure cells of both phases, vof < kPure and
+// > 1 - kPure: outside the training profiles), and |kp + y| lies in
+// [kKhLo, kKhHi] (kappa h; the training circles span 1/64 .. 1/6).
 constexpr double kUseLo = 0.02, kUseHi = 0.98, kPure = 1e-3;
-constexpr float kKhLo = 1.f / 96.f, kKhHi = 0.75f;
+constexpr double kKhLo = 1. / 96., kKhHi = 0.25;
 
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -101,85 +111,100 @@ __device__ inline s4 pack4(float a, float b, float c, float d) {
   return r;
 }
 
-// features of a 5 x 5 stencil (row-major, centre at 12): differences to the
-// centre value, and the centre value minus 1/2
-__host__ __device__ inline void features(const double* v, float* f) {
+// plain curvature kappa h of the centre cell of a 5 x 5 stencil on square
+// cells (h = 1): the corner unit gradients of the four corners of the centre
+// cell (src/interface/curvature_tensor.c:16-49) and their divergence
+// (:100-135), in the reference's operation order
+__host__ __device__ inline double plain_kh(const double* v) {
+  double nx[2][2], ny[2][2];  // corner (ci, cj): ci 0 = left, cj 0 = bottom
+  for (int cj = 0; cj < 2; cj++)
+    for (int ci = 0; ci < 2; ci++) {
+      const int ai = 2 + ci, bj = 2 + cj;  // the corner between columns ai-1, ai and rows bj-1, bj
+      const double mm = v[(bj - 1) * 5 + ai - 1], pm = v[(bj - 1) * 5 + ai];
+      const double mp = v[bj * 5 + ai - 1], pp = v[bj * 5 + ai];
+      const double gx = 1. / 1. * (-mm + pm - mp + pp);
+      const double gy = 1. / 1. * (-mm - pm + mp + pp);
+      const double ninv = 1. / fmax(sqrt(gx * gx + gy * gy), DBL_EPSILON);
+      nx[cj][ci] = gx * ninv;
+      ny[cj][ci] = gy * ninv;
+    }
+  const double nxm = 0.5 * nx[0][0] + 0.5 * nx[1][0];
+  const double nxp = 0.5 * nx[0][1] + 0.5 * nx[1][1];
+  const double nym = 0.5 * ny[0][0] + 0.5 * ny[0][1];
+  const double nyp = 0.5 * ny[1][0] + 0.5 * ny[1][1];
+  return -1. * (-nxm + nxp - nym + nyp);
+}
+
+// The stencil in canonical orientation: of its 8 images under the mirrors
+// and the transpose of the grid, the one whose vof gradient over the 3 x 3
+// core (gx, gy) has gx >= gy >= 0.  Curvature does not change under these
+// maps, so the network sees one octant of orientations only, and two
+// stencils that are mirror images of each other (the top and the bottom of
+// a bubble) give the same features bit for bit: the learned curvature has
+// no orientation bias that could push a bubble.  The sums negate exactly
+// under a mirror, so the choice is consistent.
+__host__ __device__ inline void canonical(const double* v, double* w) {
+  double gx = 0., gy = 0.;
+  for (int r = 1; r <= 3; r++) gx += v[r * 5 + 3] - v[r * 5 + 1];
+  for (int c = 1; c <= 3; c++) gy += v[3 * 5 + c] - v[1 * 5 + c];
+  const int sx = gx < 0. ? -1 : 1, sy = gy < 0. ? -1 : 1;
+  const bool tr = fabs(gy) > fabs(gx);
+  for (int dj = -2; dj <= 2; dj++)
+    for (int di = -2; di <= 2; di++) {
+      const int si = tr ? sx * dj : sx * di, sj = tr ? sy * di : sy * dj;
+      w[(dj + 2) * 5 + di + 2] = v[(sj + 2) * 5 + si + 2];
+    }
+}
+
+// features of a canonical 5 x 5 stencil (row-major, centre at 12):
+// differences to the centre value, the centre value minus 1/2, and the
+// plain curvature kp
+__host__ __device__ inline void features(const double* v, double kp, float* f) {
   for (int k = 0; k < NI; k++) f[k] = (float)(k == 12 ? v[12] - 0.5 : v[k] - v[12]);
+  f[NI] = (float)kp;
 }
 
 // ---------------------------------------------------------------------------
 // training data
 // ---------------------------------------------------------------------------
 // One sample per thread.  Random numbers k of sample i:
-//   0 shape (ellipse if >= 1/2), 1 minor semi-axis b = 2 + 62 u cells,
-//   2 axis ratio q = 1 + 2 u (ellipses), 3 rotation, 4 point parameter t,
-//   5, 6 sub-cell offsets of the grid, 7 profile (diffuse if < 1/2, circles
-//   only), 8 diffuse slope s = 0.5 * 4^u per cell, 9 symmetry (8
-//   transforms), 10 normal offset of the centre cell (diffuse), 11
-//   complement (if < 1/2), 12 neighbour shift (sharp, if >= 1/2).
-// Sharp cells hold the gas fraction inside the shape from 16 x 16 sub-cell
-// points, and the stencil is centred on the cell holding the point
-// (a cos t, b sin t) or on one of its 8 neighbours.  Diffuse cells hold
-// 1/2 (1 + tanh(s (R - d))) at the cell centre, the solver's initial profile
-// (initial_condition/main.py:40-41), and the centre cell sits at a normal
-// offset uniform in +-2/s cells, where the centre vof is about 0.02 .. 0.98.
-// The label is the exact curvature of the interface times h at that point:
-// positive with the gas inside, the reference's sign convention.  The
-// complement (liquid inside, vof -> 1 - vof) flips its sign.
+//   1 radius R = 6 + 58 u cells, 4 point angle t, 5, 6 sub-cell offsets of
+//   the grid, 8 slope s = 0.5 * 2^u per cell, 9 symmetry (8 transforms),
+//   10 normal offset of the centre cell, 11 complement (if < 1/2); streams
+//   0, 2, 3, 7 and 12 are unused.
+// Cells hold 1/2 (1 + tanh(s (R - d))) at the cell centre, d the distance to
+// the circle's centre: the solver's initial profile
+// (initial_condition/main.py:40-41), whose slope per cell is 1/2 on every
+// configured grid; THINC's own profile (beta = 1) has slope 1.  The centre
+// cell sits at a normal offset uniform in +-2/s cells from the interface,
+// where its vof is about 0.02 .. 0.98.  The label is the curvature of the
+// interface (the vof = 1/2 circle) times h, +1/R with the gas inside, the
+// reference's sign convention; the complement (liquid inside,
+// vof -> 1 - vof) flips its sign.  The training target is label - kp,
+// weighted by 1 / label^2.
 __global__ void k_mlp_samples(uint64_t seed, long n0, long n, unsigned short* __restrict__ X,
-                              float* __restrict__ T, float* __restrict__ F) {
+                              float* __restrict__ T, float* __restrict__ W, float* __restrict__ Lab,
+                              float* __restrict__ F) {
   const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const uint64_t i = (uint64_t)(n0 + s);
   double u[13];
   for (int k = 0; k < 13; k++) u[k] = uni(seed, i, (uint64_t)k);
-  const bool ellipse = u[0] >= 0.5;
-  const double b = 2. + 62. * u[1];
-  const double q = ellipse ? 1. + 2. * u[2] : 1.;
-  const double a = q * b;
-  const double phi = 2. * kPi * u[3], t = 2. * kPi * u[4];
+  const double b = 6. + 58. * u[1];
+  const double t = 2. * kPi * u[4];
   const double ox = u[5], oy = u[6];
-  const bool diffuse = !ellipse && u[7] < 0.5;
-  const double slope = 0.5 * pow(4., u[8]);
+  const double slope = 0.5 * pow(2., u[8]);
   const int sym = (int)(u[9] * 8.);
-  const double cph = cos(phi), sph = sin(phi);
-  const double ex = a * cos(t), ey = b * sin(t);
-  double px = cph * ex - sph * ey, py = sph * ex + cph * ey;
-  if (diffuse) {
-    // circle: the outward normal at the point is the point over b
-    const double off = (2. * u[10] - 1.) * 2. / slope;
-    px += off * px / b;
-    py += off * py / b;
-  }
-  const double st = sin(t), ct = cos(t);
+  const double off = (2. * u[10] - 1.) * 2. / slope;
+  const double px = (b + off) * cos(t), py = (b + off) * sin(t);
   const bool comp = u[11] < 0.5;
-  const double label = (comp ? -1. : 1.) * a * b / pow(a * a * st * st + b * b * ct * ct, 1.5);
-  double ic = floor(px - ox), jc = floor(py - oy);
-  if (!diffuse && u[12] >= 0.5) {
-    const int m = (int)((u[12] - 0.5) * 16.);  // one of the 8 neighbours
-    const int sh = m < 4 ? m : m + 1;          // skip the centre of the 3 x 3 block
-    ic += sh % 3 - 1;
-    jc += sh / 3 - 1;
-  }
+  const double label = (comp ? -1. : 1.) / b;
+  const double ic = floor(px - ox), jc = floor(py - oy);
   double v[NI];
   for (int dj = -2; dj <= 2; dj++) {
     for (int di = -2; di <= 2; di++) {
-      const double x0 = ic + di + ox, y0 = jc + dj + oy;
-      double val;
-      if (diffuse) {
-        const double cx = x0 + 0.5, cy = y0 + 0.5;
-        val = 0.5 * (1. + tanh(slope * (b - sqrt(cx * cx + cy * cy))));
-      } else {
-        int cnt = 0;
-        for (int qy = 0; qy < kSub; qy++) {
-          for (int qx = 0; qx < kSub; qx++) {
-            const double x = x0 + (qx + 0.5) / kSub, y = y0 + (qy + 0.5) / kSub;
-            const double xe = cph * x + sph * y, ye = -sph * x + cph * y;
-            cnt += (xe / a) * (xe / a) + (ye / b) * (ye / b) < 1. ? 1 : 0;
-          }
-        }
-        val = cnt / (double)(kSub * kSub);
-      }
+      const double cx = ic + di + ox + 0.5, cy = jc + dj + oy + 0.5;
+      const double val = 0.5 * (1. + tanh(slope * (b - sqrt(cx * cx + cy * cy))));
       // 8-fold symmetry: bit 0 transposes, bit 1 mirrors x, bit 2 mirrors y
       int ti = di, tj = dj;
       if (sym & 1) {
@@ -192,10 +217,15 @@ __global__ void k_mlp_samples(uint64_t seed, long n0, long n, unsigned short* __
       v[(tj + 2) * 5 + (ti + 2)] = comp ? 1. - val : val;
     }
   }
-  float f[NI];
-  features(v, f);
-  for (int k = 0; k < NIP; k++) X[(size_t)s * NIP + k] = k < NI ? f2bf(f[k]) : 0;
-  T[s] = (float)label;
+  double c[NI];
+  canonical(v, c);
+  const double kp = plain_kh(c);
+  float f[NF];
+  features(c, kp, f);
+  for (int k = 0; k < NIP; k++) X[(size_t)s * NIP + k] = k < NF ? f2bf(f[k]) : 0;
+  T[s] = (float)(label - kp);
+  W[s] = (float)(1. / (label * label));
+  if (Lab) Lab[s] = (float)label;
   if (F)
     for (int k = 0; k < NI; k++) F[(size_t)s * NI + k] = (float)v[k];
 }
@@ -293,8 +323,9 @@ constexpr int kGradS = 16;
 
 template <int S>
 __global__ __launch_bounds__(64) void k_mlp_grad(Weights w, const unsigned short* __restrict__ X,
-                                                 const float* __restrict__ T, long n, long first,
-                                                 long stride, float scale, float* __restrict__ part) {
+                                                 const float* __restrict__ T, const float* __restrict__ Wt,
+                                                 long n, long first, long stride, float scale,
+                                                 float* __restrict__ part) {
   static_assert(S % 16 == 0 && S <= 64, "samples per wavefront");
   constexpr int NT = S / 16;
   __shared__ __align__(16) unsigned short sx[S * NIP];   // [sample][feature]
@@ -302,7 +333,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(Weights w, const unsigned short
   __shared__ __align__(16) unsigned short sa1[NH * S];   // gelu(h1) [hidden][sample]
   __shared__ __align__(16) unsigned short sd2[NH * S];   // dh2 [hidden][sample]
   __shared__ __align__(16) unsigned short sd1[NH * S];   // dh1 [hidden][sample]
-  __shared__ float st[S];
+  __shared__ float st[S], sw[S];
   const int l = threadIdx.x, lr = l & 15, lg = l >> 4;
   const long tile = blockIdx.x;
   if (l < S) {
@@ -315,6 +346,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(Weights w, const unsigned short
       sxt[k * S + l] = v;
     }
     st[l] = T[idx];
+    sw[l] = Wt[idx];
   }
   AOps a;
   load_aops(w, a);
@@ -331,10 +363,11 @@ __global__ __launch_bounds__(64) void k_mlp_grad(Weights w, const unsigned short
     load_x(sx, nt, x0, x1);
     f4 h1[4], h2[4];
     const float y = forward16(w, a, x0, x1, h1, h2);
-    // output layer: dy = scale (y - t), scale = 2 / batch
+    // output layer: loss w (y - t)^2, dy = scale w (y - t), scale = 2 / batch
     const float e = y - st[16 * nt + lr];
-    const float dy = scale * e;
-    lsum += e * e;
+    const float we = sw[16 * nt + lr] * e;
+    const float dy = scale * we;
+    lsum += we * e;
     db3 += dy;
     s4 d2b[4];
 #pragma unroll
@@ -511,19 +544,20 @@ __global__ __launch_bounds__(64) void k_mlp_predict(Weights w, const unsigned sh
 
 // ---------------------------------------------------------------------------
 // inference on the solver's vof, in two launches.  k_curv_gate (one thread
-// per cell, light) writes NaN to every cell and appends the 64-cell row
-// segments whose rows j-1..j+1, columns i-1..i+1 hold a cut cell (vof in
-// [kUseLo, kUseHi]) to a list; k_curv_mlp walks that list with a fixed grid
-// and writes kappa = y / h in the cells the rule above admits (NaN in the
-// others).  Layout of kappa: interior cells, (j-1) Nx + (i-1).
+// per cell, light) writes NaN to every cell and appends the cells whose
+// rows j-1..j+1, columns i-1..i+1 hold a cut cell (vof in [kUseLo, kUseHi])
+// and whose stencil stays off the wall ghosts to a list; k_curv_mlp walks
+// that list 64 cells per wavefront with a fixed grid and writes
+// kappa = (kp + y) / h in the cells the rule above admits (NaN stays in the
+// others).  Layout of kappa and of the list entries: interior cells,
+// (j-1) Nx + (i-1).
 // ---------------------------------------------------------------------------
 __global__ void k_curv_reset(unsigned* __restrict__ count) {
   if (threadIdx.x == 0) *count = 0u;
 }
 
 // one 64-cell column strip of kGateRows rows per wavefront, four per
-// workgroup; the three rows around row j roll along the strip, and each step
-// prefetches the next row
+// workgroup; every vof load of the strip is issued before the first test
 constexpr int kGateRows = 16;
 __global__ __launch_bounds__(256) void k_curv_gate(const double* __restrict__ vof, float* __restrict__ kappa,
                                                    int nx, int ny, unsigned* __restrict__ count,
@@ -532,89 +566,93 @@ __global__ __launch_bounds__(256) void k_curv_gate(const double* __restrict__ vo
   const int i = 1 + (int)blockIdx.x * 64 + l;
   const int j0 = 1 + ((int)blockIdx.y * 4 + (int)(threadIdx.x >> 6)) * kGateRows;
   if (j0 > ny) return;  // whole wavefronts
-  const int j1 = min(ny, j0 + kGateRows - 1);
   const size_t wc = (size_t)nx + 2;
   auto in_use = [](double v) { return (v >= kUseLo) & (v <= kUseHi); };
-  // this lane's column, and for the edge lanes the column beyond the strip
-  // (cells i-1 and i+1 of the 3 x 3 block); columns outside 1..nx never count
+  // this lane's column, and for lanes 0 and 63 the column just outside the
+  // strip (cells i-1 and i+1 of the 3 x 3 block); columns outside 1..nx
+  // never count.  Rows past ny + 1 of the last strip are clamped and unused.
   const bool own = i <= nx;
   const int ie = l == 0 ? i - 1 : i + 1;
   const bool edge = (l == 0 || l == 63) && ie >= 1 && ie <= nx;
   const int ic = min(i, nx), iec = min(max(ie, 1), nx);
-  auto row_cut = [&](int jj) {
-    const double* r = vof + (size_t)(jj + 1) * wc;
-    const double a = r[ic], b = r[iec];
-    return (own & in_use(a)) | (edge & in_use(b));
-  };
-  // every row load of the strip is issued before the first test (rows past
-  // ny + 1 of the last strip are clamped and unused)
-  bool cut[kGateRows + 2];
+  bool cc[kGateRows + 2], ce[kGateRows + 2];
 #pragma unroll
-  for (int k = 0; k < kGateRows + 2; k++) cut[k] = row_cut(min(j0 - 1 + k, ny + 1));
-  // rows of the strip with a cut cell in their 3 x 3 blocks; one atomic per
-  // wavefront reserves their slots in the list (one counter shared by all
-  // wavefronts: per-row atomics on it serialise)
-  unsigned mask = 0u;
+  for (int k = 0; k < kGateRows + 2; k++) {
+    const double* r = vof + (size_t)(min(j0 - 1 + k, ny + 1) + 1) * wc;
+    cc[k] = own & in_use(r[ic]);
+    ce[k] = edge & in_use(r[iec]);
+  }
+  const bool inside = own && i >= 3 && i <= nx - 2;
+  uint64_t adm[kGateRows];
+  unsigned total = 0u;
 #pragma unroll
   for (int k = 0; k < kGateRows; k++) {
     const int j = j0 + k;
-    if (j > j1) break;
-    if (own) kappa[(size_t)(j - 1) * nx + (i - 1)] = __int_as_float(0x7FC00000);
-    if (__ballot(cut[k] | cut[k + 1] | cut[k + 2]) != 0) mask |= 1u << k;
+    if (j <= ny && own) kappa[(size_t)(j - 1) * nx + (i - 1)] = __int_as_float(0x7FC00000);
+    // columns of the strip with a cut cell in rows j-1..j+1, then the
+    // 3-column neighbourhood from the neighbouring lanes' bits
+    const uint64_t m = __ballot(cc[k] | cc[k + 1] | cc[k + 2]);
+    const uint64_t left = __ballot(l == 0 && (ce[k] | ce[k + 1] | ce[k + 2]));
+    const uint64_t right = __ballot(l == 63 && (ce[k] | ce[k + 1] | ce[k + 2]));
+    const uint64_t nb = m | (m << 1) | (m >> 1) | (left != 0 ? 1ull : 0ull) | (right != 0 ? 1ull << 63 : 0ull);
+    adm[k] = __ballot(j <= ny && inside && ((nb >> l) & 1ull));
+    total += (unsigned)__popcll(adm[k]);
   }
-  if (mask != 0u && l == 0) {
-    unsigned q = atomicAdd(count, (unsigned)__popc(mask));
-    for (int k = 0; k < kGateRows; k++)
-      if (mask & (1u << k)) list[q++] = (unsigned)(j0 + k - 1) * gridDim.x + blockIdx.x;
+  if (total == 0u) return;
+  // one atomic per wavefront reserves its cells' slots (a shared counter:
+  // finer atomics on it serialise)
+  unsigned base = 0u;
+  if (l == 0) base = atomicAdd(count, total);
+  base = __shfl(base, 0);
+  const uint64_t below = l == 0 ? 0ull : (~0ull >> (64 - l));
+#pragma unroll
+  for (int k = 0; k < kGateRows; k++) {
+    if ((adm[k] >> l) & 1ull)
+      list[base + (unsigned)__popcll(adm[k] & below)] = (unsigned)(j0 + k - 1) * (unsigned)nx + (unsigned)(i - 1);
+    base += (unsigned)__popcll(adm[k]);
   }
 }
 
 __global__ __launch_bounds__(64) void k_curv_mlp(Weights w, const double* __restrict__ vof,
                                                  float* __restrict__ kappa, int nx, int ny, float inv_h,
                                                  const unsigned* __restrict__ count,
-                                                 const unsigned* __restrict__ list, unsigned segs_x) {
+                                                 const unsigned* __restrict__ list) {
   __shared__ __align__(16) unsigned short sx[64 * NIP];
-  __shared__ double tile[5][68];  // rows j-2..j+2, columns i0-2..i0+65
   const int l = threadIdx.x, lg = l >> 4;
   const unsigned n = *count;
-  if (blockIdx.x >= n) return;  // no segment for this workgroup (all of them when none is listed)
+  if ((unsigned)blockIdx.x * 64u >= n) return;  // no cell for this workgroup (all of them when none is listed)
   const size_t wc = (size_t)nx + 2;
   AOps a;
   load_aops(w, a);
-  // every wavefront leaves the loop after at most ceil(n / gridDim.x) segments
-  for (unsigned q = blockIdx.x; q < n; q += gridDim.x) {
-    const unsigned seg = list[q];
-    const int i0 = 1 + (int)(seg % segs_x) * 64, j = 1 + (int)(seg / segs_x);
-    const int i = i0 + l;
-    const bool cell = i <= nx;
-    // the vof layout has 2 halo rows, so rows j-2..j+2 exist; columns are
-    // clamped to the wall ghosts 0 and nx+1 (cells whose stencil reaches
-    // them are not admitted)
-    __syncthreads();
-    for (int k = l; k < 5 * 68; k += 64) {
-      const int r = k / 68, c = k % 68;
-      const int ii = min(max(i0 + c - 2, 0), nx + 1);
-      tile[r][c] = vof[(size_t)(j + r - 2 + 1) * wc + ii];
-    }
-    __syncthreads();
-    bool ok = cell && i >= 3 && i <= nx - 2;
-    double lo = 1., hi = 0.;
-    bool cut = false;
+  // every wavefront leaves the loop after at most ceil(n / (64 gridDim.x))
+  // passes; a listed cell has 3 <= i <= nx - 2, so its 5 x 5 stencil lies in
+  // columns 1..nx and, with the 2 halo rows of vof, in rows j-2..j+2
+  for (unsigned q0 = (unsigned)blockIdx.x * 64u; q0 < n; q0 += (unsigned)gridDim.x * 64u) {
+    const unsigned q = q0 + (unsigned)l;
+    const bool has = q < n;
+    const unsigned c = has ? list[q] : 2u;  // lanes past the end read cell (3, 1) and write nothing
+    const int i = 1 + (int)(c % (unsigned)nx), j = 1 + (int)(c / (unsigned)nx);
     double v[NI];
+#pragma unroll
     for (int r = 0; r < 5; r++)
-      for (int d = -2; d <= 2; d++) {
-        const double x = tile[r][l + 2 + d];
-        v[r * 5 + d + 2] = x;
-        lo = fmin(lo, x);
-        hi = fmax(hi, x);
-        if (r >= 1 && r <= 3 && d >= -1 && d <= 1) cut = cut || (x >= kUseLo && x <= kUseHi);
-      }
-    ok = ok && cut && lo < kPure && hi > 1. - kPure;
+#pragma unroll
+      for (int d = -2; d <= 2; d++) v[r * 5 + d + 2] = vof[(size_t)(j + r - 2 + 1) * wc + (i + d)];
+    double lo = 1., hi = 0.;
+#pragma unroll
+    for (int k = 0; k < NI; k++) {
+      lo = fmin(lo, v[k]);
+      hi = fmax(hi, v[k]);
+    }
+    const bool ok = has && !(lo < kPure && hi > 1. - kPure);
     if (__ballot(ok) == 0) continue;  // uniform: the NaN written by the gate stays
+    double cv[NI];
+    canonical(v, cv);
+    const double kp = plain_kh(cv);
+    __syncthreads();  // the previous pass's reads of sx are done
     {
-      float f[NI];
-      features(v, f);
-      for (int k = 0; k < NIP; k++) sx[l * NIP + k] = k < NI ? f2bf(f[k]) : 0;
+      float f[NF];
+      features(cv, kp, f);
+      for (int k = 0; k < NIP; k++) sx[l * NIP + k] = k < NF ? f2bf(f[k]) : 0;
     }
     __syncthreads();
     // lane l = 16 lg + l%16 keeps the prediction of subtile lg
@@ -627,8 +665,9 @@ __global__ __launch_bounds__(64) void k_curv_mlp(Weights w, const double* __rest
       const float p = forward16(w, a, x0, x1, h1, h2);
       if (lg == nt) y = p;
     }
-    const bool use = ok && fabsf(y) >= kKhLo && fabsf(y) <= kKhHi;
-    if (use) kappa[(size_t)(j - 1) * nx + (i - 1)] = y * inv_h;
+    const double kh = kp + (double)y;
+    const bool use = ok && fabs(kh) >= kKhLo && fabs(kh) <= kKhHi;
+    if (use) kappa[c] = (float)(kh * (double)inv_h);
   }
 }
 
@@ -695,28 +734,30 @@ static int set_params(const float* host) {
 // Glorot-uniform weights from the counter generator, zero biases
 static void init_params(uint64_t seed, std::vector<float>& p) {
   p.assign(NP, 0.f);
-  const double r1 = sqrt(6. / (NI + NH)), r2 = sqrt(6. / (NH + NH)), r3 = sqrt(6. / (NH + 1));
+  const double r1 = sqrt(6. / (NF + NH)), r2 = sqrt(6. / (NH + NH)), r3 = sqrt(6. / (NH + 1));
   for (int o = 0; o < NH; o++)
-    for (int k = 0; k < NI; k++) p[OW1 + o * NIP + k] = (float)(r1 * (2. * uni(seed, 1000000000ull + o * NIP + k, 0) - 1.));
+    for (int k = 0; k < NF; k++) p[OW1 + o * NIP + k] = (float)(r1 * (2. * uni(seed, 1000000000ull + o * NIP + k, 0) - 1.));
   for (int k = 0; k < NH * NH; k++) p[OW2 + k] = (float)(r2 * (2. * uni(seed, 2000000000ull + k, 0) - 1.));
   for (int k = 0; k < NH; k++) p[OW3 + k] = (float)(r3 * (2. * uni(seed, 3000000000ull + k, 0) - 1.));
 }
 
 struct Samples {
   unsigned short* x = nullptr;
-  float* t = nullptr;
+  float* t = nullptr;  // targets
+  float* w = nullptr;  // weights
   long n = 0;
   uint64_t seed = 0;
 };
 
-static int make_samples(uint64_t seed, long n, Samples& s, float* stencils_dev) {
+static int make_samples(uint64_t seed, long n, Samples& s) {
   SBFS_REQUIRE(n > 0, "mlp: no samples");
   SBFS_HIP(hipMalloc(&s.x, sizeof(unsigned short) * NIP * (size_t)n));
-  SBFS_HIP(hipMalloc(&s.t, sizeof(float) * (size_t)n));
+  SBFS_HIP(hipMalloc(&s.t, sizeof(float) * 2 * (size_t)n));
+  s.w = s.t + n;
   s.n = n;
   s.seed = seed;
-  hipLaunchKernelGGL(k_mlp_samples, dim3(cdiv(n, 128)), dim3(128), 0, stream(), seed, 0L, n, s.x, s.t,
-                     stencils_dev);
+  hipLaunchKernelGGL(k_mlp_samples, dim3(cdiv(n, 128)), dim3(128), 0, stream(), seed, 0L, n, s.x, s.t, s.w,
+                     (float*)nullptr, (float*)nullptr);
   return check_launch("mlp: samples");
 }
 
@@ -729,7 +770,7 @@ static void free_samples(Samples& s) {
 // one pass of `tiles` tiles starting at permuted position `first`
 static int grad_pass(const Samples& s, long first, long stride, int tiles, float scale, float* part) {
   SBFS_TIMED("k_mlp_grad", hipLaunchKernelGGL(k_mlp_grad<kGradS>, dim3(tiles), dim3(64), 0, stream(), weights(), s.x,
-                                              s.t, s.n, first, stride, scale, part));
+                                              s.t, s.w, s.n, first, stride, scale, part));
   return check_launch("mlp: gradients");
 }
 
@@ -742,24 +783,23 @@ int curv_mlp_launch(const double* vof, float* kappa, int nx, int ny, double h, c
                     hipStream_t st) {
   using namespace mlp;
   SBFS_REQUIRE(g_net.trained, "learned curvature: no trained network (sbfs_mlp_train or sbfs_mlp_set_params)");
-  static unsigned* buf = nullptr;  // [0] counter, then the segment list
+  static unsigned* buf = nullptr;  // [0] counter, then the cell list
   static size_t cap = 0;
-  const unsigned segs_x = cdiv(nx, 64);
-  const size_t segs = (size_t)segs_x * ny;
-  if (segs + 1 > cap) {
+  const size_t cells = (size_t)nx * ny;
+  if (cells + 1 > cap) {
     (void)hipStreamSynchronize(stream());
     (void)hipFree(buf);
     buf = nullptr;
-    SBFS_HIP(hipMalloc(&buf, sizeof(unsigned) * (segs + 1)));
-    cap = segs + 1;
+    SBFS_HIP(hipMalloc(&buf, sizeof(unsigned) * (cells + 1)));
+    cap = cells + 1;
     graph_epoch_bump();  // captured steps hold the old address
   }
   hipLaunchKernelGGL(k_curv_reset, dim3(1), dim3(64), 0, st, buf);
-  SBFS_TIMED("k_curv_gate", hipLaunchKernelGGL(k_curv_gate, dim3(segs_x, cdiv(ny, 4 * kGateRows)), dim3(256), 0,
-                                               st, vof, kappa, nx, ny, buf, buf + 1));
-  SBFS_TIMED("k_curv_mlp", hipLaunchKernelGGL(k_curv_mlp, dim3(2048), dim3(64), 0, st, weights(), vof, kappa,
+  SBFS_TIMED("k_curv_gate", hipLaunchKernelGGL(k_curv_gate, dim3(cdiv(nx, 64), cdiv(ny, 4 * kGateRows)), dim3(256),
+                                               0, st, vof, kappa, nx, ny, buf, buf + 1));
+  SBFS_TIMED("k_curv_mlp", hipLaunchKernelGGL(k_curv_mlp, dim3(4096), dim3(64), 0, st, weights(), vof, kappa,
                                               nx, ny, (float)(1. / h), (const unsigned*)buf,
-                                              (const unsigned*)(buf + 1), segs_x));
+                                              (const unsigned*)(buf + 1)));
   *count = buf;
   return check_launch("k_curv_mlp");
 }
@@ -772,20 +812,22 @@ using namespace sbfs::mlp;
 extern "C" int sbfs_mlp_num_params(void) { return NP; }
 
 extern "C" int sbfs_mlp_samples(uint64_t seed, size_t first, size_t n, float* features, float* labels,
-                                float* stencils) {
+                                float* targets, float* weights, float* stencils) {
   SBFS_REQUIRE(n > 0 && features && labels, "sbfs_mlp_samples: bad arguments");
   unsigned short* x = nullptr;
   float *t = nullptr, *f = nullptr;
   SBFS_HIP(hipMalloc(&x, sizeof(unsigned short) * NIP * n));
-  SBFS_HIP(hipMalloc(&t, sizeof(float) * n));
+  SBFS_HIP(hipMalloc(&t, sizeof(float) * 3 * n));
   if (stencils) SBFS_HIP(hipMalloc(&f, sizeof(float) * NI * n));
   hipLaunchKernelGGL(k_mlp_samples, dim3(cdiv((long)n, 128)), dim3(128), 0, stream(), seed, (long)first, (long)n,
-                     x, t, f);
+                     x, t + n, t + 2 * n, t, f);
   int rc = check_launch("sbfs_mlp_samples");
   std::vector<unsigned short> hx(NIP * n);
   if (!rc) {
     SBFS_HIP(hipMemcpyAsync(hx.data(), x, sizeof(unsigned short) * NIP * n, hipMemcpyDeviceToHost, stream()));
     SBFS_HIP(hipMemcpyAsync(labels, t, sizeof(float) * n, hipMemcpyDeviceToHost, stream()));
+    if (targets) SBFS_HIP(hipMemcpyAsync(targets, t + n, sizeof(float) * n, hipMemcpyDeviceToHost, stream()));
+    if (weights) SBFS_HIP(hipMemcpyAsync(weights, t + 2 * n, sizeof(float) * n, hipMemcpyDeviceToHost, stream()));
     if (stencils) SBFS_HIP(hipMemcpyAsync(stencils, f, sizeof(float) * NI * n, hipMemcpyDeviceToHost, stream()));
     SBFS_HIP(hipStreamSynchronize(stream()));
     for (size_t k = 0; k < NIP * n; k++) {
@@ -815,25 +857,30 @@ extern "C" int sbfs_mlp_get_params(float* params, size_t n) {
   return 0;
 }
 
-// gradient of the mean squared error over n samples (n a multiple of 64)
-// given as BF16-representable features [n][32] and labels, at the current
-// parameters; grad[NP], *loss = the mean squared error
-extern "C" int sbfs_mlp_gradient(const float* features, const float* labels, size_t n, float* grad,
-                                 double* loss) {
-  SBFS_REQUIRE(features && labels && grad && loss && n > 0 && n % kGradS == 0, "sbfs_mlp_gradient: bad arguments");
+// gradient of the weighted mean squared error mean(w (y - t)^2) over n
+// samples (n a multiple of 16) given as BF16-representable features [n][32],
+// targets and weights (null: all 1), at the current parameters; grad[NP],
+// *loss = the weighted mean squared error
+extern "C" int sbfs_mlp_gradient(const float* features, const float* targets, const float* weights, size_t n,
+                                 float* grad, double* loss) {
+  SBFS_REQUIRE(features && targets && grad && loss && n > 0 && n % kGradS == 0, "sbfs_mlp_gradient: bad arguments");
   SBFS_REQUIRE(g_net.p, "sbfs_mlp_gradient: no parameters set");
   std::vector<unsigned short> hx(NIP * n);
   for (size_t k = 0; k < NIP * n; k++) hx[k] = f2bf(features[k]);
+  std::vector<float> hw(n, 1.f);
+  if (weights) memcpy(hw.data(), weights, sizeof(float) * n);
   Samples s;
   SBFS_HIP(hipMalloc(&s.x, sizeof(unsigned short) * NIP * n));
-  SBFS_HIP(hipMalloc(&s.t, sizeof(float) * n));
+  SBFS_HIP(hipMalloc(&s.t, sizeof(float) * 2 * n));
+  s.w = s.t + n;
   s.n = (long)n;
   const int tiles = (int)(n / kGradS);
   float *part = nullptr, *g = nullptr;
   SBFS_HIP(hipMalloc(&part, sizeof(float) * NPART * tiles));
   SBFS_HIP(hipMalloc(&g, sizeof(float) * NPART));
   SBFS_HIP(hipMemcpyAsync(s.x, hx.data(), sizeof(unsigned short) * NIP * n, hipMemcpyHostToDevice, stream()));
-  SBFS_HIP(hipMemcpyAsync(s.t, labels, sizeof(float) * n, hipMemcpyHostToDevice, stream()));
+  SBFS_HIP(hipMemcpyAsync(s.t, targets, sizeof(float) * n, hipMemcpyHostToDevice, stream()));
+  SBFS_HIP(hipMemcpyAsync(s.w, hw.data(), sizeof(float) * n, hipMemcpyHostToDevice, stream()));
   int rc = grad_pass(s, 0, 1, tiles, 2.f / (float)n, part);
   if (!rc) {
     hipLaunchKernelGGL(k_mlp_adam, dim3(cdiv(NPART * kAdamSlices, 256)), dim3(256), 0, stream(), adam_state(), (const float*)part,
@@ -853,15 +900,20 @@ extern "C" int sbfs_mlp_gradient(const float* features, const float* labels, siz
   return rc;
 }
 
-// predictions y (= kappa h) for n stencils of 25 vof values (row-major, centre 12)
+// the learned kappa h = kp + y for n stencils of 25 vof values (row-major,
+// centre 12)
 extern "C" int sbfs_mlp_predict(const double* stencils, size_t n, double* out) {
   SBFS_REQUIRE(stencils && out && n > 0, "sbfs_mlp_predict: bad arguments");
   SBFS_REQUIRE(g_net.trained, "sbfs_mlp_predict: no trained network");
   std::vector<unsigned short> hx(NIP * n, 0);
+  std::vector<double> kp(n);
   for (size_t s = 0; s < n; s++) {
-    float f[NI];
-    features(stencils + s * NI, f);
-    for (int k = 0; k < NI; k++) hx[s * NIP + k] = f2bf(f[k]);
+    float f[NF];
+    double c[NI];
+    canonical(stencils + s * NI, c);
+    kp[s] = plain_kh(c);
+    features(c, kp[s], f);
+    for (int k = 0; k < NF; k++) hx[s * NIP + k] = f2bf(f[k]);
   }
   unsigned short* x = nullptr;
   float* y = nullptr;
@@ -874,7 +926,7 @@ extern "C" int sbfs_mlp_predict(const double* stencils, size_t n, double* out) {
   if (!rc) {
     SBFS_HIP(hipMemcpyAsync(hy.data(), y, sizeof(float) * n, hipMemcpyDeviceToHost, stream()));
     SBFS_HIP(hipStreamSynchronize(stream()));
-    for (size_t s = 0; s < n; s++) out[s] = hy[s];
+    for (size_t s = 0; s < n; s++) out[s] = kp[s] + (double)hy[s];
   }
   (void)hipFree(x);
   (void)hipFree(y);
@@ -882,9 +934,9 @@ extern "C" int sbfs_mlp_predict(const double* stencils, size_t n, double* out) {
 }
 
 // train from Glorot-uniform weights on `nsamples` generated stencils (a
-// multiple of `batch`, itself a multiple of 64) for `epochs` passes with Adam
-// (learning rate `lr`, cosine decay to lr / 20); history[e] = mean squared
-// error of epoch e (may be null)
+// multiple of `batch`, itself a multiple of 16) for `epochs` passes with Adam
+// (learning rate `lr`, cosine decay to lr / 20); history[e] = weighted mean
+// squared error of epoch e (may be null)
 extern "C" int sbfs_mlp_train(uint64_t seed, size_t nsamples, int epochs, int batch, double lr,
                               double* history) {
   SBFS_REQUIRE(nsamples > 0 && epochs > 0 && batch > 0 && batch % kGradS == 0 && nsamples % (size_t)batch == 0,
@@ -893,7 +945,7 @@ extern "C" int sbfs_mlp_train(uint64_t seed, size_t nsamples, int epochs, int ba
   init_params(seed, p0);
   if (set_params(p0.data())) return 1;
   Samples s;
-  if (make_samples(seed, (long)nsamples, s, nullptr)) return 1;
+  if (make_samples(seed, (long)nsamples, s)) return 1;
   const int tiles = batch / kGradS;
   const long steps = (long)(nsamples / (size_t)batch);
   float *part = nullptr, *loss = nullptr;

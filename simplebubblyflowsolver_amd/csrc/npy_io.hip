@@ -81,10 +81,24 @@ static int npy_read(const char* dir, const char* name, const char* descr,
       while (k < t.size() && t[k] != ',') k++;
     }
   }
-  bool ok = d == descr && !fortran && sh == shape;
-  if (!ok) {
+  // the checks of src/fileio.c:200-225, in its order: ndims, shape, dtype,
+  // storage order
+  std::string bad;
+  char msg[256];
+  if (sh.size() != shape.size()) {
+    snprintf(msg, sizeof msg, "ndims: %zu expected, %zu obtained", shape.size(), sh.size());
+    bad = msg;
+  }
+  for (size_t n = 0; bad.empty() && n < shape.size(); n++)
+    if (sh[n] != shape[n]) {
+      snprintf(msg, sizeof msg, "shape[%zu]: %zu expected, %zu obtained", n, shape[n], sh[n]);
+      bad = msg;
+    }
+  if (bad.empty() && d != descr) bad = std::string("dtype: '") + descr + "' expected, '" + d + "' obtained";
+  if (bad.empty() && fortran) bad = "is_fortran_order: false expected, true obtained";
+  if (!bad.empty()) {
     fclose(fp);
-    SBFS_REQUIRE(false, "%s: dtype / order / shape differ from the expected %s", path.c_str(), descr);
+    SBFS_REQUIRE(false, "%s: %s", path.c_str(), bad.c_str());
   }
   size_t count = elem;
   for (size_t s : shape) count *= s;

@@ -139,8 +139,8 @@ class CurvatureNet:
         lib.sbfs_mlp_train.argtypes = [C.c_uint64, C.c_size_t, C.c_int, C.c_int, C.c_double, _P(C.c_double)]
         lib.sbfs_mlp_set_params.argtypes = [_P(f), C.c_size_t]
         lib.sbfs_mlp_get_params.argtypes = [_P(f), C.c_size_t]
-        lib.sbfs_mlp_samples.argtypes = [C.c_uint64, C.c_size_t, C.c_size_t, _P(f), _P(f), _P(f)]
-        lib.sbfs_mlp_gradient.argtypes = [_P(f), _P(f), C.c_size_t, _P(f), _P(C.c_double)]
+        lib.sbfs_mlp_samples.argtypes = [C.c_uint64, C.c_size_t, C.c_size_t, _P(f), _P(f), _P(f), _P(f), _P(f)]
+        lib.sbfs_mlp_gradient.argtypes = [_P(f), _P(f), _P(f), C.c_size_t, _P(f), _P(C.c_double)]
         lib.sbfs_mlp_predict.argtypes = [_P(C.c_double), C.c_size_t, _P(C.c_double)]
         lib.sbfs_set_curvature.argtypes = [C.c_int]
         lib.sbfs_mlp_coverage.argtypes = [_P(abi.DomainT), _P(abi.InterfaceT), _P(C.c_double)]
@@ -166,19 +166,28 @@ class CurvatureNet:
         self.be.check(self.lib.sbfs_mlp_set_params(self._p(p, C.c_float), p.size), "sbfs_mlp_set_params")
 
     def samples(self, seed: int, first: int, n: int):
+        """(features [n][32], labels, stencils [n][25], targets, weights)."""
         x = np.zeros((n, 32), np.float32)
+        lab = np.zeros(n, np.float32)
         t = np.zeros(n, np.float32)
+        w = np.zeros(n, np.float32)
         st = np.zeros((n, 25), np.float32)
-        self.be.check(self.lib.sbfs_mlp_samples(seed, first, n, self._p(x, C.c_float), self._p(t, C.c_float),
+        self.be.check(self.lib.sbfs_mlp_samples(seed, first, n, self._p(x, C.c_float), self._p(lab, C.c_float),
+                                                self._p(t, C.c_float), self._p(w, C.c_float),
                                                 self._p(st, C.c_float)), "sbfs_mlp_samples")
-        return x, t, st
+        return x, lab, st, t, w
 
-    def gradient(self, x: np.ndarray, t: np.ndarray):
+    def gradient(self, x: np.ndarray, t: np.ndarray, w: np.ndarray | None = None):
+        """Gradient of mean(w (y - t)^2) at the current parameters."""
         x = np.ascontiguousarray(x, np.float32)
         t = np.ascontiguousarray(t, np.float32)
+        wp = None
+        if w is not None:
+            w = np.ascontiguousarray(w, np.float32)
+            wp = self._p(w, C.c_float)
         g = np.zeros(self.num_params, np.float32)
         loss = C.c_double(0.0)
-        self.be.check(self.lib.sbfs_mlp_gradient(self._p(x, C.c_float), self._p(t, C.c_float), t.size,
+        self.be.check(self.lib.sbfs_mlp_gradient(self._p(x, C.c_float), self._p(t, C.c_float), wp, t.size,
                                                  self._p(g, C.c_float), C.byref(loss)), "sbfs_mlp_gradient")
         return loss.value, g
 

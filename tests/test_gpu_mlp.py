@@ -1,15 +1,19 @@
 """The learned curvature on the GPU, against the CPU restatement
-(oracle/mlp.py) and against the plain curvature it replaces.
+(oracle/mlp.py) and against the plain curvature it corrects.
 
-* The device rasterises the same training stencils as the restatement.
+* The device rasterises the same training stencils, targets and weights as
+  the restatement.
 * The gradients of the BF16-MFMA training kernels equal the restatement's
   with the same BF16 roundings (relative 1e-4 per parameter block).
-* Training on 2^21 stencils lowers the loss, and on held-out sharp stencils
-  the network is far more accurate than the plain curvature.
-* In the solver: on a sharp static drop the network removes most of the
-  force imbalance of the first step; on the wide tanh profile of the
-  initial condition it leaves the plain curvature in place (the result is
-  bitwise the plain one).
+* Training on 2^21 stencils lowers the loss, and on held-out stencils of
+  the configured profile the learned curvature is closer to the interface
+  curvature than the plain one.
+* In the solver (mixed precision, as the chosen form runs it): the network
+  takes the interface band of the initial condition's tanh profile; the
+  static drop of radius 32 cells on 256 x 256 keeps smaller spurious
+  currents than with the plain curvature over 5000 steps; the rising bubble
+  of the repository's default case stays within 1 % of the FP64 step's
+  rise-velocity history over its 2000 steps, with the gas volume held.
 """
 import numpy as np
 import pytest
@@ -24,11 +28,15 @@ TRAIN = dict(seed=2, nsamples=1 << 21, epochs=40, batch=4096, lr=1e-3)
 
 def test_samples_match_restatement(product):
     net = solver.CurvatureNet(product)
-    x, t, st = net.samples(21, 1000, 512)
-    xo, to, sto = omlp.samples(21, 1000, 512)
-    assert np.array_equal(t, to)
+    x, lab, st, t, w = net.samples(21, 1000, 512)
+    xo, labo, sto, to, wo = omlp.samples(21, 1000, 512, with_targets=True)
+    assert np.array_equal(lab, labo)
     assert np.abs(st - sto).max() < 1e-6
-    assert np.array_equal(x, xo)
+    # the plain-curvature feature and target pass through device and glibc
+    # tanh / sqrt: equal up to the last bits
+    assert np.mean(x == xo) > 0.999 and np.abs(x - xo).max() <= 2.0 ** -7 * np.abs(xo).max()
+    assert np.allclose(t, to, rtol=1e-5, atol=1e-7)
+    assert np.allclose(w, wo, rtol=1e-6)
 
 
 def test_gradient_matches_restatement(product):
@@ -39,9 +47,9 @@ def test_gradient_matches_restatement(product):
     p[omlp.OB2:omlp.OW3] = 0.1 * rng.standard_normal(64)
     p[omlp.OB3] = 0.05
     net.set_params(p)
-    x, t, _ = omlp.samples(8, 0, 512)
-    loss, g = net.gradient(x, t)
-    lo, go = omlp.loss_grad_bf16(p, x, t)
+    x, _, _, t, w = omlp.samples(8, 0, 512, with_targets=True)
+    loss, g = net.gradient(x, t, w)
+    lo, go = omlp.loss_grad_bf16(p, x, t, w)
     print(f"loss device {loss:.9e} restatement {lo:.9e}", flush=True)
     assert loss == pytest.approx(lo, rel=1e-5)
     for name, a, b in (("W1", omlp.OW1, omlp.OB1), ("b1", omlp.OB1, omlp.OW2), ("W2", omlp.OW2, omlp.OB2),
@@ -55,71 +63,113 @@ def test_gradient_matches_restatement(product):
 def trained(product):
     net = solver.CurvatureNet(product)
     hist = net.train(**TRAIN)
-    print("training loss per epoch", np.array2string(hist[[0, 1, 9, 19, -1]], precision=3), flush=True)
+    print("training loss per epoch", np.array2string(hist[[0, 1, 9, 19, -1]], precision=4), flush=True)
     return net, hist
 
 
 def test_training_beats_plain_curvature(trained):
     net, hist = trained
-    assert hist[-1] < hist[0] / 5
-    x, t, st = omlp.samples(99, 0, 16384)   # held out: another seed
+    assert hist[-1] < hist[0] / 3
+    _, lab, st = omlp.samples(99, 0, 16384)   # held out: another seed
     y, plain = net.predict(st), omlp.plain_curvature(st)
-    b5 = st.reshape(-1, 5, 5)
-    sharp = (b5.min(axis=(1, 2)) < 1e-3) & (b5.max(axis=(1, 2)) > 1 - 1e-3)
-    inr = (np.abs(t) >= 1 / 64) & (np.abs(t) <= 0.5)
-    for name, m in (("sharp", sharp & inr), ("diffuse", ~sharp & inr)):
-        el = np.mean(np.abs(y[m] - t[m]) / np.abs(t[m]))
-        ep = np.mean(np.abs(plain[m] - t[m]) / np.abs(t[m]))
-        print(f"{name} stencils ({m.sum()}): relative error learned {el:.3f}, plain {ep:.3f}", flush=True)
-        if name == "sharp":
-            assert el < ep / 5
+    slope = 0.5 * 2.0 ** omlp.uniform(99, np.arange(16384, dtype=np.uint64), 8)
+    for name, m in (("slope 1/2 .. 0.6 (initial condition)", slope < 0.6), ("all slopes", slope > 0)):
+        el = np.mean(np.abs(y[m] - lab[m]) / np.abs(lab[m]))
+        ep = np.mean(np.abs(plain[m] - lab[m]) / np.abs(lab[m]))
+        print(f"{name} ({m.sum()} stencils): relative error learned {el:.4f}, plain {ep:.4f}", flush=True)
+        assert el < 0.8 * ep
 
 
-def sharp_drop_vof(n, r, sub=16):
-    """Gas fraction of each cell inside a circle of radius r at (1/2, 1/2),
-    by sub x sub point sampling, with the two wall columns of the NPY layout."""
-    xc = (np.arange(n) + 0.5) / n
-    s = (np.arange(sub) + 0.5) / sub / n - 0.5 / n
-    X = xc[None, :, None, None] + s[None, None, None, :]
-    Y = xc[:, None, None, None] + s[None, None, :, None]
-    return np.pad(((X - 0.5) ** 2 + (Y - 0.5) ** 2 < r * r).mean(axis=(2, 3)), ((0, 0), (1, 1)))
-
-
-def drop(product, net, learned, vof, steps):
-    w = configs.static_drop(256, steps)
+def mixed(product, net, learned):
     assert product.lib.sbfs_set_precision(1) == 0
     net.use(learned)
+
+
+def restore(product, net):
+    net.use(False)
+    assert product.lib.sbfs_set_precision(0) == 0
+
+
+def test_network_takes_the_interface_band(product, trained):
+    net, _ = trained
+    w = configs.static_drop(256, 1)
     g = solver.Case(product, w.nx, w.ny, w.lx, w.ly, w.Re, w.Fr, w.denr, w.visr, w.We)
     try:
-        g.load_initial_condition(vof)
-        for _ in range(steps):
-            g.integrate()
-        return g.get("ux"), g.get("uy"), g.diagnostics()
+        g.load_initial_condition(w.vof())
+        cover = net.coverage(g)
+        band = np.mean(np.abs(w.vof()[:, 1:-1] - 0.5) < 0.48)
+        print(f"static drop: learned cells {cover:.4f} of the grid, cut cells {band:.4f}", flush=True)
+        assert cover >= band
     finally:
         g.close()
-        net.use(False)
-        product.lib.sbfs_set_precision(0)
 
 
-def test_sharp_drop_first_step_imbalance(product, trained):
+def drop_history(product, net, learned, steps, marks):
+    w = configs.static_drop(256, steps)
+    mixed(product, net, learned)
+    g = solver.Case(product, w.nx, w.ny, w.lx, w.ly, w.Re, w.Fr, w.denr, w.visr, w.We)
+    try:
+        g.load_initial_condition(w.vof())
+        v0 = g.diagnostics()["volume"]
+        out = {}
+        for s in range(1, steps + 1):
+            g.integrate()
+            if s in marks:
+                out[s] = max(np.abs(g.get("ux")).max(), np.abs(g.get("uy")).max())
+        d = g.diagnostics()
+        return out, abs(d["volume"] - v0) / v0
+    finally:
+        g.close()
+        restore(product, net)
+
+
+def test_static_drop_spurious_currents(product, trained):
+    """The static drop of radius 32 cells on 256 x 256 with the initial
+    condition's tanh profile, no gravity, 5000 steps: the largest velocity
+    is the spurious current of the surface-tension model."""
     net, _ = trained
-    vof = sharp_drop_vof(256, 32 / 256)
-    umax = {}
-    for learned in (False, True):
-        ux, uy, d = drop(product, net, learned, vof, 1)
-        umax[learned] = max(np.abs(ux).max(), np.abs(uy).max())
-    print(f"sharp static drop, max |u| after one step: plain {umax[False]:.3e}, learned {umax[True]:.3e}",
-          flush=True)
-    assert umax[True] < umax[False] / 3
+    marks = (1, 100, 1000, 3000, 5000)
+    plain, dp = drop_history(product, net, False, 5000, marks)
+    learned, dl = drop_history(product, net, True, 5000, marks)
+    for s in marks:
+        print(f"static drop step {s}: max |u| plain {plain[s]:.3e}, learned {learned[s]:.3e}", flush=True)
+    print(f"volume drift plain {dp:.2e}, learned {dl:.2e}", flush=True)
+    assert dl <= 1e-10
+    for s in (1000, 3000, 5000):
+        assert learned[s] <= plain[s], s
 
 
-def test_wide_profile_keeps_plain_curvature(product, trained):
+def rise_history(product, net, learned, use_mixed, steps):
+    w = configs.rising_bubble(128, 256, steps)
+    if use_mixed:
+        mixed(product, net, learned)
+    g = solver.Case(product, w.nx, w.ny, w.lx, w.ly, w.Re, w.Fr, w.denr, w.visr, w.We)
+    try:
+        g.load_initial_condition(w.vof())
+        v0 = g.diagnostics()["volume"]
+        ur, drift = [], 0.0
+        for _ in range(steps):
+            g.integrate()
+            d = g.diagnostics()
+            ur.append(d["rise_velocity"])
+            drift = max(drift, abs(d["volume"] - v0) / v0)
+        return np.array(ur), drift
+    finally:
+        g.close()
+        if use_mixed:
+            restore(product, net)
+
+
+def test_rising_bubble_learned_within_one_percent(product, trained):
+    """The repository's default case (128 x 256, 2000 steps) in mixed
+    precision with the learned curvature against the plain FP64 step."""
     net, _ = trained
-    vof = ic.bubble_vof(256, 256, 1.0, 1.0, 0.5, 0.5, 32 / 256)
-    a = drop(product, net, False, vof, 200)
-    b = drop(product, net, True, vof, 200)
-    print(f"wide-profile static drop, max |u| after 200 steps: {np.abs(a[0]).max():.3e}", flush=True)
-    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    ref, _ = rise_history(product, net, False, False, 2000)
+    got, drift = rise_history(product, net, True, True, 2000)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"rising bubble 2000 steps: rise-velocity history error {err:.3e}, volume drift {drift:.2e}", flush=True)
+    assert err <= 0.01
+    assert drift <= 1e-10
 
 
 def test_learned_steps_after_a_larger_case(product, trained):
@@ -131,18 +181,17 @@ def test_learned_steps_after_a_larger_case(product, trained):
 
     def run(n, steps):
         w = configs.static_drop(n, steps, radius_cells=n / 8)
-        assert product.lib.sbfs_set_precision(1) == 0
-        net.use(True)
+        mixed(product, net, True)
         g = solver.Case(product, w.nx, w.ny, w.lx, w.ly, w.Re, w.Fr, w.denr, w.visr, w.We)
         try:
-            g.load_initial_condition(sharp_drop_vof(n, 1 / 8))
+            g.load_initial_condition(w.vof())
+            assert net.coverage(g) > 0
             for _ in range(steps):
                 g.integrate()
             return g.get("ux"), g.get("uy"), g.get("vof")
         finally:
             g.close()
-            net.use(False)
-            product.lib.sbfs_set_precision(0)
+            restore(product, net)
 
     first = run(64, 12)
     run(256, 4)
