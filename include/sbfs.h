@@ -25,6 +25,21 @@ extern int sbfs_synchronize(void);
 /* name of the device the library runs on (e.g. "AMD Instinct MI355X") */
 extern int sbfs_device_name(char * buf, size_t len);
 
+/*
+ * decide_dt (include/decide_dt.h) without any device allocation, for
+ * callers that cannot allocate: code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  No counterpart in the reference.
+ * The reduction runs through a fixed 66 KB device array of the library
+ * instead of buffers allocated with the domain; the result is bitwise the
+ * same as decide_dt's.  The domain must already have its device context
+ * (fluid_init, interface_init, sbfs_domain_create or one decide_dt call
+ * create it); otherwise, or for a domain wider than about 2 million
+ * columns, the call fails.  All calls on one GPU share that array: calls
+ * that overlap on one GPU (several host threads) race.
+ */
+extern int decide_dt_noalloc(const domain_t * domain, const fluid_t * fluid,
+                             const interface_t * interface, double * dt);
+
 /* safety factors of the three time-step limits (overrides the environment
  * variables coef_dt_adv, coef_dt_dif, coef_dt_int) */
 extern int sbfs_set_dt_coefs(double adv, double dif, double intf);

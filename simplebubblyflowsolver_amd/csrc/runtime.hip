@@ -134,6 +134,7 @@ int init_runtime() {
   }
   const char* p = getenv("SBFS_PATH");
   if (p) g_opt.path = atoi(p);
+  if (noalloc_setup()) return 1;
   g_ready = true;
   return 0;
 }
@@ -251,6 +252,17 @@ DomainCtx* domain_ctx(const domain_t* d) {
     }
   if (!placed) g_domains.push_back({d, c});
   return c;
+}
+
+DomainCtx* domain_ctx_find(const domain_t* d) {
+  if (!g_ready || !d) return nullptr;
+  const uint64_t h = domain_hash(d);
+  std::lock_guard<std::mutex> lock(g_mutex);
+  for (auto& e : g_domains)
+    if (e.first == d && e.second && e.second->hash == h && e.second->nx == d->mysizes[0] &&
+        e.second->ny == d->mysizes[1])
+      return e.second;
+  return nullptr;
 }
 
 static void forget_domain(const domain_t* d) {

@@ -204,6 +204,15 @@ int poisson_solve(const domain_t* d, DomainCtx* c, double dt_old,
                   double dt_new, fluid_t* f);
 int compute_dt_device(const domain_t* d, DomainCtx* c, const fluid_t* f,
                       const interface_t* s);
+int dt_grid(int nx, int ny, int capacity, int* rows, int* nblk);
+int dt_launch(const domain_t* d, const DomainCtx* c, const fluid_t* f, const interface_t* s,
+              int capacity, double* partial, double* dt_out);
+// the domain's device context if one exists and is current, without
+// creating or refreshing one (nullptr otherwise)
+DomainCtx* domain_ctx_find(const domain_t* d);
+// the no-allocation variants' per-device set-up (noalloc.hip), run by
+// init_runtime for the current device
+int noalloc_setup();
 void dt_limits(const domain_t* d, const fluid_t* f, const interface_t* s,
                double* dif0, double* dif1, double* dint);
 

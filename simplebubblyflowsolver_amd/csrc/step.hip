@@ -96,8 +96,8 @@ int dt_args(const domain_t* d, DomainCtx* c, const fluid_t* f, const interface_t
   if (load_dt_coefs()) return 1;
   const int nx = (int)c->nx, ny = (int)c->ny;
   dt_limits(d, f, s, &a->dif0, &a->dif1, &a->dint);
-  int rows = 16;
-  while ((long)cdiv(nx, kRedThreads) * cdiv(ny, rows) > kRedSlots - 64) rows *= 2;
+  int rows = 0, nblk = 0;
+  if (dt_grid(nx, ny, kRedSlots - 64, &rows, &nblk)) return 1;
   a->ux = (const double*)f->ux.data;
   a->uy = (const double*)f->uy.data;
   a->hxxf = c->hxxf;
@@ -136,25 +136,46 @@ void dt_limits(const domain_t* d, const fluid_t* f, const interface_t* s, double
   *dint = di;
 }
 
-int compute_dt_device(const domain_t* d, DomainCtx* c, const fluid_t* f,
-                      const interface_t* s) {
+// Rows per partial-reduction workgroup: 16, doubled until the partial
+// minima fit `capacity` slots.  Fails (instead of looping on) a domain so
+// wide that one workgroup row alone needs more slots.
+int dt_grid(int nx, int ny, int capacity, int* rows, int* nblk) {
+  const long gx = cdiv(nx, kRedThreads);
+  int r = 16;
+  while (gx * cdiv(ny, r) > capacity && r < ny) r *= 2;
+  const long n = gx * cdiv(ny, r);
+  SBFS_REQUIRE(n <= capacity, "decide_dt: %d columns need %ld partial minima, more than the %d slots", nx, n,
+               capacity);
+  *rows = r;
+  *nblk = (int)n;
+  return 0;
+}
+
+// The dt reduction of one case into `partial` (at least dt_grid's nblk
+// doubles) and `dt_out` (one double), on the library stream.
+int dt_launch(const domain_t* d, const DomainCtx* c, const fluid_t* f, const interface_t* s,
+              int capacity, double* partial, double* dt_out) {
   if (load_dt_coefs()) return 1;
   const int nx = (int)c->nx, ny = (int)c->ny;
   double dif0, dif1, dint;
   dt_limits(d, f, s, &dif0, &dif1, &dint);
-  int rows = 16;
-  while ((long)cdiv(nx, kRedThreads) * cdiv(ny, rows) > kRedSlots - 64) rows *= 2;
+  int rows = 0, nblk = 0;
+  if (dt_grid(nx, ny, capacity, &rows, &nblk)) return 1;
   const dim3 grd(cdiv(nx, kRedThreads), cdiv(ny, rows));
-  const int nblk = (int)(grd.x * grd.y);
   SBFS_TIMED("k_dt_partial",
              hipLaunchKernelGGL(k_dt_partial, grd, dim3(kRedThreads), 0, stream(),
                                 (const double*)f->ux.data, (const double*)f->uy.data, c->hxxf,
-                                c->hy, nx, ny, rows, c->red));
+                                c->hy, nx, ny, rows, partial));
   SBFS_TIMED("k_dt_final",
              hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(kRedThreads), 0, stream(),
-                                (const double*)c->red, nblk, options().cadv, dif0, dif1, dint,
-                                c->dt_dev));
+                                (const double*)partial, nblk, options().cadv, dif0, dif1, dint,
+                                dt_out));
   return check_launch("decide_dt");
+}
+
+int compute_dt_device(const domain_t* d, DomainCtx* c, const fluid_t* f,
+                      const interface_t* s) {
+  return dt_launch(d, c, f, s, kRedSlots - 64, c->red, c->dt_dev);
 }
 
 // ---------------------------------------------------------------------------
