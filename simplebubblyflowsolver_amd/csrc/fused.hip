@@ -1,22 +1,19 @@
 // Fused integrator: per Runge-Kutta stage
 //
-//   k_pre        one LDS tile of cells per workgroup: corner gradients,
-//                normals, curvature, THINC fluxes, mass fluxes, the vof
-//                update, surface force, stress tensor, explicit sources and
-//                the predicted velocity (every stage of the reference from
-//                fluid_compute_density to fluid_predict_field).  Halo values
-//                are recomputed per tile instead of being stored, so only
-//                vof, u, p and the beta sources are read and only the new
-//                vof, the predicted u and the alpha sources are written.
-//   k_dct2_rhs_fused  Poisson right-hand side + DCT-II per row
+//   k_march*     the marching stage kernel (march.hip): every stage of the
+//                reference from fluid_compute_density to
+//                fluid_predict_field and the Poisson right-hand side, one
+//                wavefront per strip of columns walking a band of rows;
+//                from stage 1 on it also applies the previous stage's
+//                projection and pressure update (k_march*_corr)
+//   k_dct2_rows  DCT-II per row of the right-hand side
 //   k_part_*     partitioned tridiagonal solve (tdm_partition.hip)
 //   k_dct3_psi   DCT-III per row -> psi[1]
-//   k_post       projection of u and the pressure update
+//   k_post       projection of u and the pressure update of the last stage
 //
 // Every value is computed with the same device functions and operation
 // order as the staged kernels (stencils.hpp), so the fused path reproduces
-// the staged path bitwise (up to the sign of exact zeros; see k_pre's beta
-// term at stage 0).  dt stays in device memory for the whole step.
+// the staged path bitwise.  dt stays in device memory for the whole step.
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -47,382 +44,6 @@ namespace sbfs {
 // the same bits as the reference's left-to-right `a / b * x`, so the fused
 // kernels keep bitwise parity while only data-dependent divisions remain.
 
-struct PreArgs {
-  // inputs (state at the start of the stage)
-  const double* vof;
-  const double* ux;
-  const double* uy;
-  const double* p;
-  const double* srcb_vof;
-  const double* srcb_ux;
-  const double* srcb_uy;
-  // outputs
-  double* vofn;
-  double* uxs;
-  double* uys;
-  double* srca_vof;
-  double* srca_ux;
-  double* srca_uy;
-  double* den1;  // written on the last stage only (else null)
-  double* txy;   // written on the last stage only (else null)
-  // metrics
-  const double* hxxf;
-  const double* hxxc;
-  const double* jdxf;
-  const double* jdxc;
-  double hy;
-  Coef cf;
-  // physics
-  double denr, dlo, dhi, visr, vlo, vhi;
-  double refdeninv, tension, diffusivity, grav;
-  // time stepping
-  const double* dt;
-  double ca, cb, cg;
-  int stage;
-  int nx, ny;
-};
-
-// periodic image of a row index within a few rows of [1, ny]
-__device__ __forceinline__ int wrap_row(int j, int ny) {
-  return j < 1 ? j + ny : (j > ny ? j - ny : j);
-}
-
-// tile of TX x TY output cells; regions (inclusive, relative to i0, j0):
-//   vof     [i0-3, i0+TX+1] x [j0-3, j0+TY+1]
-//   corner  [i0-2, i0+TX+1] x [j0-2, j0+TY+1]
-//   normal  [i0-2, i0+TX  ] x [j0-2, j0+TY  ]
-//   curv    [i0-1, i0+TX-1] x [j0-1, j0+TY-1]
-//   fx      [i0-1, i0+TX  ] x [j0-1, j0+TY-1]   (vof flux and mass flux)
-//   fy      [i0-1, i0+TX-1] x [j0-1, j0+TY  ]
-//   vofn    [i0-1, i0+TX-1] x [j0-1, j0+TY-1]   (new vof and its source)
-//   txx     [i0-1, i0+TX-1] x [j0  , j0+TY-1]
-//   tyy     [i0  , i0+TX-1] x [j0-1, j0+TY-1]
-//   txy     [i0  , i0+TX  ] x [j0  , j0+TY  ]
-template <int TX, int TY>
-struct PreTile {
-  static constexpr int VX = TX + 5, VY = TY + 5;
-  static constexpr int CX = TX + 4, CY = TY + 4;
-  static constexpr int NX = TX + 3, NY = TY + 3;
-  static constexpr int KX = TX + 1, KY = TY + 1;
-  static constexpr int FXX = TX + 2, FXY = TY + 1;
-  static constexpr int FYX = TX + 1, FYY = TY + 2;
-  static constexpr int WX = TX + 1, WY = TY + 1;
-  static constexpr int TXXX = TX + 1, TXXY = TY;
-  static constexpr int TYYX = TX, TYYY = TY + 1;
-  static constexpr int TXYX = TX + 1, TXYY = TY + 1;
-};
-
-template <int TX, int TY>
-__global__ __launch_bounds__(TX* TY) void k_pre(PreArgs a) {
-  using T = PreTile<TX, TY>;
-  // phase A arrays (vof, corners, normals, curvature)
-  __shared__ double s_vof[T::VY][T::VX];
-  __shared__ double s_cx[T::CY][T::CX];
-  __shared__ double s_cy[T::CY][T::CX];
-  __shared__ double s_n[3][T::NY][T::NX];
-  __shared__ double s_curv[T::KY][T::KX];
-  // phase B arrays
-  __shared__ double s_fx[T::FXY][T::FXX];
-  __shared__ double s_mfx[T::FXY][T::FXX];
-  __shared__ double s_fy[T::FYY][T::FYX];
-  __shared__ double s_mfy[T::FYY][T::FYX];
-  __shared__ double s_vofn[T::WY][T::WX];
-  __shared__ double s_src[T::WY][T::WX];
-  __shared__ double s_txx[T::TXXY][T::TXXX];
-  __shared__ double s_tyy[T::TYYY][T::TYYX];
-  __shared__ double s_txy[T::TXYY][T::TXYX];
-
-  const int nx = a.nx, ny = a.ny;
-  const int i0 = 1 + blockIdx.x * TX;
-  const int j0 = 1 + blockIdx.y * TY;
-  const int tid = threadIdx.x + threadIdx.y * TX;
-  constexpr int NT = TX * TY;
-  const double hy = a.hy;
-  auto UX = [&](int i, int j) { return a.ux[L::xf(i, wrap_row(j, ny), nx)]; };
-  auto UY = [&](int i, int j) { return a.uy[L::cc(i, wrap_row(j, ny), nx)]; };
-  auto VOF = [&](int i, int j) -> double { return s_vof[j - (j0 - 3)][i - (i0 - 3)]; };
-  auto CRN = [&](int i, int j) -> V2 {
-    return {s_cx[j - (j0 - 2)][i - (i0 - 2)], s_cy[j - (j0 - 2)][i - (i0 - 2)]};
-  };
-  auto NRM = [&](int i, int j) -> V3 {
-    const int y = j - (j0 - 2), x = i - (i0 - 2);
-    return {s_n[0][y][x], s_n[1][y][x], s_n[2][y][x]};
-  };
-  auto CURV = [&](int i, int j) -> double { return s_curv[j - (j0 - 1)][i - (i0 - 1)]; };
-  auto FX = [&](int i, int j) -> double { return s_fx[j - (j0 - 1)][i - (i0 - 1)]; };
-  auto MFX = [&](int i, int j) -> double { return s_mfx[j - (j0 - 1)][i - (i0 - 1)]; };
-  auto FY = [&](int i, int j) -> double { return s_fy[j - (j0 - 1)][i - (i0 - 1)]; };
-  auto MFY = [&](int i, int j) -> double { return s_mfy[j - (j0 - 1)][i - (i0 - 1)]; };
-  auto VOFN = [&](int i, int j) -> double { return s_vofn[j - (j0 - 1)][i - (i0 - 1)]; };
-  auto TXX = [&](int i, int j) -> double { return s_txx[j - j0][i - (i0 - 1)]; };
-  auto TYY = [&](int i, int j) -> double { return s_tyy[j - (j0 - 1)][i - i0]; };
-  auto TXY = [&](int i, int j) -> double { return s_txy[j - j0][i - i0]; };
-  auto DEN0 = [&](int i, int j) { return mix_clamp(a.denr, a.dlo, a.dhi, VOF(i, j)); };
-  auto DEN1 = [&](int i, int j) { return mix_clamp(a.denr, a.dlo, a.dhi, VOFN(i, j)); };
-  // viscosity at x faces / y faces (src/fluid/viscosity.c:37-83)
-  auto VISUX = [&](int i, int j) {
-    double v;
-    if (i == 1) v = VOF(0, j);
-    else if (i == nx + 1) v = VOF(nx + 1, j);
-    else v = 0.5 * VOF(i - 1, j) + 0.5 * VOF(i, j);
-    return mix_clamp(a.visr, a.vlo, a.vhi, v);
-  };
-  auto VISUY = [&](int i, int j) {
-    return mix_clamp(a.visr, a.vlo, a.vhi, 0.5 * VOF(i, j - 1) + 0.5 * VOF(i, j));
-  };
-
-  // --- vof tile ----------------------------------------------------------
-  for (int k = tid; k < T::VX * T::VY; k += NT) {
-    const int x = k % T::VX, y = k / T::VX;
-    const int i = i0 - 3 + x, j = j0 - 3 + y;
-    s_vof[y][x] = (i >= 0 && i <= nx + 1) ? a.vof[L::vof(i, wrap_row(j, ny), nx)] : 0.;
-  }
-  __syncthreads();
-  // --- corner unit gradients (curvature_tensor.c:16-49) --------------------
-  for (int k = tid; k < T::CX * T::CY; k += NT) {
-    const int x = k % T::CX, y = k / T::CX;
-    const int i = i0 - 2 + x, j = j0 - 2 + y;
-    V2 g = {0., 0.};
-    if (i >= 1 && i <= nx + 1) {
-      const double mm = VOF(i - 1, j - 1), pm = VOF(i, j - 1), mp = VOF(i - 1, j), pp = VOF(i, j);
-      if (!(mm == pm && mm == mp && mm == pp)) {
-        // curvature_tensor.c:29-45
-        const double gx = a.cf.rhf[i] * (-mm + pm - mp + pp);
-        const double gy = a.cf.rhy * (-mm - pm + mp + pp);
-        const double norm = sqrt(gx * gx + gy * gy);
-        const double ninv = 1. / fmax(norm, DBL_EPSILON);
-        g = {gx * ninv, gy * ninv};
-      }
-    }
-    s_cx[y][x] = g.x;
-    s_cy[y][x] = g.y;
-  }
-  __syncthreads();
-  // --- normals inside the band (:51-97) and curvature (:100-135) ----------
-  for (int k = tid; k < T::NX * T::NY; k += NT) {
-    const int x = k % T::NX, y = k / T::NX;
-    const int i = i0 - 2 + x, j = j0 - 2 + y;
-    V3 n = {0., 0., 0.};
-    if (i >= 1 && i <= nx) {
-      const double v = VOF(i, j);
-      if (in_band(v)) n = cell_normal(CRN(i, j), CRN(i + 1, j), CRN(i, j + 1), CRN(i + 1, j + 1), a.hxxc[i - 1], hy, v);
-    }
-    s_n[0][y][x] = n.x;
-    s_n[1][y][x] = n.y;
-    s_n[2][y][x] = n.z;
-  }
-  for (int k = tid; k < T::KX * T::KY; k += NT) {
-    const int x = k % T::KX, y = k / T::KX;
-    const int i = i0 - 1 + x, j = j0 - 1 + y;
-    double c = 0.;
-    if (i >= 1 && i <= nx) {
-      // curvature_tensor.c:119-131
-      const V2 c00 = CRN(i, j), c10 = CRN(i + 1, j), c01 = CRN(i, j + 1), c11 = CRN(i + 1, j + 1);
-      const double nxm = 0.5 * c00.x + 0.5 * c01.x;
-      const double nxp = 0.5 * c10.x + 0.5 * c11.x;
-      const double nym = 0.5 * c00.y + 0.5 * c10.y;
-      const double nyp = 0.5 * c01.y + 0.5 * c11.y;
-      const double div = a.cf.rjc[i] * (-a.cf.qf[i] * nxm + a.cf.qf[i + 1] * nxp - a.cf.cyc[i] * nym + a.cf.cyc[i] * nyp);
-      c = -1. * div;
-    }
-    s_curv[y][x] = c;
-  }
-  __syncthreads();
-  // --- THINC fluxes and mass fluxes (update/fluxx.c, fluxy.c, mass_flux.c)
-  for (int k = tid; k < T::FXX * T::FXY; k += NT) {
-    const int x = k % T::FXX, y = k / T::FXX;
-    const int i = i0 - 1 + x, j = j0 - 1 + y;
-    double f = 0., mf = 0.;
-    if (i >= 1 && i <= nx + 1) {
-      const double vel = UX(i, j);
-      if (i != 1 && i != nx + 1) {
-        const int ii = vel < 0. ? i : i - 1;
-        const double v = VOF(ii, j);
-        f = thinc_flux_x(vel, v, in_band(v) ? NRM(ii, j) : V3{0., 0., 0.});
-      }
-      mf = vel + (a.denr - 1.) * f;
-    }
-    s_fx[y][x] = f;
-    s_mfx[y][x] = mf;
-  }
-  for (int k = tid; k < T::FYX * T::FYY; k += NT) {
-    const int x = k % T::FYX, y = k / T::FYX;
-    const int i = i0 - 1 + x, j = j0 - 1 + y;
-    double f = 0., mf = 0.;
-    if (i >= 0 && i <= nx + 1) {
-      const double vel = UY(i, j);
-      if (i != 0 && i != nx + 1) {
-        const int jj = vel < 0. ? j : j - 1;
-        const double v = VOF(i, jj);
-        f = thinc_flux_y(vel, v, in_band(v) ? NRM(i, jj) : V3{0., 0., 0.});
-      }
-      mf = vel + (a.denr - 1.) * f;
-    }
-    s_fy[y][x] = f;
-    s_mfy[y][x] = mf;
-  }
-  // --- stress tensor (predict/txx.c, txy.c, tyy.c) -------------------------
-  for (int k = tid; k < T::TXXX * T::TXXY; k += NT) {
-    const int x = k % T::TXXX, y = k / T::TXXX;
-    const int i = i0 - 1 + x, j = j0 + y;
-    double t = 0.;
-    if (i >= 1 && i <= nx) {
-      const double vx = 0.5 * VISUX(i, j) + 0.5 * VISUX(i + 1, j);
-      const double l = a.cf.rhc[i] * (-UX(i, j) + UX(i + 1, j));
-      t = vx * l + vx * l;
-    }
-    s_txx[y][x] = t;
-  }
-  for (int k = tid; k < T::TYYX * T::TYYY; k += NT) {
-    const int x = k % T::TYYX, y = k / T::TYYX;
-    const int i = i0 + x, j = j0 - 1 + y;
-    double t = 0.;
-    if (i <= nx) {
-      const double vy = 0.5 * VISUY(i, j) + 0.5 * VISUY(i, j + 1);
-      const double l = a.cf.rhy * (-UY(i, j) + UY(i, j + 1));
-      t = vy * l + vy * l;
-    }
-    s_tyy[y][x] = t;
-  }
-  for (int k = tid; k < T::TXYX * T::TXYY; k += NT) {
-    const int x = k % T::TXYX, y = k / T::TXYX;
-    const int i = i0 + x, j = j0 + y;
-    double t = 0.;
-    if (i <= nx + 1) {
-      const double vx = 0.5 * VISUX(i, j - 1) + 0.5 * VISUX(i, j);
-      const double vy = 0.5 * VISUY(i - 1, j) + 0.5 * VISUY(i, j);
-      const double lxy = a.cf.rhy * (-UX(i, j - 1) + UX(i, j));
-      const double lyx = a.cf.rhf[i] * (-UY(i - 1, j) + UY(i, j));
-      t = vx * lxy + vy * lyx;
-    }
-    s_txy[y][x] = t;
-  }
-  __syncthreads();
-  // --- vof update on the tile and its one-cell skirt (update/main.c:54-120)
-  const double dt = *a.dt;
-  const double dtca = dt * a.ca, dtcb = dt * a.cb;
-  for (int k = tid; k < T::WX * T::WY; k += NT) {
-    const int x = k % T::WX, y = k / T::WX;
-    const int i = i0 - 1 + x, j = j0 - 1 + y;
-    double v = 0., s = 0.;
-    if (i >= 1 && i <= nx) {
-      // update/main.c:70-84
-      s = a.cf.rjc[i] * (+a.cf.qf[i] * FX(i, j) - a.cf.qf[i + 1] * FX(i + 1, j) + a.cf.cyc[i] * FY(i, j) -
-                         a.cf.cyc[i] * FY(i, j + 1));
-      v = VOF(i, j);
-      v += dtca * s;
-      if (a.stage != 0) v += dtcb * a.srcb_vof[L::ci(i, wrap_row(j, ny), nx)];
-    }
-    s_vofn[y][x] = v;
-    s_src[y][x] = s;
-  }
-  __syncthreads();
-
-  // --- outputs of this thread's cell --------------------------------------
-  const int i = i0 + (int)threadIdx.x, j = j0 + (int)threadIdx.y;
-  if (i > nx || j > ny) return;
-  auto ivof = [nx](int p, int q) { return L::vof(p, q, nx); };
-  auto icc = [nx](int p, int q) { return L::cc(p, q, nx); };
-  auto ixf = [nx](int p, int q) { return L::xf(p, q, nx); };
-  {
-    const double v = VOFN(i, j);
-    store_with_halo(a.vofn, i, j, ny, 2, v, ivof);
-    if (i == 1) store_with_halo(a.vofn, 0, j, ny, 2, 0., ivof);
-    if (i == nx) store_with_halo(a.vofn, nx + 1, j, ny, 2, 0., ivof);
-    if (a.srca_vof) a.srca_vof[L::ci(i, j, nx)] = s_src[j - (j0 - 1)][i - (i0 - 1)];
-    if (a.den1) {
-      store_with_halo(a.den1, i, j, ny, 1, DEN1(i, j), icc);
-      const double dg = mix_clamp(a.denr, a.dlo, a.dhi, 0.);  // vof = 0 in the wall ghosts
-      if (i == 1) store_with_halo(a.den1, 0, j, ny, 1, dg, icc);
-      if (i == nx) store_with_halo(a.den1, nx + 1, j, ny, 1, dg, icc);
-    }
-    if (a.txy) {
-      store_with_halo(a.txy, i, j, ny, 1, TXY(i, j), ixf);
-      if (i == nx) store_with_halo(a.txy, nx + 1, j, ny, 1, TXY(nx + 1, j), ixf);
-    }
-  }
-  // ux at face i (predict/ux.c:29-273)
-  {
-    const Coef& cf = a.cf;
-    double u = 0.;
-    if (i >= 2) {
-      double s = 0.;
-      {
-        const double mm = cf.eqf[i - 1] * MFX(i - 1, j) + cf.eqf[i] * MFX(i, j);
-        const double mp = cf.eqf[i] * MFX(i, j) + cf.eqf[i + 1] * MFX(i + 1, j);
-        const double um = 0.5 * UX(i - 1, j) + 0.5 * UX(i, j);
-        const double up = 0.5 * UX(i, j) + 0.5 * UX(i + 1, j);
-        s -= cf.rjf[i] * (-mm * um + mp * up);
-      }
-      {
-        const double mm = cf.ecy[i - 1] * MFY(i - 1, j) + cf.ecy[i] * MFY(i, j);
-        const double mp = cf.ecy[i - 1] * MFY(i - 1, j + 1) + cf.ecy[i] * MFY(i, j + 1);
-        const double um = 0.5 * UX(i, j - 1) + 0.5 * UX(i, j);
-        const double up = 0.5 * UX(i, j) + 0.5 * UX(i, j + 1);
-        s -= cf.rjf[i] * (-mm * um + mp * up);
-      }
-      s += cf.dfx[i] * (-cf.qc[i - 1] * TXX(i - 1, j) + cf.qc[i] * TXX(i, j));
-      s += cf.dfx[i] * (-cf.cyf[i] * TXY(i, j) + cf.cyf[i] * TXY(i, j + 1));
-      double g = 0.;
-      g -= cf.rhf[i] * (-a.p[icc(i - 1, j)] + a.p[icc(i, j)]);
-      {
-        // force.c:37-49
-        const double d = 0.5 * DEN0(i - 1, j) + 0.5 * DEN0(i, j);
-        const double k = 0.5 * CURV(i - 1, j) + 0.5 * CURV(i, j);
-        const double delta = cf.rhf[i] * (-VOF(i - 1, j) + VOF(i, j));
-        s += d * a.refdeninv * a.tension * k * delta;
-      }
-      s += a.grav * (0.5 * DEN0(i - 1, j) + 0.5 * DEN0(i, j));
-      if (a.srca_ux) a.srca_ux[L::fx(i, j, nx)] = s;
-      const double sb = a.stage != 0 ? a.srcb_ux[L::fx(i, j, nx)] : 0.;
-      const double d0 = 0.5 * DEN0(i - 1, j) + 0.5 * DEN0(i, j);
-      u = d0 * UX(i, j) + a.ca * dt * s + a.cb * dt * sb + a.cg * dt * g;
-      const double d1 = 0.5 * DEN1(i - 1, j) + 0.5 * DEN1(i, j);
-      u /= d1;
-    }
-    store_with_halo(a.uxs, i, j, ny, 1, u, ixf);
-    if (i == nx) store_with_halo(a.uxs, nx + 1, j, ny, 1, 0., ixf);
-  }
-  // uy at face j (predict/uy.c:29-251)
-  {
-    const Coef& cf = a.cf;
-    double s = 0.;
-    {
-      const double mm = cf.eqf[i] * MFX(i, j - 1) + cf.eqf[i] * MFX(i, j);
-      const double mp = cf.eqf[i + 1] * MFX(i + 1, j - 1) + cf.eqf[i + 1] * MFX(i + 1, j);
-      const double um = 0.5 * UY(i - 1, j) + 0.5 * UY(i, j);
-      const double up = 0.5 * UY(i, j) + 0.5 * UY(i + 1, j);
-      s -= cf.rjc[i] * (-mm * um + mp * up);
-    }
-    {
-      const double mm = cf.ecy[i] * MFY(i, j - 1) + cf.ecy[i] * MFY(i, j);
-      const double mp = cf.ecy[i] * MFY(i, j) + cf.ecy[i] * MFY(i, j + 1);
-      const double um = 0.5 * UY(i, j - 1) + 0.5 * UY(i, j);
-      const double up = 0.5 * UY(i, j) + 0.5 * UY(i, j + 1);
-      s -= cf.rjc[i] * (-mm * um + mp * up);
-    }
-    s += cf.dcx[i] * (-cf.qf[i] * TXY(i, j) + cf.qf[i + 1] * TXY(i + 1, j));
-    s += cf.dcx[i] * (-cf.cyc[i] * TYY(i, j - 1) + cf.cyc[i] * TYY(i, j));
-    double g = 0.;
-    g -= cf.rhy * (-a.p[icc(i, j - 1)] + a.p[icc(i, j)]);
-    {
-      // force.c:72-84
-      const double d = 0.5 * DEN0(i, j - 1) + 0.5 * DEN0(i, j);
-      const double k = 0.5 * CURV(i, j - 1) + 0.5 * CURV(i, j);
-      const double delta = cf.rhy * (-VOF(i, j - 1) + VOF(i, j));
-      s += d * a.refdeninv * a.tension * k * delta;
-    }
-    if (a.srca_uy) a.srca_uy[L::ci(i, j, nx)] = s;
-    const double sb = a.stage != 0 ? a.srcb_uy[L::ci(i, j, nx)] : 0.;
-    const double d0 = 0.5 * DEN0(i, j - 1) + 0.5 * DEN0(i, j);
-    double u = d0 * UY(i, j) + a.ca * dt * s + a.cb * dt * sb + a.cg * dt * g;
-    const double d1 = 0.5 * DEN1(i, j - 1) + 0.5 * DEN1(i, j);
-    u /= d1;
-    store_with_halo(a.uys, i, j, ny, 1, u, icc);
-    if (i == 1) store_with_halo(a.uys, 0, j, ny, 1, kUyXm, icc);
-    if (i == nx) store_with_halo(a.uys, nx + 1, j, ny, 1, kUyXp, icc);
-  }
-}
 // ---------------------------------------------------------------------------
 // projection + pressure (correct_velocity/ux.c, uy.c; update_pressure.c)
 // ---------------------------------------------------------------------------
@@ -483,8 +104,6 @@ __global__ __launch_bounds__(256) void k_post_batch(const PostArgs* __restrict__
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-constexpr int kTX = 64, kTY = 4;
-
 struct FusedScratch {
   size_t nx = 0, ny = 0;
   double* vof = nullptr;
@@ -620,9 +239,7 @@ static int coef_tables(const domain_t* d, const DomainCtx* c, double Re, Coef* o
 
 int poisson_transform_fused(const domain_t* d, DomainCtx* c, fluid_t* f, double** rhs);  // poisson.hip
 
-int poisson_solve_fused(const domain_t* d, DomainCtx* c, const double* dt_dev, double g_old,
-                        double g_new, const double* uxs, const double* uys, const double* vofn,
-                        fluid_t* f);  // poisson.hip
+ // poisson.hip
 
 #define D(a) ((double*)(a).data)
 
@@ -796,14 +413,14 @@ static void fill_post(const DomainCtx* c, const fluid_t* fluid, const interface_
 // pointer swaps the reference performs.  No allocation and no host
 // synchronisation, so the sequence can be captured into a graph.
 static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, interface_t* interface,
-                      const Coef& cf, double* rhs, int band, bool use_tiles, double learned_h) {
+                      const Coef& cf, double* rhs, int band, double learned_h) {
   const int nx = (int)c->nx, ny = (int)c->ny;
   // learned curvature of each stage's input vof: on the side stream, forked
   // after the kernel that wrote that vof and joined before the stage kernel
   // that reads kappa, so it overlaps the dt reduction (stage 0) or the
   // previous stage's Poisson solve; with per-launch timing it stays on the
   // library stream, right before the stage kernel
-  const bool learned = learned_h > 0. && !use_tiles;
+  const bool learned = learned_h > 0.;
   const bool overlap = learned && !prof_active();
   const unsigned* kappa_n = nullptr;
   auto fork_curv = [&](int k, const double* vof) -> int {
@@ -816,12 +433,12 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
   if (overlap && fork_curv(0, D(interface->vof))) return 1;
   if (compute_dt_device(domain, c, fluid, interface)) return 1;
   const double* dtp = c->dt_dev;
-  const bool fold = !use_tiles && march_fold();
+  const bool fold = march_fold();
   // the units in order of the interface rows the last stage kernel found in
   // them (the previous step's; the bubbles move a fraction of a cell per
   // step), for all three stages of this step
   const bool mixed = options().precision == SBFS_PRECISION_MIXED;
-  const bool ordered = !use_tiles && march_ordered(mixed);
+  const bool ordered = march_ordered(mixed);
   const int units = march_units(nx, ny, band);
   if (ordered) {
     SBFS_REQUIRE(g_scr.units >= units, "integrate: no dispatch-order buffers for %d units", units);
@@ -841,61 +458,14 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
       swap_data(fluid->srcux[0], fluid->srcux[1]);
       swap_data(fluid->srcuy[0], fluid->srcuy[1]);
     }
-    const RK r = rk(k);
-    PreArgs a;
-    a.vof = D(interface->vof);
-    a.ux = D(fluid->ux);
-    a.uy = D(fluid->uy);
-    a.p = D(fluid->p);
-    a.srcb_vof = D(interface->src[1]);
-    a.srcb_ux = D(fluid->srcux[1]);
-    a.srcb_uy = D(fluid->srcuy[1]);
-    a.vofn = g_scr.vof;
-    a.uxs = us[k];
-    a.uys = vs[k];
-    // the alpha sources of the last stage are never read again (beta of
-    // stage 0 is zero), so they are not written
-    a.srca_vof = k < 2 ? D(interface->src[0]) : nullptr;
-    a.srca_ux = k < 2 ? D(fluid->srcux[0]) : nullptr;
-    a.srca_uy = k < 2 ? D(fluid->srcuy[0]) : nullptr;
-    a.den1 = k == 2 ? D(fluid->den[1]) : nullptr;
-    a.txy = k == 2 ? D(fluid->txy) : nullptr;
-    a.hxxf = c->hxxf;
-    a.hxxc = c->hxxc;
-    a.jdxf = c->jdxf;
-    a.jdxc = c->jdxc;
-    a.hy = c->hy;
-    a.cf = cf;
-    a.denr = fluid->denr;
-    a.dlo = fmin(1., fluid->denr);
-    a.dhi = fmax(1., fluid->denr);
-    a.visr = fluid->visr;
-    a.vlo = fmin(1., fluid->visr);
-    a.vhi = fmax(1., fluid->visr);
-    a.refdeninv = 2. / (1. + fluid->denr);
-    a.tension = 1. / interface->We;
-    a.diffusivity = 1. / fluid->Re;
-    a.grav = -1. / fluid->Fr / fluid->Fr;
-    a.dt = dtp;
-    a.ca = r.a;
-    a.cb = r.b;
-    a.cg = r.g;
-    a.stage = k;
-    a.nx = nx;
-    a.ny = ny;
-    const double g_old = rk((k + 2) % 3).g, g_new = r.g;
-    if (use_tiles) {
-      const dim3 grid(cdiv(nx, kTX), cdiv(ny, kTY)), block(kTX, kTY);
-      SBFS_TIMED("k_pre", hipLaunchKernelGGL((k_pre<kTX, kTY>), grid, block, 0, stream(), a));
-      if (check_launch("integrate: k_pre")) return 1;
-    } else {
+    {
       MarchArgs m;
       fill_march(domain, c, fluid, interface, cf, rhs, band, k, corr, us, vs, g_scr.vof, g_scr.p, dtp, m);
       if (learned) {
         if (overlap) {
           SBFS_HIP(hipStreamWaitEvent(stream(), join_event(k), 0));
           m.kappa_n = kappa_n;
-        } else if (curv_mlp_launch(a.vof, g_scr.kappa, nx, ny, learned_h, &m.kappa_n, stream())) {
+        } else if (curv_mlp_launch(D(interface->vof), g_scr.kappa, nx, ny, learned_h, &m.kappa_n, stream())) {
           return 1;
         }
         m.kappa = g_scr.kappa;
@@ -930,12 +500,7 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
     // the next stage's curvature from the new vof, beside this stage's
     // Poisson solve (the stage kernel that read kappa has been launched)
     if (overlap && k < 2 && fork_curv(k + 1, D(interface->vof))) return 1;
-    if (use_tiles) {
-      if (poisson_solve_fused(domain, c, dtp, g_old, g_new, g_scr.ux, g_scr.uy, D(interface->vof), fluid))
-        return 1;
-    } else if (poisson_transform_fused(domain, c, fluid, nullptr)) {
-      return 1;
-    }
+    if (poisson_transform_fused(domain, c, fluid, nullptr)) return 1;
     if (fold && k < 2) {
       // projected by the next stage's kernel
       swap_data(fluid->psi[0], fluid->psi[1]);
@@ -982,7 +547,7 @@ static array_t* state_arrays(fluid_t* f, interface_t* s, int k) {
 constexpr int kStateArrays = 17;
 
 static StepKey step_key(const DomainCtx* c, fluid_t* f, interface_t* s, const Coef& cf, double* rhs,
-                        int band, bool use_tiles) {
+                        int band) {
   StepKey k;
   key_add(k, c);
   k.v.push_back(c->hash);
@@ -999,7 +564,7 @@ static StepKey step_key(const DomainCtx* c, fluid_t* f, interface_t* s, const Co
   for (double x : {f->Re, f->Fr, f->denr, f->visr, f->refden, s->We, options().cadv, options().cdif,
                    options().cint})
     key_add(k, x);
-  k.v.push_back((uint64_t)band | ((uint64_t)use_tiles << 32));
+  k.v.push_back((uint64_t)band);
   k.v.push_back((uint64_t)options().gauge | ((uint64_t)options().tdm << 8) |
                 ((uint64_t)options().chunk << 16) | ((uint64_t)options().precision << 48) |
                 ((uint64_t)options().curvature << 56));
@@ -1030,9 +595,9 @@ static bool graphs_enabled() {
 }
 
 static int step_graph(const domain_t* domain, DomainCtx* c, fluid_t* fluid, interface_t* interface,
-                      const Coef& cf, double* rhs, int band, bool use_tiles, double learned_h) {
-  if (!graphs_enabled()) return fused_body(domain, c, fluid, interface, cf, rhs, band, use_tiles, learned_h);
-  const StepKey key = step_key(c, fluid, interface, cf, rhs, band, use_tiles);
+                      const Coef& cf, double* rhs, int band, double learned_h) {
+  if (!graphs_enabled()) return fused_body(domain, c, fluid, interface, cf, rhs, band, learned_h);
+  const StepKey key = step_key(c, fluid, interface, cf, rhs, band);
   g_clock++;
   for (StepGraph& g : g_graph) {
     if (g.exec && g.key == key) {
@@ -1047,7 +612,7 @@ static int step_graph(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
   if (!seen) {
     // first sight of this configuration: run eagerly (builds plans / tables)
     g_seen[g_clock & 1] = key;
-    return fused_body(domain, c, fluid, interface, cf, rhs, band, use_tiles, learned_h);
+    return fused_body(domain, c, fluid, interface, cf, rhs, band, learned_h);
   }
   StepGraph& g = g_graph[0].last_use <= g_graph[1].last_use ? g_graph[0] : g_graph[1];
   if (g.exec) {
@@ -1056,7 +621,7 @@ static int step_graph(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
   }
   hipGraph_t graph = nullptr;
   SBFS_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeRelaxed));
-  const int rc = fused_body(domain, c, fluid, interface, cf, rhs, band, use_tiles, learned_h);
+  const int rc = fused_body(domain, c, fluid, interface, cf, rhs, band, learned_h);
   const hipError_t ce = hipStreamEndCapture(stream(), &graph);
   if (rc) {
     if (graph) (void)hipGraphDestroy(graph);
@@ -1093,19 +658,17 @@ int integrate_fused(const domain_t* domain, fluid_t* fluid, interface_t* interfa
   Coef cf;
   if (coef_tables(domain, c, fluid->Re, &cf)) return 1;
   const int nx = (int)c->nx, ny = (int)c->ny;
-  const bool use_tiles = getenv("SBFS_PRE_TILES") != nullptr;
   double* rhs = nullptr;
-  if (!use_tiles && poisson_transform_fused(domain, c, fluid, &rhs)) return 1;
+  if (poisson_transform_fused(domain, c, fluid, &rhs)) return 1;
   const int band = march_band(nx, ny, options().precision == SBFS_PRECISION_MIXED);
   if (band <= 0) return 1;
-  if (!use_tiles && march_ordered(options().precision == SBFS_PRECISION_MIXED) &&
+  if (march_ordered(options().precision == SBFS_PRECISION_MIXED) &&
       order_buffers(march_units(nx, ny, band)))
     return 1;
   if (load_dt_coefs()) return 1;
   double learned_h = 0.;
   if (options().curvature == SBFS_CURVATURE_LEARNED) {
     // the network sees 5 x 5 stencils of square cells of one size
-    SBFS_REQUIRE(!use_tiles, "learned curvature: needs the marching stage kernel");
     const double h = domain->hy;
     for (int i = 0; i < nx; i++)
       SBFS_REQUIRE(fabs(domain->hxxc[i] - h) <= 1e-12 * h,
@@ -1113,7 +676,7 @@ int integrate_fused(const domain_t* domain, fluid_t* fluid, interface_t* interfa
                    domain->hxxc[i], h);
     learned_h = h;
   }
-  if (step_graph(domain, c, fluid, interface, cf, rhs, band, use_tiles, learned_h)) return 1;
+  if (step_graph(domain, c, fluid, interface, cf, rhs, band, learned_h)) return 1;
   SBFS_HIP(hipMemcpyAsync(c->host_pin, c->dt_dev, sizeof(double), hipMemcpyDeviceToHost, stream()));
   SBFS_HIP(hipStreamSynchronize(stream()));
   *dt = c->host_pin[0];

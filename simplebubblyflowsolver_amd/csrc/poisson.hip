@@ -210,33 +210,6 @@ __global__ void k_dct2_rhs(const double* __restrict__ ux, const double* __restri
   dct2_row_lds(A, B, nx, m, tw, t1, t2, out + (size_t)(j - 1) * nx);
 }
 
-// fused-path variant: predicted velocity from scratch buffers, den[1] from
-// the new vof, coefficients from the device-resident dt
-__global__ void k_dct2_rhs_fused(const double* __restrict__ ux, const double* __restrict__ uy,
-                                 const double* __restrict__ psi, const double* __restrict__ vofn,
-                                 const double* __restrict__ hxxf, const double* __restrict__ jdxf,
-                                 const double* __restrict__ jdxc, double hy, double refden,
-                                 double denr, double dlo, double dhi, double norm,
-                                 const double* __restrict__ dtp, double g_old, double g_new,
-                                 int nx, int m, const double2* __restrict__ tw,
-                                 const double2* __restrict__ t1, const double2* __restrict__ t2,
-                                 double* __restrict__ out) {
-  extern __shared__ double2 lds[];
-  double2* A = lds;
-  double2* B = lds + m;
-  const int j = blockIdx.x + 1;
-  const double dt = *dtp;
-  const double dt_old = g_old * dt, dt_new = g_new * dt;
-  const double c0 = 1. / norm / dt_new * refden;
-  const double c1 = 1. / norm * -1. / dt_new * dt_old;
-  double* row = (double*)B;
-  auto dd = [=](int p, int q) { return mix_clamp(denr, dlo, dhi, vofn[L::vof(p, q, nx)]); };
-  for (int i = threadIdx.x + 1; i <= nx; i += blockDim.x)
-    row[i - 1] = poisson_rhs(i, j, nx, ux, uy, psi, dd, hxxf, jdxf, jdxc, hy, refden, c0, c1);
-  __syncthreads();
-  dct2_row_lds(A, B, nx, m, tw, t1, t2, out + (size_t)(j - 1) * nx);
-}
-
 // ---------------------------------------------------------------------------
 // Row transforms of the marching path, the partitioned solve's expansion and
 // the tests: one wavefront per group of R rows (rowfft.hpp).  The arithmetic
@@ -744,24 +717,6 @@ int poisson_transform_fused(const domain_t* d, DomainCtx* c, fluid_t* f, double*
 #undef SBFS_ROWS_CASE
   }
   if (check_launch("poisson: DCT-II")) return 1;
-  return poisson_tail(d, p, f);
-}
-
-int poisson_solve_fused(const domain_t* d, DomainCtx* c, const double* dt_dev, double g_old,
-                        double g_new, const double* uxs, const double* uys, const double* vofn,
-                        fluid_t* f) {
-  PoissonPlan* p = plan_get(d, c);
-  if (!p) return 1;
-  const int nx = p->nx, ny = p->ny, m = p->m;
-  const double norm = 2. * (double)d->glsizes[0];
-  const size_t lds = sizeof(double2) * 2 * (size_t)(m > 0 ? m : 1);
-  SBFS_TIMED("k_dct2_rhs",
-             hipLaunchKernelGGL(k_dct2_rhs_fused, dim3(ny), dim3(row_threads(m)), lds, stream(),
-                                uxs, uys, (const double*)f->psi[0].data, vofn, c->hxxf, c->jdxf,
-                                c->jdxc, c->hy, f->refden, f->denr, fmin(1., f->denr),
-                                fmax(1., f->denr), norm, dt_dev, g_old, g_new, nx, m, p->tw,
-                                p->t1, p->t2, p->buf));
-  if (check_launch("poisson: rhs + DCT-II")) return 1;
   return poisson_tail(d, p, f);
 }
 
