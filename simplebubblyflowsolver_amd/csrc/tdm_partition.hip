@@ -71,13 +71,16 @@ __device__ void periodic_tdm(double* __restrict__ q, double* __restrict__ v,
 // tables, forward g_k = inv_k (f_k - a g_{k-1}), backward g_k -= v_k g_{k+1};
 // the interior solution replaces the chunk's rows, and its first and last
 // values are returned.  Long chunks keep only their last KR forward values in
-// registers: the first K - KR go back to their rows during the forward sweep
-// and are read again by the backward sweep (same operations, half the
-// registers: K = 63 in registers would allow one wavefront per SIMD)
-template <int K>
+// registers (K = 63 in registers would allow one wavefront per SIMD); the
+// first K - KR wait for the backward sweep in `spill`, a per-thread LDS
+// column of stride kSpillStride when kLds, otherwise their own rows of q
+// (read again by the backward sweep: a second pass over half the chunk in
+// HBM).  Same operations either way.
+constexpr int kSpillStride = 256;
+template <int K, bool kLds>
 __device__ __forceinline__ void chunk_reduce(double* __restrict__ q, int nx, int mo, int row0,
                                              const double* __restrict__ inv, const double* __restrict__ vv,
-                                             double a, double& first, double& last) {
+                                             double a, double& first, double& last, double* spill) {
   constexpr int KR = K > 40 ? K / 2 + 1 : K;
   constexpr int KM = K - KR;
   double g[KR];
@@ -86,7 +89,8 @@ __device__ __forceinline__ void chunk_reduce(double* __restrict__ q, int nx, int
   for (int k = 0; k < KM; k++) {
     const double f = q[(size_t)(row0 + k) * nx + mo];
     const double gk = inv[(size_t)k * nx + mo] * (f - a * prev);
-    q[(size_t)(row0 + k) * nx + mo] = gk;
+    if constexpr (kLds) spill[k * kSpillStride] = gk;
+    else q[(size_t)(row0 + k) * nx + mo] = gk;
     prev = gk;
   }
 #pragma unroll
@@ -102,7 +106,8 @@ __device__ __forceinline__ void chunk_reduce(double* __restrict__ q, int nx, int
   double next = g[0];
 #pragma unroll 8
   for (int k = KM - 1; k >= 0; k--) {
-    const double gk = q[(size_t)(row0 + k) * nx + mo] - vv[(size_t)k * nx + mo] * next;
+    const double fk = kLds ? spill[k * kSpillStride] : q[(size_t)(row0 + k) * nx + mo];
+    const double gk = fk - vv[(size_t)k * nx + mo] * next;
     q[(size_t)(row0 + k) * nx + mo] = gk;
     next = gk;
   }
@@ -110,15 +115,25 @@ __device__ __forceinline__ void chunk_reduce(double* __restrict__ q, int nx, int
   last = g[KR - 1];
 }
 
+template <int K>
+constexpr int spill_rows() {
+  return K > 40 ? K - (K / 2 + 1) : 0;
+}
+
+// 64 modes x 4 chunks per workgroup (kSpillStride threads); the chunks of
+// more than 40 rows keep their spilled forward values in LDS
 template <int L>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_part_reduce(double* __restrict__ q, const double* __restrict__ tab,
                               double* __restrict__ gfl, int nx, int P, double a) {
   constexpr int K = L - 1;
+  constexpr int KM = spill_rows<K>();
+  __shared__ double spill[KM > 0 ? KM * kSpillStride : 1];
   const int mo = blockIdx.x * blockDim.x + threadIdx.x;
   const int ch = blockIdx.y * blockDim.y + threadIdx.y;
   if (mo >= nx || ch >= P) return;
   double first, last;
-  chunk_reduce<K>(q, nx, mo, ch * L, tab, tab + (size_t)K * nx, a, first, last);
+  chunk_reduce<K, (KM > 0)>(q, nx, mo, ch * L, tab, tab + (size_t)K * nx, a, first, last,
+                            spill + threadIdx.y * blockDim.x + threadIdx.x);
   gfl[(size_t)(2 * ch) * nx + mo] = first;
   gfl[(size_t)(2 * ch + 1) * nx + mo] = last;
 }
@@ -329,7 +344,7 @@ __device__ __forceinline__ void part_reduce_body(const RowArgs& a, int mo, int c
   constexpr int K = L - 1;
   if (mo >= a.nx || ch >= a.P) return;
   double first, last;
-  chunk_reduce<K>(a.buf, a.nx, mo, ch * L, a.ptab, a.ptab + (size_t)K * a.nx, a.pa, first, last);
+  chunk_reduce<K, false>(a.buf, a.nx, mo, ch * L, a.ptab, a.ptab + (size_t)K * a.nx, a.pa, first, last, nullptr);
   a.gfl[(size_t)(2 * ch) * a.nx + mo] = first;
   a.gfl[(size_t)(2 * ch + 1) * a.nx + mo] = last;
 }
