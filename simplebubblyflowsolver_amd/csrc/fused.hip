@@ -293,7 +293,10 @@ static bool march_ordered(bool mixed) {
 // 4096 swarm, k_march / k_march_corr: 397 / 522 us for one round, 363 / 510
 // for three unordered, 304 / 404 for two ordered and 289 / 398 for 1.75
 // ordered (band 41; 2.4 and 1.5 rounds were slower).  The FP32 bodies'
-// cheaper fluxes leave less to balance, and one round is fastest for them.
+// cheaper fluxes leave less to balance: one round is fastest for them while
+// its bands are short (48 rows on the 2048 x 4096 swarm), two when one round
+// would give bands of 160 rows or more (4096 x 8192: 96-row bands, 6.16 ms
+// per mixed step against 6.22 with 191 rows; 128 rows 6.32, ordered 6.47-7.0).
 // SBFS_MARCH_BAND overrides the choice.
 static int march_band(int nx, int ny, bool mixed) {
   static long slots[2] = {0, 0};
@@ -314,9 +317,13 @@ static int march_band(int nx, int ny, bool mixed) {
   }
   const long strips = (nx + kStrip - 1) / kStrip;
   // rounds of the device, in quarters
-  const long quarters = mixed ? 4 : (march_ordered(mixed) ? 7 : 12);
+  long quarters = mixed ? 4 : (march_ordered(mixed) ? 7 : 12);
   long bands = quarters * slots_now / (4 * strips);
   if (bands < 1) bands = 1;
+  if (mixed && !march_ordered(mixed) && (ny + bands - 1) / bands >= 160) {
+    quarters = 8;
+    bands = quarters * slots_now / (4 * strips);
+  }
   long band = (ny + bands - 1) / bands;
   if (band < 4) band = 4;
   return (int)(band < ny ? band : ny);
