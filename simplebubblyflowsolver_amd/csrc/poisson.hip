@@ -293,33 +293,85 @@ __device__ __forceinline__ void dct2_wave(const double* __restrict__ in, double*
 // partitioned solve's chunk solutions with zero ends plus the two unit
 // spikes weighted by the bounding separators, x = (g + xl w) + xu z, and
 // the separator itself on a chunk's last row (the expansion step of
-// tdm_partition.hip, done as the rows are loaded)
+// tdm_partition.hip, done as the rows are loaded by expanded_to_lds)
 struct ModeSrc {
   const double* buf;
   const double* X;
-  const double* w;
+  const double* w;  // spike tables [L - 1][nx]
   const double* z;
   int nx, L, P;  // L is a power of two
-  // values c = 2h and 2h + 1 of mode row jm
-  __device__ __forceinline__ double2 pair(int jm, int h) const {
-    const size_t at = (size_t)jm * nx + 2 * h;
-    if (L == 0) return *(const double2*)(buf + at);
-    const int ch = jm >> (31 - __clz(L)), k = jm & (L - 1);
-    const double2 xu = *(const double2*)(X + (size_t)ch * nx + 2 * h);
-    if (k == L - 1) return xu;
-    const double2 xl = *(const double2*)(X + (size_t)(ch == 0 ? P - 1 : ch - 1) * nx + 2 * h);
-    const double2 g = *(const double2*)(buf + at);
-    const double2 wk = *(const double2*)(w + (size_t)k * nx + 2 * h);
-    const double2 zk = *(const double2*)(z + (size_t)k * nx + 2 * h);
-    return make_double2((g.x + xl.x * wk.x) + xu.x * zk.x, (g.y + xl.y * wk.y) + xu.y * zk.y);
+  // values c = 2h and 2h + 1 of mode row jm, L == 0
+  __device__ __forceinline__ double2 solved(int jm, int h) const {
+    return *(const double2*)(buf + (size_t)jm * nx + 2 * h);
   }
 };
+
+// rows_to_lds for the partitioned solve's modes (L != 0).  The chunk rows g
+// of all the lane's points are loaded first (from HBM), then the separators
+// and spikes in groups of four points (from L2), so that the unit waits for
+// one HBM latency and PER / 4 L2 latencies rather than one of each per
+// point.  With one row per unit (R = 1) the row, its chunk and whether it
+// is the chunk's last row (the separator itself) are the same for every
+// lane; with several, the loads are unconditional (a chunk's last row
+// reads spike row 0, whose product it discards) and the value a select.
+template <int LOG>
+__device__ __forceinline__ void expanded_to_lds(double2* lds, int lane, int nrows, const ModeSrc& src, int row0) {
+  using F = WaveFft<LOG>;
+  constexpr int H = F::N / 2;  // double2 per row
+  static_assert(F::R * H >= F::U, "every lane loads at least one point");
+  constexpr int PER = F::R * H / F::U, GRP = PER < 4 ? PER : 4;
+  static_assert(PER % GRP == 0, "groups of points");
+  const int nx = src.nx, sh = 31 - __clz(src.L);
+  auto row = [&](int i) { return F::R == 1 ? row0 : row0 + min((lane + i * F::U) / H, nrows - 1); };
+  auto col = [&](int i) { return 2 * ((lane + i * F::U) % H); };
+  auto put = [&](int i, double2 v) {
+    const int t = lane + i * F::U;
+    lds[(t / H) * F::PM + fpad(t % H)] = v;
+  };
+  if constexpr (F::R == 1) {
+    const int ch = row0 >> sh, k = row0 & (src.L - 1);
+    const double* xu = src.X + (size_t)ch * nx;
+    if (k == src.L - 1) {
+      double2 v[PER];
+#pragma unroll
+      for (int i = 0; i < PER; i++) v[i] = *(const double2*)(xu + col(i));
+#pragma unroll
+      for (int i = 0; i < PER; i++) put(i, v[i]);
+      return;
+    }
+  }
+  double2 g[PER];
+#pragma unroll
+  for (int i = 0; i < PER; i++) g[i] = *(const double2*)(src.buf + (size_t)row(i) * nx + col(i));
+#pragma unroll
+  for (int i0 = 0; i0 < PER; i0 += GRP) {
+    double2 xu[GRP], xl[GRP], wk[GRP], zk[GRP];
+#pragma unroll
+    for (int u = 0; u < GRP; u++) {
+      const int jm = row(i0 + u), c = col(i0 + u);
+      const int ch = jm >> sh, k = jm & (src.L - 1);
+      const int ks = k == src.L - 1 ? 0 : k;
+      xu[u] = *(const double2*)(src.X + (size_t)ch * nx + c);
+      xl[u] = *(const double2*)(src.X + (size_t)(ch == 0 ? src.P - 1 : ch - 1) * nx + c);
+      wk[u] = *(const double2*)(src.w + (size_t)ks * nx + c);
+      zk[u] = *(const double2*)(src.z + (size_t)ks * nx + c);
+    }
+#pragma unroll
+    for (int u = 0; u < GRP; u++) {
+      const bool last = F::R > 1 && (row(i0 + u) & (src.L - 1)) == src.L - 1;
+      const double2 a = g[i0 + u];
+      const double ex = (a.x + xl[u].x * wk[u].x) + xu[u].x * zk[u].x;
+      const double ey = (a.y + xl[u].y * wk[u].y) + xu[u].y * zk[u].y;
+      put(i0 + u, make_double2(last ? xu[u].x : ex, last ? xu[u].y : ey));
+    }
+  }
+}
 
 // REDFT01 of mode rows row0 .. row0 + R - 1 (of `rows`) into out: with PSI,
 // psi[1] rows j = row + 1 with the Neumann x ghosts and the periodic y halo
 // (compute_potential.c:365-380, src/fluid/boundary/psi.c; out must be
 // 16-byte aligned); otherwise plain rows of N
-template <int LOG, bool PSI>
+template <int LOG, bool PSI, int AHEAD>
 __device__ __forceinline__ void dct3_wave(const ModeSrc& src, int rows, int row0, const double2* __restrict__ tw,
                                           const double2* __restrict__ t1, const double2* __restrict__ t2,
                                           double* __restrict__ out, double2* lds, int lane) {
@@ -329,41 +381,62 @@ __device__ __forceinline__ void dct3_wave(const ModeSrc& src, int rows, int row0
   const int nrows = min(F::R, rows - row0);
   if (nrows <= 0) return;  // a second unit past the last row (one wavefront, no barrier)
   const auto tw0 = pass_tw<LOG, 0>(lt, tw);
-  rows_to_lds<LOG>(lds, lane, nrows, [&src, row0](int rr, int h) { return src.pair(row0 + rr, h); });
+  // the pre-twiddle's factors of k and M - k for iteration i (k = lt + i G,
+  // the last iteration only for k = M / 2): with AHEAD > 0 the first AHEAD
+  // iterations' are loaded before the row and the others AHEAD iterations
+  // ahead of their use, so that the pass does not wait for L2 in each
+  // iteration (AHEAD 0: each at its use, for kernels short of registers)
+  struct PreTw {
+    double2 t1, t2a, t2b;  // t1[k], t2[k], t2[k + M]
+  };
+  constexpr int NPRE = M / 2 / G + 1;
+  PreTw ta[NPRE], tb[NPRE];
+  auto fetch = [&ta, &tb, t1, t2, lt](int i) {
+    const int k = min(lt + i * G, M / 2), kb = M - max(k, 1);  // kb: unused for k = 0
+    ta[i] = PreTw{t1[k], t2[k], t2[k + M]};
+    tb[i] = PreTw{t1[kb], t2[kb], t2[kb + M]};
+  };
+#pragma unroll
+  for (int i = 0; i < AHEAD; i++) fetch(i);
+  if (src.L == 0)
+    rows_to_lds<LOG>(lds, lane, nrows, [&src, row0](int rr, int h) { return src.solved(row0 + rr, h); });
+  else
+    expanded_to_lds<LOG>(lds, lane, nrows, src, row0);
   F::sync();
   double2* Z = lds + rg * F::PM;
   // pre-twiddle in place: the values of k and M - k (0 <= k <= M/2) need
   // exactly x[k], x[M-k], x[M+k] and x[N-k], so A[k] takes the slots of
   // x[k] and x[M-k], and A[M-k] those of x[M+k] and x[N-k]; no lane writes
   // a slot that another lane reads
-  auto pre = [Z, t1, t2](int k, double xk, double xnk, double xmk, double xkm) {
+  auto pre = [](const PreTw& t, double xk, double xnk, double xmk, double xkm) {
     // xk = x[k], xnk = x[N-k] (0 for k = 0), xmk = x[M-k], xkm = x[k+M]
     const double pr = xk, pim = -xnk;
-    const double2 vk = wmul(make_double2(pr, pim), make_double2(t2[k].x, -t2[k].y));
+    const double2 vk = wmul(make_double2(pr, pim), make_double2(t.t2a.x, -t.t2a.y));
     const double qr = xkm, qim = -xmk;
-    const double2 vm = wmul(make_double2(qr, qim), make_double2(t2[k + M].x, -t2[k + M].y));
+    const double2 vm = wmul(make_double2(qr, qim), make_double2(t.t2b.x, -t.t2b.y));
     const double sr = vk.x + vm.x, si = vk.y + vm.y;
-    const double2 f = wmul(make_double2(vk.x - vm.x, vk.y - vm.y), make_double2(t1[k].x, -t1[k].y));
+    const double2 f = wmul(make_double2(vk.x - vm.x, vk.y - vm.y), make_double2(t.t1.x, -t.t1.y));
     return make_double2(sr - f.y, -(si + f.x));
   };
 #pragma unroll
-  for (int i = 0; i <= M / 2 / G; i++) {
+  for (int i = 0; i < NPRE; i++) {
+    if (i + AHEAD < NPRE) fetch(i + AHEAD);
     const int k = lt + i * G;
     if (2 * k > M) break;
     if (k == 0) {
       const double x0 = zd(Z, 0), xm = zd(Z, M);
-      const double2 a = pre(0, x0, 0., xm, xm);
+      const double2 a = pre(ta[i], x0, 0., xm, xm);
       zd(Z, 0) = a.x;
       zd(Z, M) = a.y;
     } else if (2 * k == M) {
       const double xa = zd(Z, k), xb = zd(Z, M + k);
-      const double2 a = pre(k, xa, xb, xa, xb);
+      const double2 a = pre(ta[i], xa, xb, xa, xb);
       zd(Z, k) = a.x;
       zd(Z, M + k) = a.y;
     } else {
       const double xk = zd(Z, k), xmk = zd(Z, M - k), xkm = zd(Z, M + k), xnk = zd(Z, N - k);
-      const double2 a = pre(k, xk, xnk, xmk, xkm);
-      const double2 b = pre(M - k, xmk, xkm, xk, xnk);
+      const double2 a = pre(ta[i], xk, xnk, xmk, xkm);
+      const double2 b = pre(tb[i], xmk, xkm, xk, xnk);
       zd(Z, k) = a.x;
       zd(Z, M - k) = a.y;
       zd(Z, M + k) = b.x;
@@ -435,7 +508,7 @@ __global__ SBFS_ROWS_ATTR(LOG) void k_dct3_rows(const double* __restrict__ in, d
   constexpr int N = WaveFft<LOG>::N;
   extern __shared__ double2 lds[];
   const ModeSrc src{in, nullptr, nullptr, nullptr, N, 0, 0};
-  dct3_wave<LOG, false>(src, rows, (int)blockIdx.x * WaveFft<LOG>::R, tw, t1, t2, out, lds, (int)threadIdx.x);
+  dct3_wave<LOG, false, 1>(src, rows, (int)blockIdx.x * WaveFft<LOG>::R, tw, t1, t2, out, lds, (int)threadIdx.x);
 }
 
 template <int LOG>
@@ -456,7 +529,7 @@ __global__ SBFS_ROWS_ATTR(LOG) void k_dct3_psi(ModeSrc src, double* __restrict__
   } else {
     unit = xcd_group(b, n);
   }
-  dct3_wave<LOG, true>(src, ny, unit * WaveFft<LOG>::R, tw, t1, t2, psi, lds, (int)threadIdx.x);
+  dct3_wave<LOG, true, 1>(src, ny, unit * WaveFft<LOG>::R, tw, t1, t2, psi, lds, (int)threadIdx.x);
 }
 
 #define SBFS_LOG_CASES(X) \
@@ -565,7 +638,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void k
   case LG: {                                                                                            \
     using F = WaveFft<LG>;                                                                              \
     const int units = (int)blockDim.x / F::U, u = (int)threadIdx.x / F::U;                              \
-    dct3_wave<LG, true>(src, a.ny, ((int)mp.y * units + u) * F::R, a.tw, a.t1, a.t2, a.psi,              \
+    dct3_wave<LG, true, 0>(src, a.ny, ((int)mp.y * units + u) * F::R, a.tw, a.t1, a.t2, a.psi,              \
                         lds + (size_t)u * F::R * F::PM, (int)threadIdx.x % F::U);                        \
   } break;
     SBFS_LOG_CASES(SBFS_DCT3_CASE)
