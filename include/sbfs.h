@@ -103,6 +103,7 @@ extern int sbfs_mlp_num_params(void);
  * device rasterises (the initial condition's tanh profile around circles
  * of radius 6 .. 64 cells, slopes 1/2 .. 1 per cell), minimising
  * mean(w (y - t)^2) with target t = label - kp and weight w = 1 / label^2;
+ * `batch` a multiple of 32, `nsamples` a multiple of `batch`;
  * history[e] = that loss over epoch e */
 extern int sbfs_mlp_train(uint64_t seed, size_t nsamples, int epochs, int batch,
                           double lr, double * history);
@@ -114,7 +115,7 @@ extern int sbfs_mlp_get_params(float * params, size_t n);
 extern int sbfs_mlp_samples(uint64_t seed, size_t first, size_t n,
                             float * features, float * labels, float * targets,
                             float * weights, float * stencils);
-/* gradient of mean(w (y - t)^2) over n (a multiple of 16) samples at the
+/* gradient of mean(w (y - t)^2) over n (a multiple of 32) samples at the
  * current parameters, by the training kernels (weights null: all 1);
  * *loss = that mean */
 extern int sbfs_mlp_gradient(const float * features, const float * targets,

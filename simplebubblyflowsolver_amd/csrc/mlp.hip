@@ -15,10 +15,11 @@
 //                     slopes and normal offsets, in the 8 symmetries; the
 //                     target is the label minus kp, weighted by 1 / label^2
 //                     (a relative squared error);
-//   * k_mlp_grad      forward + backward of kGradS (16) samples per wavefront
-//                     on BF16 MFMA with FP32 accumulation; writes one gradient
-//                     partial per wavefront (a 4096-sample batch is 256
-//                     wavefronts);
+//   * k_mlp_grad      forward + backward of a tile of 32 samples on BF16
+//                     MFMA with FP32 accumulation, eight wavefronts per tile
+//                     (16 samples by a block of 16 hidden units each);
+//                     writes one gradient partial per tile (a 4096-sample
+//                     batch is 128 tiles);
 //   * k_mlp_adam      sums the partials in a fixed order and takes an Adam
 //                     step on the FP32 master weights, refreshing the BF16
 //                     copies the MFMA operands read;
@@ -28,7 +29,9 @@
 //                     kappa (NaN where the plain curvature stays).
 //
 // MFMA: v_mfma_f32_16x16x16bf16_1k.  Lane l holds A[l%16][4(l/16)+r],
-// B[4(l/16)+r][l%16] and D[4(l/16)+r][l%16] (r = 0..3).  Activations are
+// B[4(l/16)+r][l%16] and D[4(l/16)+r][l%16] (r = 0..3); training uses
+// gfx950's v_mfma_f32_16x16x32_bf16, whose lane l holds eight consecutive
+// k, A[l%16][8(l/16)+j] and B[8(l/16)+j][l%16].  Activations are
 // kept as (hidden x samples) matrices, the B operand, with the weights as the
 // A operand (out x in), so one layer's D output is the next layer's B operand
 // lane for lane.
@@ -143,17 +146,27 @@ __host__ __device__ inline double plain_kh(const double* v) {
 // a bubble) give the same features bit for bit: the learned curvature has
 // no orientation bias that could push a bubble.  The sums negate exactly
 // under a mirror, so the choice is consistent.
+// w(di, dj) = v(sx dj, sy di) if transposed, else v(sx di, sy dj): the x
+// mirror, then the y mirror, then the transpose, each a select between two
+// fixed positions, so that no register array is indexed at run time.
 __host__ __device__ inline void canonical(const double* v, double* w) {
   double gx = 0., gy = 0.;
   for (int r = 1; r <= 3; r++) gx += v[r * 5 + 3] - v[r * 5 + 1];
   for (int c = 1; c <= 3; c++) gy += v[3 * 5 + c] - v[1 * 5 + c];
-  const int sx = gx < 0. ? -1 : 1, sy = gy < 0. ? -1 : 1;
-  const bool tr = fabs(gy) > fabs(gx);
-  for (int dj = -2; dj <= 2; dj++)
-    for (int di = -2; di <= 2; di++) {
-      const int si = tr ? sx * dj : sx * di, sj = tr ? sy * di : sy * dj;
-      w[(dj + 2) * 5 + di + 2] = v[(sj + 2) * 5 + si + 2];
-    }
+  const bool mx = gx < 0., my = gy < 0., tr = fabs(gy) > fabs(gx);
+  double a[NI], u[NI];
+#pragma unroll
+  for (int r = 0; r < 5; r++)
+#pragma unroll
+    for (int c = 0; c < 5; c++) a[r * 5 + c] = mx ? v[r * 5 + 4 - c] : v[r * 5 + c];
+#pragma unroll
+  for (int r = 0; r < 5; r++)
+#pragma unroll
+    for (int c = 0; c < 5; c++) u[r * 5 + c] = my ? a[(4 - r) * 5 + c] : a[r * 5 + c];
+#pragma unroll
+  for (int r = 0; r < 5; r++)
+#pragma unroll
+    for (int c = 0; c < 5; c++) w[r * 5 + c] = tr ? u[c * 5 + r] : u[r * 5 + c];
 }
 
 // features of a canonical 5 x 5 stencil (row-major, centre at 12):
@@ -301,150 +314,208 @@ __device__ inline void load_x(const unsigned short* X, int nt, s4& x0, s4& x1) {
   x1 = ld4(X + (16 * nt + lr) * NIP + 16 + 4 * lg);
 }
 
-// sum over the 16 lanes l%16 of a group (butterfly)
+// sum over the 16 lanes l%16 of a group: a butterfly through DPP lane
+// moves (quad_perm [1,0,3,2] and [2,3,0,1], then the half-row and row
+// mirrors, which after the first two steps fetch the same partial sums as
+// lane l^4 and l^8 would)
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
 __device__ inline float sum16(float v) {
-  v += __shfl_xor(v, 1);
-  v += __shfl_xor(v, 2);
-  v += __shfl_xor(v, 4);
-  v += __shfl_xor(v, 8);
+  v += dpp_mov<0xB1>(v);
+  v += dpp_mov<0x4E>(v);
+  v += dpp_mov<0x141>(v);
+  v += dpp_mov<0x140>(v);
   return v;
 }
 
-// ---------------------------------------------------------------------------
-// training: one wavefront = one tile of S samples; part[tile][NPART].
-// Forward and the deltas run per 16-sample subtile; the weight gradients,
-// which sum over the samples, read the staged activations and deltas from
-// LDS as [hidden][sample].
-// ---------------------------------------------------------------------------
-// S samples per wavefront (a multiple of 16, at most 64): 16 keeps a
-// 4096-sample batch at 256 wavefronts; the weight gradients contract over
-// the S samples in S / 16 MFMA k-steps.
-constexpr int kGradS = 16;
+// gfx950's v_mfma_f32_16x16x32_bf16: lane l holds A[l%16][8(l/16)+j] and
+// B[8(l/16)+j][l%16] (j = 0..7); D as for the 16x16x16 form
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
+__device__ inline f4 mma32(b8 a, b8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+__device__ inline b8 ld8(const unsigned short* p) { return *(const b8*)p; }
 
-template <int S>
-__global__ __launch_bounds__(64) void k_mlp_grad(Weights w, const unsigned short* __restrict__ X,
-                                                 const float* __restrict__ T, const float* __restrict__ Wt,
-                                                 long n, long first, long stride, float scale,
-                                                 float* __restrict__ part) {
-  static_assert(S % 16 == 0 && S <= 64, "samples per wavefront");
-  constexpr int NT = S / 16;
-  __shared__ __align__(16) unsigned short sx[S * NIP];   // [sample][feature]
-  __shared__ __align__(16) unsigned short sxt[NIP * S];  // [feature][sample]
-  __shared__ __align__(16) unsigned short sa1[NH * S];   // gelu(h1) [hidden][sample]
-  __shared__ __align__(16) unsigned short sd2[NH * S];   // dh2 [hidden][sample]
-  __shared__ __align__(16) unsigned short sd1[NH * S];   // dh1 [hidden][sample]
-  __shared__ float st[S], sw[S];
-  const int l = threadIdx.x, lr = l & 15, lg = l >> 4;
+// ---------------------------------------------------------------------------
+// training: one workgroup = one tile of S = 16 ST samples and 4 ST
+// wavefronts.  Wavefront w takes the samples of subtile st = w / 4 and the
+// hidden units of block mt = w % 4 (16 of each layer's 64), so the GELU
+// work, the bias sums and the dependent chain of one subtile are split four
+// ways; the layers exchange their activations and deltas through LDS,
+// stored both as [sample][hidden] (the B operand of the next product over
+// hidden units) and [hidden][sample] (the operands of the weight gradients,
+// which contract over the samples).  part[tile][NPART].
+// ---------------------------------------------------------------------------
+constexpr int kGradST = 2;               // subtiles per tile
+constexpr int kGradS = 16 * kGradST;     // samples per tile
+constexpr int kGradWaves = 4 * kGradST;  // wavefronts per tile
+
+template <int ST>
+__global__ __launch_bounds__(256 * ST) void k_mlp_grad(Weights w, const unsigned short* __restrict__ X,
+                                                       const float* __restrict__ T, const float* __restrict__ Wt,
+                                                       long n, long first, long stride, float scale,
+                                                       float* __restrict__ part) {
+  constexpr int S = 16 * ST, NW = 4 * ST;
+  constexpr int XP = NIP + 8, HP = NH + 8, SP = S + 8;  // padded rows (bf16), 16-byte multiples
+  __shared__ __align__(16) unsigned short sx[S * XP];    // x [sample][feature]
+  __shared__ __align__(16) unsigned short sxt[NIP * SP]; // x [feature][sample]
+  __shared__ __align__(16) unsigned short a1s[S * HP];   // gelu(h1) [sample][hidden]
+  __shared__ __align__(16) unsigned short a1h[NH * SP];  // gelu(h1) [hidden][sample]
+  __shared__ __align__(16) unsigned short d2s[S * HP];   // dh2 [sample][hidden]
+  __shared__ __align__(16) unsigned short d2h[NH * SP];  // dh2 [hidden][sample]
+  __shared__ __align__(16) unsigned short d1h[NH * SP];  // dh1 [hidden][sample]
+  __shared__ float sy[4][S];          // output partial of each hidden block
+  __shared__ float stg[S], swt[S];    // targets, weights
+  __shared__ float sb[ST][3][NH];     // per-subtile sums: gw3, gb2, gb1
+  __shared__ float sl[ST][2];         // per-subtile sums: loss, db3
+  const int t = (int)threadIdx.x, wv = t >> 6, l = t & 63, lr = l & 15, lg = l >> 4;
+  const int st = wv >> 2, mt = wv & 3;
   const long tile = blockIdx.x;
-  if (l < S) {
-    // sample of this lane: a stride permutation of the data set
-    const long idx = (first + (tile * S + l) * stride) % n;
-    const unsigned short* src = X + (size_t)idx * NIP;
-    for (int k = 0; k < NIP; k++) {
-      const unsigned short v = src[k];
-      sx[l * NIP + k] = v;
-      sxt[k * S + l] = v;
+  // the tile's samples (a stride permutation of the data set), 16 threads
+  // per sample, 2 features each
+  {
+    const int sm = t >> 4, fp = t & 15;
+    const long idx = (first + (tile * S + sm) * stride) % n;
+    const unsigned v = *(const unsigned*)(X + (size_t)idx * NIP + 2 * fp);
+    *(unsigned*)(sx + sm * XP + 2 * fp) = v;
+    sxt[(2 * fp) * SP + sm] = (unsigned short)(v & 0xFFFFu);
+    sxt[(2 * fp + 1) * SP + sm] = (unsigned short)(v >> 16);
+    if (fp == 0) {
+      stg[sm] = T[idx];
+      swt[sm] = Wt[idx];
     }
-    st[l] = T[idx];
-    sw[l] = Wt[idx];
   }
-  AOps a;
-  load_aops(w, a);
+  const int hb = 16 * mt + 4 * lg;  // this lane's hidden units hb + r of the D layout
+  const int sc = 16 * st + lr;      // and its sample
+  const b8 aw1 = ld8(w.w1 + (16 * mt + lr) * NIP + 8 * lg);
+  b8 aw2[2], aw2t[2];
+#pragma unroll
+  for (int ks = 0; ks < 2; ks++) {
+    aw2[ks] = ld8(w.w2 + (16 * mt + lr) * NH + 32 * ks + 8 * lg);
+    aw2t[ks] = ld8(w.w2t + (16 * mt + lr) * NH + 32 * ks + 8 * lg);
+  }
+  float b1[4], b2[4], w3[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    b1[r] = w.p[OB1 + hb + r];
+    b2[r] = w.p[OB2 + hb + r];
+    w3[r] = w.p[OW3 + hb + r];
+  }
   __syncthreads();
-  float lsum = 0.f, db3 = 0.f;
-  float gw3[4][4], gb2[4][4], gb1[4][4];  // per-lane partial sums, hidden 16 mt + 4 lg + r
-#pragma unroll
-  for (int mt = 0; mt < 4; mt++)
-#pragma unroll
-    for (int r = 0; r < 4; r++) gw3[mt][r] = gb2[mt][r] = gb1[mt][r] = 0.f;
-#pragma unroll 1
-  for (int nt = 0; nt < NT; nt++) {
-    s4 x0, x1;
-    load_x(sx, nt, x0, x1);
-    f4 h1[4], h2[4];
-    const float y = forward16(w, a, x0, x1, h1, h2);
-    // output layer: loss w (y - t)^2, dy = scale w (y - t), scale = 2 / batch
-    const float e = y - st[16 * nt + lr];
-    const float we = sw[16 * nt + lr] * e;
-    const float dy = scale * we;
-    lsum += we * e;
-    db3 += dy;
-    s4 d2b[4];
-#pragma unroll
-    for (int mt = 0; mt < 4; mt++) {
-      f4 d;
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int hid = 16 * mt + 4 * lg + r;
-        const float h = h2[mt][r];
-        gw3[mt][r] += gelu(h) * dy;
-        d[r] = w.p[OW3 + hid] * dy * dgelu(h);
-        gb2[mt][r] += d[r];
-        sa1[hid * S + 16 * nt + lr] = f2bf(gelu(h1[mt][r]));
-      }
-      d2b[mt] = pack4(d[0], d[1], d[2], d[3]);
-#pragma unroll
-      for (int r = 0; r < 4; r++) sd2[(16 * mt + 4 * lg + r) * S + 16 * nt + lr] = (unsigned short)d2b[mt][r];
-    }
-    // dh1 = gelu'(h1) * (W2^T dh2)
-#pragma unroll
-    for (int mt = 0; mt < 4; mt++) {
-      f4 c = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int ks = 0; ks < 4; ks++) c = mma(ld4(w.w2t + (16 * mt + lr) * NH + 16 * ks + 4 * lg), d2b[ks], c);
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const float d = c[r] * dgelu(h1[mt][r]);
-        gb1[mt][r] += d;
-        sd1[(16 * mt + 4 * lg + r) * S + 16 * nt + lr] = f2bf(d);
-      }
-    }
-  }
-  float* out = part + (size_t)tile * NPART;
-  lsum = sum16(lsum);
-  db3 = sum16(db3);
-  if (l == 0) {
-    out[OB3] = db3;
-    out[NP] = lsum;
-  }
-#pragma unroll
-  for (int mt = 0; mt < 4; mt++)
+  // layer 1: h1 = W1 x + b1 on this block's 16 hidden units
+  f4 h1 = mma32(aw1, ld8(sx + sc * XP + 8 * lg), f4{0.f, 0.f, 0.f, 0.f});
+  {
+    unsigned short ab[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      const float g3 = sum16(gw3[mt][r]), g2 = sum16(gb2[mt][r]), g1 = sum16(gb1[mt][r]);
-      if (lr == 0) {
-        const int hid = 16 * mt + 4 * lg + r;
-        out[OW3 + hid] = g3;
-        out[OB2 + hid] = g2;
-        out[OB1 + hid] = g1;
-      }
+      h1[r] += b1[r];
+      ab[r] = f2bf(gelu(h1[r]));
+      a1h[(hb + r) * SP + sc] = ab[r];
     }
+    *(uint2*)(a1s + sc * HP + hb) = make_uint2(ab[0] | ((unsigned)ab[1] << 16), ab[2] | ((unsigned)ab[3] << 16));
+  }
   __syncthreads();
-  // dW2[out][in] = sum_s dh2[out][s] gelu(h1)[in][s]
-#pragma unroll 1
-  for (int mt = 0; mt < 4; mt++) {
+  // layer 2 and this block's part of the output y = w3 . gelu(h2) + b3
+  f4 h2 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int nt = 0; nt < 4; nt++) {
-      f4 c = {0.f, 0.f, 0.f, 0.f};
+  for (int ks = 0; ks < 2; ks++) h2 = mma32(aw2[ks], ld8(a1s + sc * HP + 32 * ks + 8 * lg), h2);
+  float ys = 0.f;
 #pragma unroll
-      for (int ks = 0; ks < NT; ks++)
-        c = mma(ld4(sd2 + (16 * mt + lr) * S + 16 * ks + 4 * lg), ld4(sa1 + (16 * nt + lr) * S + 16 * ks + 4 * lg), c);
+  for (int r = 0; r < 4; r++) {
+    h2[r] += b2[r];
+    ys += w3[r] * gelu(h2[r]);
+  }
+  ys += __shfl_xor(ys, 16);
+  ys += __shfl_xor(ys, 32);
+  if (lg == 0) sy[mt][sc] = ys;
+  __syncthreads();
+  // output layer: loss w (y - t)^2, dy = scale w (y - t), scale = 2 / batch
+  const float y = ((sy[0][sc] + sy[1][sc]) + (sy[2][sc] + sy[3][sc])) + w.p[OB3];
+  const float e = y - stg[sc];
+  const float we = swt[sc] * e;
+  const float dy = scale * we;
+  float g3[4], g2[4], g1[4];
+  {
+    unsigned short db[4];
 #pragma unroll
-      for (int r = 0; r < 4; r++) out[OW2 + (16 * mt + 4 * lg + r) * NH + 16 * nt + lr] = c[r];
+    for (int r = 0; r < 4; r++) {
+      g3[r] = gelu(h2[r]) * dy;
+      g2[r] = w3[r] * dy * dgelu(h2[r]);
+      db[r] = f2bf(g2[r]);
+      d2h[(hb + r) * SP + sc] = db[r];
+    }
+    *(uint2*)(d2s + sc * HP + hb) = make_uint2(db[0] | ((unsigned)db[1] << 16), db[2] | ((unsigned)db[3] << 16));
+  }
+  __syncthreads();
+  // dh1 = gelu'(h1) (W2^T dh2) on this block's hidden units
+  {
+    f4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 2; ks++) c = mma32(aw2t[ks], ld8(d2s + sc * HP + 32 * ks + 8 * lg), c);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      g1[r] = c[r] * dgelu(h1[r]);
+      d1h[(hb + r) * SP + sc] = f2bf(g1[r]);
     }
   }
-  // dW1[out][in] = sum_s dh1[out][s] x[s][in]
-#pragma unroll 1
-  for (int mt = 0; mt < 4; mt++) {
+  // bias gradients: sums over the subtile's 16 samples
 #pragma unroll
-    for (int nt = 0; nt < 2; nt++) {
-      f4 c = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int ks = 0; ks < NT; ks++)
-        c = mma(ld4(sd1 + (16 * mt + lr) * S + 16 * ks + 4 * lg), ld4(sxt + (16 * nt + lr) * S + 16 * ks + 4 * lg), c);
-#pragma unroll
-      for (int r = 0; r < 4; r++) out[OW1 + (16 * mt + 4 * lg + r) * NIP + 16 * nt + lr] = c[r];
+  for (int r = 0; r < 4; r++) {
+    const float s3 = sum16(g3[r]), s2 = sum16(g2[r]), s1 = sum16(g1[r]);
+    if (lr == 0) {
+      sb[st][0][hb + r] = s3;
+      sb[st][1][hb + r] = s2;
+      sb[st][2][hb + r] = s1;
     }
+  }
+  if (mt == 0) {
+    const float sls = sum16(we * e), sdy = sum16(dy);
+    if (l == 0) {
+      sl[st][0] = sls;
+      sl[st][1] = sdy;
+    }
+  }
+  __syncthreads();
+  float* out = part + (size_t)tile * NPART;
+  // weight gradients, contracting over the tile's S samples: dW2 =
+  // dh2 gelu(h1)^T (16 blocks of 16 x 16), dW1 = dh1 x (8 blocks)
+  for (int b = wv; b < 24; b += NW) {
+    const bool two = b < 16;
+    const int ra = two ? b >> 2 : (b - 16) >> 1, cb = two ? b & 3 : (b - 16) & 1;
+    const unsigned short* A = (two ? d2h : d1h) + (16 * ra + lr) * SP;
+    const unsigned short* B = (two ? a1h : sxt) + (16 * cb + lr) * SP;
+    f4 c = {0.f, 0.f, 0.f, 0.f};
+    if constexpr (S % 32 == 0) {
+#pragma unroll
+      for (int ks = 0; ks < S / 32; ks++) c = mma32(ld8(A + 32 * ks + 8 * lg), ld8(B + 32 * ks + 8 * lg), c);
+    } else {
+      c = mma(ld4(A + 4 * lg), ld4(B + 4 * lg), c);
+    }
+    float* o = out + (two ? OW2 + (16 * ra + 4 * lg) * NH : OW1 + (16 * ra + 4 * lg) * NIP) + 16 * cb + lr;
+    const int ld = two ? NH : NIP;
+#pragma unroll
+    for (int r = 0; r < 4; r++) o[r * ld] = c[r];
+  }
+  if (t < NH) {
+    float s3 = 0.f, s2 = 0.f, s1 = 0.f;
+#pragma unroll
+    for (int u = 0; u < ST; u++) {
+      s3 += sb[u][0][t];
+      s2 += sb[u][1][t];
+      s1 += sb[u][2][t];
+    }
+    out[OW3 + t] = s3;
+    out[OB2 + t] = s2;
+    out[OB1 + t] = s1;
+  } else if (t == NH) {
+    float sls = 0.f, sdy = 0.f;
+#pragma unroll
+    for (int u = 0; u < ST; u++) {
+      sls += sl[u][0];
+      sdy += sl[u][1];
+    }
+    out[NP] = sls;
+    out[OB3] = sdy;
   }
 }
 
@@ -769,7 +840,7 @@ static void free_samples(Samples& s) {
 
 // one pass of `tiles` tiles starting at permuted position `first`
 static int grad_pass(const Samples& s, long first, long stride, int tiles, float scale, float* part) {
-  SBFS_TIMED("k_mlp_grad", hipLaunchKernelGGL(k_mlp_grad<kGradS>, dim3(tiles), dim3(64), 0, stream(), weights(), s.x,
+  SBFS_TIMED("k_mlp_grad", hipLaunchKernelGGL(k_mlp_grad<kGradST>, dim3(tiles), dim3(64 * kGradWaves), 0, stream(), weights(), s.x,
                                               s.t, s.w, s.n, first, stride, scale, part));
   return check_launch("mlp: gradients");
 }
@@ -858,7 +929,7 @@ extern "C" int sbfs_mlp_get_params(float* params, size_t n) {
 }
 
 // gradient of the weighted mean squared error mean(w (y - t)^2) over n
-// samples (n a multiple of 16) given as BF16-representable features [n][32],
+// samples (n a multiple of 32) given as BF16-representable features [n][32],
 // targets and weights (null: all 1), at the current parameters; grad[NP],
 // *loss = the weighted mean squared error
 extern "C" int sbfs_mlp_gradient(const float* features, const float* targets, const float* weights, size_t n,
@@ -934,13 +1005,13 @@ extern "C" int sbfs_mlp_predict(const double* stencils, size_t n, double* out) {
 }
 
 // train from Glorot-uniform weights on `nsamples` generated stencils (a
-// multiple of `batch`, itself a multiple of 16) for `epochs` passes with Adam
+// multiple of `batch`, itself a multiple of 32) for `epochs` passes with Adam
 // (learning rate `lr`, cosine decay to lr / 20); history[e] = weighted mean
 // squared error of epoch e (may be null)
 extern "C" int sbfs_mlp_train(uint64_t seed, size_t nsamples, int epochs, int batch, double lr,
                               double* history) {
   SBFS_REQUIRE(nsamples > 0 && epochs > 0 && batch > 0 && batch % kGradS == 0 && nsamples % (size_t)batch == 0,
-               "sbfs_mlp_train: nsamples must be a multiple of batch, batch a multiple of 16");
+               "sbfs_mlp_train: nsamples must be a multiple of batch, batch a multiple of 32");
   std::vector<float> p0;
   init_params(seed, p0);
   if (set_params(p0.data())) return 1;
