@@ -113,6 +113,7 @@ struct FusedScratch {
   double* uy2 = nullptr;
   double* p = nullptr;    // pressure written by a stage kernel with the folded projection
   float* kappa = nullptr;  // learned curvature of a stage's input vof (interior cells)
+  unsigned* klist = nullptr;  // its listed cells (count first)
   // the stage kernels' dispatch order and unit weights (march.hpp)
   unsigned* weight = nullptr;
   unsigned* order = nullptr;
@@ -126,6 +127,7 @@ static int scratch(const domain_t* d, const interface_t* s, const fluid_t* f) {
   graph_epoch_bump();
   for (double* q : {g_scr.vof, g_scr.ux, g_scr.uy, g_scr.ux2, g_scr.uy2, g_scr.p}) (void)hipFree(q);
   (void)hipFree(g_scr.kappa);
+  (void)hipFree(g_scr.klist);
   (void)hipFree(g_scr.weight);
   (void)hipFree(g_scr.order);
   g_scr = FusedScratch();
@@ -136,7 +138,10 @@ static int scratch(const domain_t* d, const interface_t* s, const fluid_t* f) {
     SBFS_HIP(hipMalloc(b.first, b.second));
     SBFS_HIP(hipMemsetAsync(*b.first, 0, b.second, stream()));
   }
-  SBFS_HIP(hipMalloc(&g_scr.kappa, sizeof(float) * d->mysizes[0] * d->mysizes[1]));
+  const size_t cells = d->mysizes[0] * d->mysizes[1];
+  SBFS_HIP(hipMalloc(&g_scr.kappa, sizeof(float) * cells));
+  SBFS_HIP(hipMalloc(&g_scr.klist, sizeof(unsigned) * (cells + 1)));
+  if (curv_field_init(g_scr.kappa, g_scr.klist, (long)cells, stream())) return 1;
   g_scr.nx = d->mysizes[0];
   g_scr.ny = d->mysizes[1];
   return 0;
@@ -429,11 +434,10 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
   // library stream, right before the stage kernel
   const bool learned = learned_h > 0.;
   const bool overlap = learned && !prof_active();
-  const unsigned* kappa_n = nullptr;
   auto fork_curv = [&](int k, const double* vof) -> int {
     SBFS_HIP(hipEventRecord(fork_event(k), stream()));
     SBFS_HIP(hipStreamWaitEvent(side_stream(), fork_event(k), 0));
-    if (curv_mlp_launch(vof, g_scr.kappa, nx, ny, learned_h, &kappa_n, side_stream())) return 1;
+    if (curv_mlp_launch(vof, g_scr.kappa, g_scr.klist, nx, ny, learned_h, side_stream())) return 1;
     SBFS_HIP(hipEventRecord(join_event(k), side_stream()));
     return 0;
   };
@@ -471,11 +475,11 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
       if (learned) {
         if (overlap) {
           SBFS_HIP(hipStreamWaitEvent(stream(), join_event(k), 0));
-          m.kappa_n = kappa_n;
-        } else if (curv_mlp_launch(D(interface->vof), g_scr.kappa, nx, ny, learned_h, &m.kappa_n, stream())) {
+        } else if (curv_mlp_launch(D(interface->vof), g_scr.kappa, g_scr.klist, nx, ny, learned_h, stream())) {
           return 1;
         }
         m.kappa = g_scr.kappa;
+        m.kappa_n = g_scr.klist;
       }
       if (ordered) {
         m.order = g_scr.order;
@@ -576,6 +580,7 @@ static StepKey step_key(const DomainCtx* c, fluid_t* f, interface_t* s, const Co
                 ((uint64_t)options().chunk << 16) | ((uint64_t)options().precision << 48) |
                 ((uint64_t)options().curvature << 56));
   key_add(k, g_scr.kappa);
+  key_add(k, g_scr.klist);
   k.v.push_back(march_ordered(options().precision == SBFS_PRECISION_MIXED) ? 1 : 0);
   key_add(k, g_scr.order);
   k.v.push_back(graph_epoch());

@@ -312,7 +312,7 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
   double np = ld_cc(a.p, wrap_r(s_first, ny)), nps = ld_cc(a.psi0, CORR ? w1 : wrap_r(s_first, ny));
   double npo = CORR ? ld_cc(a.psi_old, w1) : 0.;
   // learned curvature of row s, prefetched a row ahead (0 where absent)
-  // (no listed segment: every value is NaN, so none is read)
+  // (no listed cell: every value is NaN, so none is read)
   const float* kap = (a.kappa && *a.kappa_n != 0u) ? a.kappa : nullptr;
   auto ld_kap = [&](int jw) { return (kap && c_in) ? lane_at(ci_row(kap, jw), loff >> 1) : 0.f; };
   float nkap = ld_kap(wrap_r(s_first, ny));

@@ -78,8 +78,8 @@ struct MarchArgs {
   double pg_old, pg_new;  // Runge-Kutta gamma of the previous stage's dt_old / dt_new
   // learned curvature of the stage's input vof (interior cells, (j-1) Nx +
   // (i-1)); NaN where the plain curvature stays; null: plain everywhere.
-  // kappa_n: the gate's count of listed row segments (0: no cell learned,
-  // and the kernel skips the kappa reads)
+  // kappa_n: the gate's count of listed cells (0: no cell learned, every
+  // value is NaN, and the kernel skips the kappa reads)
   const float* kappa;
   const unsigned* kappa_n;
   // dispatch order of the (strip, band) units, unit = band * strips + strip

@@ -98,9 +98,32 @@ __host__ __device__ inline unsigned short f2bf(float x) {
   return (unsigned short)(u >> 16);
 }
 
-__device__ inline float gelu(float x) { return 0.5f * x * (1.f + erff(x * 0.70710678118654752f)); }
+// 1 + erf(x / sqrt 2) from erfc of z = |x| / sqrt 2: 2 - erfc(z) for
+// x >= 0, erfc(z) for x < 0, so that the negative side keeps its relative
+// accuracy (1 + erf there is a small difference of O(1) numbers).  erfc is
+// the Chebyshev fit of Numerical Recipes (erfcc, fractional error below
+// 1.2e-7), t exp(-z^2 + P(t)) with t = 1 / (1 + z/2); in FP32 the rounding
+// of the exponent keeps it within 1e-5 relative.  One reciprocal and one
+// exponential, where erff takes several branches' worth of work.
+__device__ inline float one_plus_erf(float x) {
+  const float z = fabsf(x) * 0.70710678118654752f;
+  const float t = __builtin_amdgcn_rcpf(fmaf(0.5f, z, 1.f));
+  float p = 0.17087277f;
+  p = fmaf(t, p, -0.82215223f);
+  p = fmaf(t, p, 1.48851587f);
+  p = fmaf(t, p, -1.13520398f);
+  p = fmaf(t, p, 0.27886807f);
+  p = fmaf(t, p, -0.18628806f);
+  p = fmaf(t, p, 0.09678418f);
+  p = fmaf(t, p, 0.37409196f);
+  p = fmaf(t, p, 1.00002368f);
+  p = fmaf(t, p, -1.26551223f);
+  const float c = t * __expf(fmaf(-z, z, p));
+  return x >= 0.f ? 2.f - c : c;
+}
+__device__ inline float gelu(float x) { return 0.5f * x * one_plus_erf(x); }
 __device__ inline float dgelu(float x) {
-  return 0.5f * (1.f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
+  return 0.5f * one_plus_erf(x) + x * 0.39894228040143268f * __expf(-0.5f * x * x);
 }
 
 __device__ inline f4 mma(s4 a, s4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
@@ -623,6 +646,20 @@ __global__ __launch_bounds__(64) void k_mlp_predict(Weights w, const unsigned sh
 // others).  Layout of kappa and of the list entries: interior cells,
 // (j-1) Nx + (i-1).
 // ---------------------------------------------------------------------------
+// The kappa field and its list of cells (list[0] the count, then the cell
+// indices) are kept together: every cell outside the list holds NaN.  A
+// new field is filled with NaN and its list emptied once; each launch then
+// puts the listed cells of the previous launch back to NaN before it lists
+// and fills the new ones, so the field is never written whole again.
+__global__ void k_curv_fill(float* __restrict__ kappa, long n, unsigned* __restrict__ list) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) kappa[c] = __int_as_float(0x7FC00000);
+  if (blockIdx.x == 0 && threadIdx.x == 0) list[0] = 0u;
+}
+__global__ void k_curv_restore(float* __restrict__ kappa, const unsigned* __restrict__ list) {
+  const unsigned n = list[0], stride = gridDim.x * blockDim.x;
+  for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) kappa[list[1 + q]] = __int_as_float(0x7FC00000);
+}
 __global__ void k_curv_reset(unsigned* __restrict__ count) {
   if (threadIdx.x == 0) *count = 0u;
 }
@@ -630,9 +667,8 @@ __global__ void k_curv_reset(unsigned* __restrict__ count) {
 // one 64-cell column strip of kGateRows rows per wavefront, four per
 // workgroup; every vof load of the strip is issued before the first test
 constexpr int kGateRows = 16;
-__global__ __launch_bounds__(256) void k_curv_gate(const double* __restrict__ vof, float* __restrict__ kappa,
-                                                   int nx, int ny, unsigned* __restrict__ count,
-                                                   unsigned* __restrict__ list) {
+__global__ __launch_bounds__(256) void k_curv_gate(const double* __restrict__ vof, int nx, int ny,
+                                                   unsigned* __restrict__ count, unsigned* __restrict__ list) {
   const int l = threadIdx.x & 63;
   const int i = 1 + (int)blockIdx.x * 64 + l;
   const int j0 = 1 + ((int)blockIdx.y * 4 + (int)(threadIdx.x >> 6)) * kGateRows;
@@ -659,7 +695,6 @@ __global__ __launch_bounds__(256) void k_curv_gate(const double* __restrict__ vo
 #pragma unroll
   for (int k = 0; k < kGateRows; k++) {
     const int j = j0 + k;
-    if (j <= ny && own) kappa[(size_t)(j - 1) * nx + (i - 1)] = __int_as_float(0x7FC00000);
     // columns of the strip with a cut cell in rows j-1..j+1, then the
     // 3-column neighbourhood from the neighbouring lanes' bits
     const uint64_t m = __ballot(cc[k] | cc[k + 1] | cc[k + 2]);
@@ -847,31 +882,27 @@ static int grad_pass(const Samples& s, long first, long stride, int tiles, float
 
 }  // namespace mlp
 
-// the learned curvature of a stage's input vof: the gate, then the network
-// on the listed segments (list and counter kept per process, grown with the
-// grid)
-int curv_mlp_launch(const double* vof, float* kappa, int nx, int ny, double h, const unsigned** count,
-                    hipStream_t st) {
+// the kappa field of `cells` cells with its list (cells + 1 entries):
+// NaN everywhere, empty list
+int curv_field_init(float* kappa, unsigned* list, long cells, hipStream_t st) {
+  hipLaunchKernelGGL(mlp::k_curv_fill, dim3(1024), dim3(256), 0, st, kappa, cells, list);
+  return check_launch("k_curv_fill");
+}
+
+// the learned curvature of a stage's input vof: the previous launch's cells
+// back to NaN, then the gate lists the cells of this vof and the network
+// fills them
+int curv_mlp_launch(const double* vof, float* kappa, unsigned* list, int nx, int ny, double h, hipStream_t st) {
   using namespace mlp;
   SBFS_REQUIRE(g_net.trained, "learned curvature: no trained network (sbfs_mlp_train or sbfs_mlp_set_params)");
-  static unsigned* buf = nullptr;  // [0] counter, then the cell list
-  static size_t cap = 0;
-  const size_t cells = (size_t)nx * ny;
-  if (cells + 1 > cap) {
-    (void)hipStreamSynchronize(stream());
-    (void)hipFree(buf);
-    buf = nullptr;
-    SBFS_HIP(hipMalloc(&buf, sizeof(unsigned) * (cells + 1)));
-    cap = cells + 1;
-    graph_epoch_bump();  // captured steps hold the old address
-  }
-  hipLaunchKernelGGL(k_curv_reset, dim3(1), dim3(64), 0, st, buf);
+  SBFS_TIMED("k_curv_restore", hipLaunchKernelGGL(k_curv_restore, dim3(256), dim3(256), 0, st, kappa,
+                                                  (const unsigned*)list));
+  hipLaunchKernelGGL(k_curv_reset, dim3(1), dim3(64), 0, st, list);
   SBFS_TIMED("k_curv_gate", hipLaunchKernelGGL(k_curv_gate, dim3(cdiv(nx, 64), cdiv(ny, 4 * kGateRows)), dim3(256),
-                                               0, st, vof, kappa, nx, ny, buf, buf + 1));
+                                               0, st, vof, nx, ny, list, list + 1));
   SBFS_TIMED("k_curv_mlp", hipLaunchKernelGGL(k_curv_mlp, dim3(4096), dim3(64), 0, st, weights(), vof, kappa,
-                                              nx, ny, (float)(1. / h), (const unsigned*)buf,
-                                              (const unsigned*)(buf + 1)));
-  *count = buf;
+                                              nx, ny, (float)(1. / h), (const unsigned*)list,
+                                              (const unsigned*)(list + 1)));
   return check_launch("k_curv_mlp");
 }
 
@@ -1089,12 +1120,14 @@ extern "C" int sbfs_mlp_coverage(const domain_t* domain, const interface_t* inte
   const int nx = (int)domain->mysizes[0], ny = (int)domain->mysizes[1];
   const long n = (long)nx * ny;
   float* k = nullptr;
+  unsigned* list = nullptr;
   unsigned long long* cnt = nullptr;
   SBFS_HIP(hipMalloc(&k, sizeof(float) * n));
+  SBFS_HIP(hipMalloc(&list, sizeof(unsigned) * (n + 1)));
   SBFS_HIP(hipMalloc(&cnt, sizeof(unsigned long long)));
   SBFS_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), stream()));
-  const unsigned* listed = nullptr;
-  int rc = curv_mlp_launch((const double*)interface->vof.data, k, nx, ny, domain->hy, &listed, stream());
+  int rc = curv_field_init(k, list, n, stream());
+  if (!rc) rc = curv_mlp_launch((const double*)interface->vof.data, k, list, nx, ny, domain->hy, stream());
   unsigned long long h = 0;
   if (!rc) {
     hipLaunchKernelGGL(k_count_learned, dim3(1024), dim3(256), 0, stream(), (const float*)k, n, cnt);
@@ -1106,6 +1139,7 @@ extern "C" int sbfs_mlp_coverage(const domain_t* domain, const interface_t* inte
     *fraction = (double)h / (double)n;
   }
   (void)hipFree(k);
+  (void)hipFree(list);
   (void)hipFree(cnt);
   return rc;
 }

@@ -155,12 +155,15 @@ int npy_write(const char* dir, const char* name, const char* descr,
 int load_dt_coefs();
 
 // learned curvature (mlp.hip): kappa of the interior cells of `vof`, NaN
-// where the plain curvature stays; h is the (uniform, square) cell size;
-// *count receives the device address of the number of listed row segments
-int curv_mlp_launch(const double* vof, float* kappa, int nx, int ny, double h, const unsigned** count,
-                    hipStream_t st);
+// where the plain curvature stays; h is the (uniform, square) cell size.
+// `list` (nx ny + 1 entries: the count, then the listed cells) belongs to
+// `kappa`: curv_field_init fills a new field with NaN and empties its list,
+// and each curv_mlp_launch first returns the cells its predecessor listed
+// to NaN, so only listed cells ever hold a value.
+int curv_field_init(float* kappa, unsigned* list, long cells, hipStream_t st);
+int curv_mlp_launch(const double* vof, float* kappa, unsigned* list, int nx, int ny, double h, hipStream_t st);
 // Captured step graphs hold device addresses that their keys do not list
-// (Poisson plan buffers, domain metrics, the inference's segment list).
+// (Poisson plan buffers, domain metrics).
 // Whatever frees such memory bumps this epoch, which is part of every key,
 // so no graph replays against freed memory, even when a new allocation
 // reuses the old addresses.

@@ -173,10 +173,11 @@ def test_rising_bubble_learned_within_one_percent(product, trained):
 
 
 def test_learned_steps_after_a_larger_case(product, trained):
-    """A larger case grows the inference's segment buffer; steps of a
-    smaller case captured before it must not replay with the old buffer
-    (the step graphs are keyed on the buffer), so the small case repeats
-    bitwise."""
+    """A larger case reallocates the step's learned-curvature field and
+    its cell list; steps of a smaller case captured before it must not
+    replay with the old buffers (the step graphs are keyed on them), and a
+    list of the larger case must never index the smaller field, so the
+    small case repeats bitwise."""
     net, _ = trained
 
     def run(n, steps):
