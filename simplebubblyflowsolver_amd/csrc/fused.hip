@@ -114,6 +114,7 @@ struct FusedScratch {
   double* p = nullptr;    // pressure written by a stage kernel with the folded projection
   float* kappa = nullptr;  // learned curvature of a stage's input vof (interior cells)
   unsigned* klist = nullptr;  // its listed cells (count first)
+  unsigned long long* cut = nullptr;  // cut-cell bitmask of a stage kernel's new vof
   // the stage kernels' dispatch order and unit weights (march.hpp)
   unsigned* weight = nullptr;
   unsigned* order = nullptr;
@@ -128,6 +129,7 @@ static int scratch(const domain_t* d, const interface_t* s, const fluid_t* f) {
   for (double* q : {g_scr.vof, g_scr.ux, g_scr.uy, g_scr.ux2, g_scr.uy2, g_scr.p}) (void)hipFree(q);
   (void)hipFree(g_scr.kappa);
   (void)hipFree(g_scr.klist);
+  (void)hipFree(g_scr.cut);
   (void)hipFree(g_scr.weight);
   (void)hipFree(g_scr.order);
   g_scr = FusedScratch();
@@ -142,6 +144,7 @@ static int scratch(const domain_t* d, const interface_t* s, const fluid_t* f) {
   SBFS_HIP(hipMalloc(&g_scr.kappa, sizeof(float) * cells));
   SBFS_HIP(hipMalloc(&g_scr.klist, sizeof(unsigned) * (cells + 1)));
   if (curv_field_init(g_scr.kappa, g_scr.klist, (long)cells, stream())) return 1;
+  SBFS_HIP(hipMalloc(&g_scr.cut, sizeof(unsigned long long) * d->mysizes[1] * march_strips((int)d->mysizes[0])));
   g_scr.nx = d->mysizes[0];
   g_scr.ny = d->mysizes[1];
   return 0;
@@ -353,6 +356,7 @@ static void fill_march(const domain_t* domain, const DomainCtx* c, const fluid_t
   m.pg_new = rk((k + 2) % 3).g;
   m.kappa = nullptr;
   m.kappa_n = nullptr;
+  m.cut = nullptr;
   m.order = nullptr;
   m.weight = nullptr;
   m.srcb_vof = D(interface->src[1]);
@@ -434,10 +438,14 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
   // library stream, right before the stage kernel
   const bool learned = learned_h > 0.;
   const bool overlap = learned && !prof_active();
+  // (stages 1 and 2 list their cells from the previous stage kernel's
+  // bitmask of the vof it wrote; stage 0 from the state's vof, which the
+  // caller may have changed since the last step)
+  auto cut_of = [](int k) -> const unsigned long long* { return k > 0 ? g_scr.cut : nullptr; };
   auto fork_curv = [&](int k, const double* vof) -> int {
     SBFS_HIP(hipEventRecord(fork_event(k), stream()));
     SBFS_HIP(hipStreamWaitEvent(side_stream(), fork_event(k), 0));
-    if (curv_mlp_launch(vof, g_scr.kappa, g_scr.klist, nx, ny, learned_h, side_stream())) return 1;
+    if (curv_mlp_launch(vof, cut_of(k), g_scr.kappa, g_scr.klist, nx, ny, learned_h, side_stream())) return 1;
     SBFS_HIP(hipEventRecord(join_event(k), side_stream()));
     return 0;
   };
@@ -475,11 +483,13 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
       if (learned) {
         if (overlap) {
           SBFS_HIP(hipStreamWaitEvent(stream(), join_event(k), 0));
-        } else if (curv_mlp_launch(D(interface->vof), g_scr.kappa, g_scr.klist, nx, ny, learned_h, stream())) {
+        } else if (curv_mlp_launch(D(interface->vof), cut_of(k), g_scr.kappa, g_scr.klist, nx, ny, learned_h,
+                                   stream())) {
           return 1;
         }
         m.kappa = g_scr.kappa;
         m.kappa_n = g_scr.klist;
+        if (k < 2) m.cut = g_scr.cut;
       }
       if (ordered) {
         m.order = g_scr.order;
@@ -581,6 +591,7 @@ static StepKey step_key(const DomainCtx* c, fluid_t* f, interface_t* s, const Co
                 ((uint64_t)options().curvature << 56));
   key_add(k, g_scr.kappa);
   key_add(k, g_scr.klist);
+  key_add(k, g_scr.cut);
   k.v.push_back(march_ordered(options().precision == SBFS_PRECISION_MIXED) ? 1 : 0);
   key_add(k, g_scr.order);
   k.v.push_back(graph_epoch());

@@ -619,6 +619,10 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
       txy0 = txy1;
       // -- outputs of row s -----------------------------------------------------
       const double txy0r = double(A::right(txy_prev));
+      if (a.cut) {
+        const uint64_t m = __ballot(outrow && c_out && vn0 >= kCutLo && vn0 <= kCutHi);
+        if (outrow && lane == 0) a.cut[(size_t)(s - 1) * march_strips(nx) + bx] = m >> 3;
+      }
       if (outrow && c_out) {
         auto put = [&](double* f, int nh, auto row, double v) {
           AT(row(f, s)) = v;

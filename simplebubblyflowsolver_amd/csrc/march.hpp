@@ -82,6 +82,10 @@ struct MarchArgs {
   // value is NaN, and the kernel skips the kappa reads)
   const float* kappa;
   const unsigned* kappa_n;
+  // cut-cell bitmask of the new vof for the next stage's learned curvature
+  // (null: not written): word (j-1) strips + bx, bit t for column
+  // 1 + bx kStrip + t, set where the vof lies in [kCutLo, kCutHi]
+  unsigned long long* cut;
   // dispatch order of the (strip, band) units, unit = band * strips + strip
   // (null: the 2D grid's own order), and each unit's count of marched rows
   // with interface cells, written for the next order (null: not counted)

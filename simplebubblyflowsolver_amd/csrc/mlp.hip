@@ -48,6 +48,7 @@
 #include <cstring>
 #include <vector>
 
+#include "march.hpp"
 #include "sbfs.h"
 #include "sbfs_internal.hpp"
 
@@ -70,7 +71,7 @@ constexpr double kPi = 3.14159265358979323846;
 // not a sharp interface (near-pure cells of both phases, vof < kPure and
 // > 1 - kPure: outside the training profiles), and |kp + y| lies in
 // [kKhLo, kKhHi] (kappa h; the training circles span 1/64 .. 1/6).
-constexpr double kUseLo = 0.02, kUseHi = 0.98, kPure = 1e-3;
+constexpr double kUseLo = kCutLo, kUseHi = kCutHi, kPure = 1e-3;
 constexpr double kKhLo = 1. / 96., kKhHi = 0.25;
 
 typedef short s4 __attribute__((ext_vector_type(4)));
@@ -664,43 +665,40 @@ __global__ void k_curv_reset(unsigned* __restrict__ count) {
   if (threadIdx.x == 0) *count = 0u;
 }
 
-// one 64-cell column strip of kGateRows rows per wavefront, four per
-// workgroup; every vof load of the strip is issued before the first test
-constexpr int kGateRows = 16;
+// one strip of kGateCols columns and kGateRows rows per wavefront, four
+// per workgroup: lane l holds column kGateCols bx + l, lanes 0 and 63 the
+// columns on either side of the strip, so the 3 x 3 block of every strip
+// cell lies in the wavefront's own loads (one per lane and row, all issued
+// before the first test)
+constexpr int kGateCols = 62, kGateRows = 32;
 __global__ __launch_bounds__(256) void k_curv_gate(const double* __restrict__ vof, int nx, int ny,
                                                    unsigned* __restrict__ count, unsigned* __restrict__ list) {
   const int l = threadIdx.x & 63;
-  const int i = 1 + (int)blockIdx.x * 64 + l;
+  const int i = (int)blockIdx.x * kGateCols + l;
   const int j0 = 1 + ((int)blockIdx.y * 4 + (int)(threadIdx.x >> 6)) * kGateRows;
   if (j0 > ny) return;  // whole wavefronts
   const size_t wc = (size_t)nx + 2;
   auto in_use = [](double v) { return (v >= kUseLo) & (v <= kUseHi); };
-  // this lane's column, and for lanes 0 and 63 the column just outside the
-  // strip (cells i-1 and i+1 of the 3 x 3 block); columns outside 1..nx
-  // never count.  Rows past ny + 1 of the last strip are clamped and unused.
-  const bool own = i <= nx;
-  const int ie = l == 0 ? i - 1 : i + 1;
-  const bool edge = (l == 0 || l == 63) && ie >= 1 && ie <= nx;
-  const int ic = min(i, nx), iec = min(max(ie, 1), nx);
-  bool cc[kGateRows + 2], ce[kGateRows + 2];
+  // columns outside 1..nx never count; rows past ny + 1 of the last strip
+  // are clamped and unused
+  const bool col = i >= 1 && i <= nx;
+  const int ic = min(max(i, 1), nx);
+  bool cc[kGateRows + 2];
 #pragma unroll
   for (int k = 0; k < kGateRows + 2; k++) {
     const double* r = vof + (size_t)(min(j0 - 1 + k, ny + 1) + 1) * wc;
-    cc[k] = own & in_use(r[ic]);
-    ce[k] = edge & in_use(r[iec]);
+    cc[k] = col & in_use(r[ic]);
   }
-  const bool inside = own && i >= 3 && i <= nx - 2;
+  const bool inside = l >= 1 && l <= kGateCols && i >= 3 && i <= nx - 2;
   uint64_t adm[kGateRows];
   unsigned total = 0u;
 #pragma unroll
   for (int k = 0; k < kGateRows; k++) {
     const int j = j0 + k;
-    // columns of the strip with a cut cell in rows j-1..j+1, then the
-    // 3-column neighbourhood from the neighbouring lanes' bits
+    // columns with a cut cell in rows j-1..j+1, then their 3-column
+    // neighbourhood
     const uint64_t m = __ballot(cc[k] | cc[k + 1] | cc[k + 2]);
-    const uint64_t left = __ballot(l == 0 && (ce[k] | ce[k + 1] | ce[k + 2]));
-    const uint64_t right = __ballot(l == 63 && (ce[k] | ce[k + 1] | ce[k + 2]));
-    const uint64_t nb = m | (m << 1) | (m >> 1) | (left != 0 ? 1ull : 0ull) | (right != 0 ? 1ull << 63 : 0ull);
+    const uint64_t nb = m | (m << 1) | (m >> 1);
     adm[k] = __ballot(j <= ny && inside && ((nb >> l) & 1ull));
     total += (unsigned)__popcll(adm[k]);
   }
@@ -774,6 +772,62 @@ __global__ __launch_bounds__(64) void k_curv_mlp(Weights w, const double* __rest
     const double kh = kp + (double)y;
     const bool use = ok && fabs(kh) >= kKhLo && fabs(kh) <= kKhHi;
     if (use) kappa[c] = (float)(kh * (double)inv_h);
+  }
+}
+
+// The same list from the stage kernel's cut-cell bitmask of this vof (8
+// bytes per kStrip cells of a row, march.hpp) instead of the FP64 field.  A
+// wavefront takes one bitmask strip (bit t: column 1 + b kStrip + t) and 64
+// rows, lane l row j0 + l: the lane dilates its row's word by one column
+// each way (with the edge bits of the neighbouring strips), the rows above
+// and below arrive by lane shuffles, and a prefix sum over the lanes places
+// each row's cells in the list behind one atomic per wavefront.
+__global__ __launch_bounds__(256) void k_curv_gate_cut(const unsigned long long* __restrict__ cut, int strips,
+                                                       int nx, int ny, unsigned* __restrict__ count,
+                                                       unsigned* __restrict__ list) {
+  const int l = threadIdx.x & 63, b = (int)blockIdx.x;
+  const int j0 = 1 + ((int)blockIdx.y * 4 + (int)(threadIdx.x >> 6)) * 64;
+  if (j0 > ny) return;  // whole wavefronts
+  const int j = j0 + l;
+  // columns of the strip within one column of a cut cell of row r (rows
+  // wrap periodically)
+  auto near = [&](int r) -> uint64_t {
+    int rw = r - 1;
+    if (rw < 0) rw += ny;
+    while (rw >= ny) rw -= ny;
+    const unsigned long long* w = cut + (size_t)rw * strips + b;
+    const uint64_t c = w[0];
+    const uint64_t lw = b > 0 ? w[-1] : 0ull, rwd = b + 1 < strips ? w[1] : 0ull;
+    return c | (c << 1) | (c >> 1) | ((lw >> (kStrip - 1)) & 1ull) | ((rwd & 1ull) << (kStrip - 1));
+  };
+  const uint64_t mine = near(j);
+  const uint64_t edge = l == 0 ? near(j0 - 1) : (l == 63 ? near(j0 + 64) : 0ull);
+  // both shuffles with every lane active (a lane masked off during a lane
+  // permute would hand its neighbour zero), then the edge rows selected
+  const uint64_t dn_ = __shfl_up(mine, 1), up_ = __shfl_down(mine, 1);
+  const uint64_t dn = l == 0 ? edge : dn_;   // row j - 1
+  const uint64_t up = l == 63 ? edge : up_;  // row j + 1
+  // columns whose 5 x 5 stencil stays off the wall ghosts, 3 <= i <= nx - 2
+  const int i0 = 1 + b * kStrip;
+  const int lo = max(3 - i0, 0), hi = min(nx - 2 - i0, kStrip - 1);
+  const uint64_t inside = hi < lo ? 0ull : ((~0ull >> (63 - hi)) & (~0ull << lo));
+  uint64_t adm = j <= ny ? (dn | mine | up) & inside : 0ull;
+  const unsigned n = (unsigned)__popcll(adm);
+  unsigned incl = n;  // inclusive prefix sum over the lanes
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const unsigned v = __shfl_up(incl, d);
+    if (l >= d) incl += v;
+  }
+  const unsigned total = __shfl(incl, 63);
+  if (total == 0u) return;
+  unsigned base = 0u;
+  if (l == 63) base = atomicAdd(count, total);
+  unsigned pos = __shfl(base, 63) + incl - n;
+  const unsigned row = (unsigned)(j - 1) * (unsigned)nx + (unsigned)(i0 - 1);
+  while (adm) {
+    list[pos++] = row + (unsigned)(__ffsll((unsigned long long)adm) - 1);
+    adm &= adm - 1ull;
   }
 }
 
@@ -892,14 +946,20 @@ int curv_field_init(float* kappa, unsigned* list, long cells, hipStream_t st) {
 // the learned curvature of a stage's input vof: the previous launch's cells
 // back to NaN, then the gate lists the cells of this vof and the network
 // fills them
-int curv_mlp_launch(const double* vof, float* kappa, unsigned* list, int nx, int ny, double h, hipStream_t st) {
+int curv_mlp_launch(const double* vof, const unsigned long long* cut, float* kappa, unsigned* list, int nx, int ny,
+                    double h, hipStream_t st) {
   using namespace mlp;
   SBFS_REQUIRE(g_net.trained, "learned curvature: no trained network (sbfs_mlp_train or sbfs_mlp_set_params)");
   SBFS_TIMED("k_curv_restore", hipLaunchKernelGGL(k_curv_restore, dim3(256), dim3(256), 0, st, kappa,
                                                   (const unsigned*)list));
   hipLaunchKernelGGL(k_curv_reset, dim3(1), dim3(64), 0, st, list);
-  SBFS_TIMED("k_curv_gate", hipLaunchKernelGGL(k_curv_gate, dim3(cdiv(nx, 64), cdiv(ny, 4 * kGateRows)), dim3(256),
-                                               0, st, vof, nx, ny, list, list + 1));
+  if (cut)
+    SBFS_TIMED("k_curv_gate_cut",
+               hipLaunchKernelGGL(k_curv_gate_cut, dim3(march_strips(nx), cdiv(ny, 4 * 64)), dim3(256), 0, st,
+                                  cut, march_strips(nx), nx, ny, list, list + 1));
+  else
+    SBFS_TIMED("k_curv_gate", hipLaunchKernelGGL(k_curv_gate, dim3(cdiv(nx, kGateCols), cdiv(ny, 4 * kGateRows)),
+                                                 dim3(256), 0, st, vof, nx, ny, list, list + 1));
   SBFS_TIMED("k_curv_mlp", hipLaunchKernelGGL(k_curv_mlp, dim3(4096), dim3(64), 0, st, weights(), vof, kappa,
                                               nx, ny, (float)(1. / h), (const unsigned*)list,
                                               (const unsigned*)(list + 1)));
@@ -1127,7 +1187,7 @@ extern "C" int sbfs_mlp_coverage(const domain_t* domain, const interface_t* inte
   SBFS_HIP(hipMalloc(&cnt, sizeof(unsigned long long)));
   SBFS_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), stream()));
   int rc = curv_field_init(k, list, n, stream());
-  if (!rc) rc = curv_mlp_launch((const double*)interface->vof.data, k, list, nx, ny, domain->hy, stream());
+  if (!rc) rc = curv_mlp_launch((const double*)interface->vof.data, nullptr, k, list, nx, ny, domain->hy, stream());
   unsigned long long h = 0;
   if (!rc) {
     hipLaunchKernelGGL(k_count_learned, dim3(1024), dim3(256), 0, stream(), (const float*)k, n, cnt);

@@ -154,6 +154,9 @@ int npy_write(const char* dir, const char* name, const char* descr,
               const std::vector<size_t>& shape, size_t elem, const void* data);
 int load_dt_coefs();
 
+// the learned curvature's band: cells within one cell (3 x 3 block) of a
+// cut cell, vof in [kCutLo, kCutHi]
+constexpr double kCutLo = 0.02, kCutHi = 0.98;
 // learned curvature (mlp.hip): kappa of the interior cells of `vof`, NaN
 // where the plain curvature stays; h is the (uniform, square) cell size.
 // `list` (nx ny + 1 entries: the count, then the listed cells) belongs to
@@ -161,7 +164,10 @@ int load_dt_coefs();
 // and each curv_mlp_launch first returns the cells its predecessor listed
 // to NaN, so only listed cells ever hold a value.
 int curv_field_init(float* kappa, unsigned* list, long cells, hipStream_t st);
-int curv_mlp_launch(const double* vof, float* kappa, unsigned* list, int nx, int ny, double h, hipStream_t st);
+// `cut`, when not null, is the cut-cell bitmask that the stage kernel wrote
+// with this vof (march.hpp), which the gate then reads instead of the vof.
+int curv_mlp_launch(const double* vof, const unsigned long long* cut, float* kappa, unsigned* list, int nx, int ny,
+                    double h, hipStream_t st);
 // Captured step graphs hold device addresses that their keys do not list
 // (Poisson plan buffers, domain metrics).
 // Whatever frees such memory bumps this epoch, which is part of every key,
