@@ -257,10 +257,10 @@ static void swap_data(array_t& x, array_t& y) {
   y.data = t;
 }
 
-__global__ void k_march(MarchArgs a);             // march.hip, plain FP64
-__global__ void k_march_corr(MarchArgs a);        // + the previous stage's projection
-__global__ void k_march_mixed(MarchArgs a);       // FP32 stencils
-__global__ void k_march_mixed_corr(MarchArgs a);  // FP32 stencils + projection
+__global__ void k_march(const MarchArgs* a);             // march.hip, plain FP64
+__global__ void k_march_corr(const MarchArgs* a);        // + the previous stage's projection
+__global__ void k_march_mixed(const MarchArgs* a);       // FP32 stencils
+__global__ void k_march_mixed_corr(const MarchArgs* a);  // FP32 stencils + projection
 
 // Stages 1 and 2 fold the previous stage's projection and pressure update
 // into their stage kernel (k_march_corr), which reads the predicted velocity,
@@ -424,6 +424,58 @@ static void fill_post(const DomainCtx* c, const fluid_t* fluid, const interface_
   q.ny = (int)c->ny;
 }
 
+// The stage kernels' arguments in device memory (the kernels read them
+// through a pointer, march.hip).  A step configuration builds the same
+// arguments at every step, so they sit in a small cache keyed by their
+// bytes: the eager first pass of a configuration uploads them (a
+// synchronous copy, allowed there), and the graph captured on its second
+// pass finds the same bytes and records only the pointer.  The cache is
+// emptied when the graph epoch moves (buffers were freed) or when it fills.
+struct ArgsSlot {
+  MarchArgs host;
+  MarchArgs* dev;
+};
+static std::vector<ArgsSlot> g_args;
+static uint64_t g_args_epoch = 0;
+constexpr size_t kMaxArgSlots = 64;
+
+static void args_clear() {
+  (void)hipStreamSynchronize(stream());
+  for (ArgsSlot& s : g_args) (void)hipFree(s.dev);
+  g_args.clear();
+}
+
+static int device_args(const MarchArgs& m, const MarchArgs** out) {
+  if (g_args_epoch != graph_epoch()) {
+    args_clear();
+    g_args_epoch = graph_epoch();
+  }
+  for (const ArgsSlot& s : g_args)
+    if (memcmp(&s.host, &m, sizeof(m)) == 0) {
+      *out = s.dev;
+      return 0;
+    }
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  SBFS_HIP(hipStreamIsCapturing(stream(), &cs));
+  SBFS_REQUIRE(cs == hipStreamCaptureStatusNone,
+               "integrate: stage arguments of a captured step were not prepared by its eager pass");
+  if (g_args.size() >= kMaxArgSlots) {
+    // captured graphs may hold the old pointers: move the epoch so that
+    // none of them replays
+    args_clear();
+    graph_epoch_bump();
+    g_args_epoch = graph_epoch();
+  }
+  MarchArgs* d = nullptr;
+  SBFS_HIP(hipMalloc(&d, sizeof(MarchArgs)));
+  const hipError_t e = hipMemcpy(d, &m, sizeof(MarchArgs), hipMemcpyHostToDevice);
+  if (e != hipSuccess) (void)hipFree(d);
+  SBFS_HIP(e);
+  g_args.push_back({m, d});
+  *out = d;
+  return 0;
+}
+
 // Every launch of one fused time step (decide_dt, then per Runge-Kutta stage
 // the stage kernel, the Poisson solve and the projection) plus the host-side
 // pointer swaps the reference performs.  No allocation and no host
@@ -479,6 +531,7 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
     }
     {
       MarchArgs m;
+      memset(&m, 0, sizeof(m));  // padding too: the arguments are compared bytewise
       fill_march(domain, c, fluid, interface, cf, rhs, band, k, corr, us, vs, g_scr.vof, g_scr.p, dtp, m);
       if (learned) {
         if (overlap) {
@@ -496,14 +549,16 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
         m.weight = g_scr.weight;
       }
       const dim3 grid = ordered ? dim3(units) : dim3(cdiv(nx, kStrip), cdiv(ny, band)), block(64);
+      const MarchArgs* ma = nullptr;
+      if (device_args(m, &ma)) return 1;
       if (mixed && corr)
-        SBFS_TIMED("k_march_mixed_corr", hipLaunchKernelGGL(k_march_mixed_corr, grid, block, 0, stream(), m));
+        SBFS_TIMED("k_march_mixed_corr", hipLaunchKernelGGL(k_march_mixed_corr, grid, block, 0, stream(), ma));
       else if (mixed)
-        SBFS_TIMED("k_march_mixed", hipLaunchKernelGGL(k_march_mixed, grid, block, 0, stream(), m));
+        SBFS_TIMED("k_march_mixed", hipLaunchKernelGGL(k_march_mixed, grid, block, 0, stream(), ma));
       else if (corr)
-        SBFS_TIMED("k_march_corr", hipLaunchKernelGGL(k_march_corr, grid, block, 0, stream(), m));
+        SBFS_TIMED("k_march_corr", hipLaunchKernelGGL(k_march_corr, grid, block, 0, stream(), ma));
       else
-        SBFS_TIMED("k_march", hipLaunchKernelGGL(k_march, grid, block, 0, stream(), m));
+        SBFS_TIMED("k_march", hipLaunchKernelGGL(k_march, grid, block, 0, stream(), ma));
       if (check_launch("integrate: k_march")) return 1;
       if (corr) {
         // the stage kernel wrote the updated pressure of stage k-1

@@ -810,9 +810,15 @@ __global__ __launch_bounds__(1024) void k_march_order(const unsigned* __restrict
 // would spill dozens of registers to scratch, measured 1.4x slower).  The
 // FP32 bodies fit 168 VGPRs, three wavefronts per SIMD, which hides more of
 // their latency (the folded variant 1.35x faster than at two).
+// The arguments arrive through a pointer to device memory, not by value:
+// held as kernel arguments, the ~50 uniform pointers and constants of
+// MarchArgs outgrow the scalar registers, and the compiler spills them into
+// vector lanes (v_writelane / v_readlane in the row loop, up to 185 of them
+// in the FP64 folded body); from memory it reloads them with scalar loads.
 #define SBFS_MARCH_KERNEL(name, R, CORR, WAVES)                                            \
   __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES))) void name( \
-      MarchArgs a) {                                                                      \
+      const MarchArgs* __restrict__ args) {                                               \
+    const MarchArgs& a = *args;                                                           \
     int bx = blockIdx.x, by = blockIdx.y;                                                 \
     if (a.order) {                                                                        \
       const unsigned u = a.order[blockIdx.x];                                             \
