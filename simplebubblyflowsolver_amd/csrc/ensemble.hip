@@ -337,6 +337,11 @@ extern "C" int sbfs_ensemble_create(const sbfs_case_t* cases, int n, sbfs_ensemb
   for (int c = 0; c < n && !rc; c++) {
     auto& k = e->cs[c];
     const sbfs_case_t& q = cases[c];
+    if (!fp64_march_addressable(q.nx, q.ny)) {
+      set_error("sbfs_ensemble_create: case %d (%zu x %zu): fields of 4 GiB or more", c, q.nx, q.ny);
+      rc = 1;
+      break;
+    }
     rc = sbfs_domain_create(q.nx, q.ny, q.lx, q.ly, &k.d);
     if (!rc) rc = sbfs_fluid_create(&k.d, q.Re, q.Fr, q.denr, q.visr, &k.f);
     if (!rc) rc = sbfs_interface_create(&k.d, q.We, &k.s);

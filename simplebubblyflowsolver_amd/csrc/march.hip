@@ -56,23 +56,46 @@ __device__ __forceinline__ double from_right(double v) {
   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
-// The element of a wave-uniform row at the lane's byte offset.  The row
-// address is pinned to SGPRs (readfirstlane of a uniform value), so every
-// access takes the scalar-base + 32-bit vector-offset form and all arrays
-// share one offset VGPR; left to itself the compiler hoists a 64-bit per-lane
-// pointer per array out of the row loop, two VGPRs each.
+// Two ways to address a wave-uniform row, so that every access is a scalar
+// base plus the lane's byte offset:
+//  * lane_at: the row's 64-bit address, pinned to SGPRs (readfirstlane of a
+//    uniform value), plus the lane offset; all arrays share one offset VGPR.
+//    The FP32 bodies use it: they run three wavefronts per SIMD at 168 VGPRs
+//    and have none to spare.
+//  * RowRef: the field's base and a 32-bit byte offset of lane 0's element,
+//    plus the lane offset; rows of one layout share one offset for all
+//    their fields, where a 64-bit row address per field and row costs
+//    several scalar or vector operations.  The FP64 bodies use it (2 %
+//    faster there; in the FP32 bodies its offsets push the register count
+//    into scratch).  The fused path keeps every field below 4 GiB.  Lanes
+//    whose column lies before the array wrap the 32-bit offset; they are
+//    the masked halo lanes and access nothing.
+// Both go through a global-address-space pointer, so the accesses stay
+// global_load / global_store (a generic pointer made from an integer would
+// compile to flat accesses, which also count against the LDS wait counter).
+template <typename T>
+using GlobalBytes =
+    __attribute__((address_space(1))) typename std::conditional<std::is_const<T>::value, const char, char>::type;
+
 template <typename T>
 __device__ __forceinline__ T& lane_at(T* row, unsigned loff) {
   const unsigned long long u = (unsigned long long)row;
   const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
   const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
-  // a global-address-space pointer, so the accesses stay global_load /
-  // global_store (a generic pointer made from an integer would compile to
-  // flat accesses, which also count against the LDS wait counter)
-  using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
-  using GB = __attribute__((address_space(1))) B;
-  GB* base = (GB*)(((unsigned long long)hi << 32) | lo);
+  GlobalBytes<T>* base = (GlobalBytes<T>*)(((unsigned long long)hi << 32) | lo);
   return *(T*)(base + loff);
+}
+
+template <typename T>
+struct RowRef {
+  T* base;
+  unsigned off;
+  __device__ __forceinline__ RowRef operator+(int d) const { return {base, off + (unsigned)(d * (int)sizeof(T))}; }
+};
+template <typename T>
+__device__ __forceinline__ T& lane_at(RowRef<T> r, unsigned loff) {
+  GlobalBytes<T>* base = (GlobalBytes<T>*)r.base;
+  return *(T*)(base + (r.off + loff));
 }
 
 // A wave-uniform value pinned to SGPRs.  FP64 arithmetic has no scalar
@@ -234,23 +257,20 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
   const double pdtc = uni(pdtn * pcoef), irefden = uni(1. / a.refden);
   const R refdeninv = uni(R(a.refdeninv)), tension = uni(R(a.tension)), grav = uni(R(a.grav));
 
-  // Row pointers are uniform and shifted to lane 0's column, so every access
-  // is a scalar base plus the lane's byte offset.
+  // Rows of each layout, shifted to lane 0's column: a pinned row pointer
+  // (FP32 bodies) or a (field, byte offset) pair (FP64 bodies), see RowRef.
   const ptrdiff_t wc = nx + 2, wf = nx + 1;
-  auto vof_row = [&](auto* f, int j) { return f + (ptrdiff_t)(j + 1) * wc + cl; };   // L::vof
-  auto cc_row = [&](auto* f, int j) { return f + (ptrdiff_t)j * wc + cl; };    // L::cc
-  auto xf_row = [&](auto* f, int j) { return f + (ptrdiff_t)j * wf + cl - 1; };  // L::xf
-  auto ci_row = [&](auto* f, int j) { return f + (ptrdiff_t)(j - 1) * nx + cl - 1; };  // L::ci
-  auto fx_row = [&](auto* f, int j) { return f + (ptrdiff_t)(j - 1) * (nx - 1) + cl - 2; };  // L::fx
-  const unsigned loff = (unsigned)lane * 8u;
-  // Pinned (scalar-base) addressing frees the VGPRs the folded projection
-  // and the three-wavefront FP32 body need; the plain FP64 body is faster
-  // with the compiler's own per-lane pointers.
-  constexpr bool kPinned = CORR || std::is_same<R, float>::value;
-  auto AT = [&](auto* row) -> auto& {
-    if constexpr (kPinned) return lane_at(row, loff);
-    else return row[lane];
+  auto row = [&](auto* f, ptrdiff_t e) {
+    if constexpr (std::is_same<R, float>::value) return f + e;
+    else return RowRef<std::remove_reference_t<decltype(*f)>>{f, (unsigned)(e * (ptrdiff_t)sizeof(*f))};
   };
+  auto vof_row = [&](auto* f, int j) { return row(f, (ptrdiff_t)(j + 1) * wc + cl); };         // L::vof
+  auto cc_row = [&](auto* f, int j) { return row(f, (ptrdiff_t)j * wc + cl); };                // L::cc
+  auto xf_row = [&](auto* f, int j) { return row(f, (ptrdiff_t)j * wf + cl - 1); };            // L::xf
+  auto ci_row = [&](auto* f, int j) { return row(f, (ptrdiff_t)(j - 1) * nx + cl - 1); };      // L::ci
+  auto fx_row = [&](auto* f, int j) { return row(f, (ptrdiff_t)(j - 1) * (nx - 1) + cl - 2); };  // L::fx
+  const unsigned loff = (unsigned)lane * 8u;
+  auto AT = [&](auto r) -> auto& { return lane_at(r, loff); };
   // loads of row jw, already wrapped into 1..ny
   auto ld_vof = [&](int jw) { return c_vof ? AT(vof_row(a.vof, jw)) : 0.; };
   auto ld_ux = [&](int jw) { return c_fx ? AT(xf_row(a.ux, jw)) : 0.; };

@@ -154,6 +154,10 @@ int npy_write(const char* dir, const char* name, const char* descr,
               const std::vector<size_t>& shape, size_t elem, const void* data);
 int load_dt_coefs();
 
+// The FP64 stage kernels address a field by its base and a 32-bit byte
+// offset (march.hip): every field of the grid, the vof with its two halo
+// rows on each side the largest, must stay below 4 GiB.
+inline bool fp64_march_addressable(size_t nx, size_t ny) { return (ny + 4) * (nx + 2) * 8 < (1ull << 32); }
 // the learned curvature's band: cells within one cell (3 x 3 block) of a
 // cut cell, vof in [kCutLo, kCutHi]
 constexpr double kCutLo = 0.02, kCutHi = 0.98;

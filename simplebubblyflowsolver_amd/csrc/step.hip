@@ -303,7 +303,15 @@ int decide_dt(const domain_t* domain, const fluid_t* fluid, const interface_t* i
 int integrate(const domain_t* domain, fluid_t* fluid, interface_t* interface, double* dt) {
   if (init_runtime()) return 1;
   SBFS_REQUIRE(domain && fluid && interface && dt, "integrate: null argument");
-  if (options().path == SBFS_PATH_FUSED) return integrate_fused(domain, fluid, interface, dt);
+  if (options().path == SBFS_PATH_FUSED) {
+    if (options().precision == SBFS_PRECISION_MIXED ||
+        fp64_march_addressable(domain->mysizes[0], domain->mysizes[1]))
+      return integrate_fused(domain, fluid, interface, dt);
+    // fields of 4 GiB or more: the FP64 step runs on the staged path
+    SBFS_REQUIRE(options().curvature != SBFS_CURVATURE_LEARNED,
+                 "integrate: the learned curvature runs on the fused path, whose FP64 step needs fields below 4 GiB "
+                 "(%zu x %zu cells); use mixed precision", domain->mysizes[0], domain->mysizes[1]);
+  }
   return integrate_staged(domain, fluid, interface, dt);
 }
 

@@ -94,3 +94,14 @@ def test_missing_file_and_null_arguments(lib, tmp_path):
     dom = abi.DomainT()
     assert lib.domain_init(None, C.byref(dom)) != 0
     assert "null argument" in lib.sbfs_last_error().decode()
+
+
+def test_ensemble_refuses_fields_of_4_gib(lib):
+    """The FP64 stage kernels address a field by a 32-bit byte offset
+    (simplebubblyflowsolver_amd/csrc/march.hip), so a case whose vof field
+    reaches 4 GiB is refused before anything is allocated."""
+    lib.sbfs_ensemble_create.argtypes = [C.POINTER(solver.CaseSpec), C.c_int, C.POINTER(C.c_void_p)]
+    case = solver.CaseSpec(32768, 16384, 1.0, 0.5, 35.0, 1.0, 1e-3, 1e-2, 10.0)
+    handle = C.c_void_p()
+    assert lib.sbfs_ensemble_create(C.byref(case), 1, C.byref(handle)) != 0
+    assert "4 GiB" in lib.sbfs_last_error().decode()
