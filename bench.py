@@ -71,8 +71,8 @@ BYTES_PER_CELL = {
     # fused path
     "k_march": 12 * 8, "k_march_mixed": 12 * 8, "k_dct2_rows": 16,
     "k_post": 72, "k_march_corr": (17 + 16) * 8 / 2, "k_march_mixed_corr": (17 + 16) * 8 / 2,
-    # learned curvature: the gate reads vof and writes the FP32 kappa field
-    "k_curv_gate": 12,
+    # learned curvature: the stage-0 gate reads the vof field
+    "k_curv_gate": 8,
 }
 
 # HBM bytes per launch measured with rocprofv3 counters (FETCH_SIZE and
@@ -82,7 +82,7 @@ PROFILES = ROOT / "profiles"
 
 # the learned curvature's training run (2^21 stencils)
 TRAIN = dict(seed=2, nsamples=1 << 21, epochs=40, batch=4096, lr=1e-3)
-MLP_FLOP_PER_SAMPLE = 3 * 2 * (25 * 64 + 64 * 64 + 64 * 1)  # forward + backward, 25-64-64-1
+MLP_FLOP_PER_SAMPLE = 3 * 2 * (26 * 64 + 64 * 64 + 64 * 1)  # forward + backward, 26-64-64-1
 BF16_PEAK_TFLOPS = 2500.0
 
 

@@ -44,7 +44,11 @@ extern int decide_dt_noalloc(const domain_t * domain, const fluid_t * fluid,
  * variables coef_dt_adv, coef_dt_dif, coef_dt_int) */
 extern int sbfs_set_dt_coefs(double adv, double dif, double intf);
 
-/* integrate() execution path */
+/* integrate() execution path.  The fused FP64 step addresses each field by
+ * a 32-bit byte offset: on grids whose fields reach 4 GiB (about 23000^2
+ * cells) integrate() runs the staged path instead (the same results), and
+ * the learned curvature, which runs on the fused path only, needs mixed
+ * precision there. */
 #define SBFS_PATH_STAGED 0  /* one kernel group per reference stage */
 #define SBFS_PATH_FUSED  1  /* fused per-substep kernels (default) */
 extern int sbfs_set_path(int path);
