@@ -121,8 +121,16 @@ enum { T_RHF, T_RHC, T_RJF, T_RJC, T_QF, T_QC, T_EQF, T_ECY, T_CYC, T_CYF, T_DFX
 
 // Arithmetic of the interface and momentum stencils.  FP64 is the reference's
 // arithmetic, expression for expression (stencils.hpp); FP32 is the mixed-
-// precision step's, with the device's fast reciprocal, square root,
-// exponential and logarithm.
+// precision step's, with the hardware's reciprocal, square root, exponential
+// and logarithm (v_rcp_f32, v_sqrt_f32, v_exp_f32, v_log_f32, about an ulp
+// each).  __fdividef, __fsqrt_rn and __logf do not map to these without
+// fast-math flags: they compile to the IEEE sequences (11, 6 and 13 VALU
+// instructions).
+// The FP64 parts (the density-weighted updates, the Poisson right-hand
+// side, the folded projection) divide by the column width and hy as the
+// reference does in the FP64 body; the mixed body, whose error budget is
+// FP32's, multiplies by the stored reciprocals (rhf, 1/hy) instead (per_h,
+// per_hy in march_body).
 template <typename R>
 struct Arith;
 
@@ -147,18 +155,20 @@ struct Arith<float> {
   static __device__ __forceinline__ float right(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
   }
-  static __device__ __forceinline__ float div(float a, float b) { return __fdividef(a, b); }
-  static __device__ __forceinline__ float sqr(float x) { return __fsqrt_rn(x); }
+  static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+  static __device__ __forceinline__ float sqr(float x) { return __builtin_amdgcn_sqrtf(x); }
   static __device__ __forceinline__ float eps() { return (float)DBL_EPSILON; }
   // tanh from one exponential: 1 - 2 / (e^{2|x|} + 1), sign restored
   static __device__ __forceinline__ float indicator(float sf) {
     const float x = (float)kVofBeta * sf;
-    const float t = 1.f - __fdividef(2.f, __expf(2.f * fabsf(x)) + 1.f);
+    const float t = 1.f - 2.f * __builtin_amdgcn_rcpf(__expf(2.f * fabsf(x)) + 1.f);
     return 0.5f * (1.f + copysignf(t, x));
   }
-  // 0.5 / beta (ln vof - ln(1 - vof)); 1 - vof in FP64 keeps the band edge
+  // 0.5 / beta (ln vof - ln(1 - vof)) from base-2 logarithms; 1 - vof in
+  // FP64 keeps the band edge
   static __device__ __forceinline__ float intercept(double v) {
-    return 0.5f / (float)kVofBeta * (__logf((float)v) - __logf((float)(1. - v)));
+    constexpr float c = (float)(0.5 / kVofBeta * 0.69314718055994530942);
+    return c * (__builtin_amdgcn_logf((float)v) - __builtin_amdgcn_logf((float)(1. - v)));
   }
 };
 
@@ -240,7 +250,12 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
   auto SB = [&](int t) { return R(st[t][lane]); };
   auto SL = [&](int t) { return R(st[t][ll]); };
   auto SR = [&](int t) { return R(st[t][lr]); };
+  // x / HXXF of the lane's column and x / hy: divided in the FP64 body,
+  // multiplied by the stored reciprocals in the mixed one
+  constexpr bool kExactQ = std::is_same<R, double>::value;
+  auto per_h = [&](double x) { return kExactQ ? x / TB(T_HXXF) : x * TB(T_RHF); };
   const double hy = a.hy, rhy = a.cf.rhy;
+  auto per_hy = [&](double x) { return kExactQ ? x / hy : x * rhy; };
   const R hyr = uni(R(hy)), rhyr = uni(R(rhy));
   const double dt = uni(*a.dt);
   // (dt * a.ca equals a.ca * dt bit for bit)
@@ -696,10 +711,10 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
       const double psm1v = ring_ld(RG_PSM1, psm1);
       const double psl = from_left(psm1v);
       const double dxm = ring_ld(RG_D1X, d1x_m1);
-      const double gxm = (a.refden / dxm - 1.) * (-psl + psm1v) / TB(T_HXXF);
+      const double gxm = per_h((a.refden / dxm - 1.) * (-psl + psm1v));
       const double qgx = TB(T_QF) * gxm;
       const double qgxr = from_right(qgx);
-      const double gyp = (a.refden / d1y - 1.) * (-psm1v + ps0) / hy;
+      const double gyp = per_hy((a.refden / d1y - 1.) * (-psm1v + ps0));
       const double cg_yp = TB(T_CYC) * gyp;
       const int r = s - 1;
       if (r >= jb && r <= je && c_out) {
@@ -733,9 +748,9 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
         double u = 0.;
         if (c >= 2 && c <= nx) {
           u = nux;
-          u -= pcx0 / TB(T_HXXF) * (-nps_l + nps);
+          u -= per_h(pcx0) * (-nps_l + nps);
           const double d = 0.5 * dbl + 0.5 * db;
-          u += pdtc * (1. / d - irefden) / TB(T_HXXF) * (-npo_l + npo);
+          u += per_h(pdtc * (1. / d - irefden)) * (-npo_l + npo);
         }
         nux = u;
       }
@@ -747,7 +762,7 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
           u = nuy;
           u -= pcy * (-psp1 + nps);
           const double d = 0.5 * da + 0.5 * db;
-          u += pdtc * (1. / d - irefden) / hy * (-pop1 + npo);
+          u += per_hy(pdtc * (1. / d - irefden)) * (-pop1 + npo);
         }
         nuy = u;
       }

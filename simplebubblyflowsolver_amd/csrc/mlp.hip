@@ -91,13 +91,16 @@ __host__ __device__ inline double uni(uint64_t seed, uint64_t i, uint64_t k) {
   return (double)(mix64(key) >> 11) * 0x1.0p-53;
 }
 
-// FP32 -> BF16, round to nearest even (finite inputs)
+// FP32 -> BF16, round to nearest even (finite inputs): the host form
 __host__ __device__ inline unsigned short f2bf(float x) {
   uint32_t u;
   memcpy(&u, &x, 4);
   u += 0x7FFFu + ((u >> 16) & 1u);
   return (unsigned short)(u >> 16);
 }
+// the same on the device, by gfx950's v_cvt_pk_bf16_f32 (round to nearest
+// even under the default mode; the same bits as f2bf for finite values)
+__device__ inline unsigned short bf(float x) { return __builtin_bit_cast(unsigned short, (__bf16)x); }
 
 // 1 + erf(x / sqrt 2) from erfc of z = |x| / sqrt 2: 2 - erfc(z) for
 // x >= 0, erfc(z) for x < 0, so that the negative side keeps its relative
@@ -131,10 +134,10 @@ __device__ inline f4 mma(s4 a, s4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16
 __device__ inline s4 ld4(const unsigned short* p) { return *(const s4*)p; }
 __device__ inline s4 pack4(float a, float b, float c, float d) {
   s4 r;
-  r[0] = (short)f2bf(a);
-  r[1] = (short)f2bf(b);
-  r[2] = (short)f2bf(c);
-  r[3] = (short)f2bf(d);
+  r[0] = (short)bf(a);
+  r[1] = (short)bf(b);
+  r[2] = (short)bf(c);
+  r[3] = (short)bf(d);
   return r;
 }
 
@@ -259,7 +262,7 @@ __global__ void k_mlp_samples(uint64_t seed, long n0, long n, unsigned short* __
   const double kp = plain_kh(c);
   float f[NF];
   features(c, kp, f);
-  for (int k = 0; k < NIP; k++) X[(size_t)s * NIP + k] = k < NF ? f2bf(f[k]) : 0;
+  for (int k = 0; k < NIP; k++) X[(size_t)s * NIP + k] = k < NF ? bf(f[k]) : 0;
   T[s] = (float)(label - kp);
   W[s] = (float)(1. / (label * label));
   if (Lab) Lab[s] = (float)label;
@@ -433,7 +436,7 @@ __global__ __launch_bounds__(256 * ST) void k_mlp_grad(Weights w, const unsigned
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       h1[r] += b1[r];
-      ab[r] = f2bf(gelu(h1[r]));
+      ab[r] = bf(gelu(h1[r]));
       a1h[(hb + r) * SP + sc] = ab[r];
     }
     *(uint2*)(a1s + sc * HP + hb) = make_uint2(ab[0] | ((unsigned)ab[1] << 16), ab[2] | ((unsigned)ab[3] << 16));
@@ -465,7 +468,7 @@ __global__ __launch_bounds__(256 * ST) void k_mlp_grad(Weights w, const unsigned
     for (int r = 0; r < 4; r++) {
       g3[r] = gelu(h2[r]) * dy;
       g2[r] = w3[r] * dy * dgelu(h2[r]);
-      db[r] = f2bf(g2[r]);
+      db[r] = bf(g2[r]);
       d2h[(hb + r) * SP + sc] = db[r];
     }
     *(uint2*)(d2s + sc * HP + hb) = make_uint2(db[0] | ((unsigned)db[1] << 16), db[2] | ((unsigned)db[3] << 16));
@@ -479,7 +482,7 @@ __global__ __launch_bounds__(256 * ST) void k_mlp_grad(Weights w, const unsigned
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       g1[r] = c[r] * dgelu(h1[r]);
-      d1h[(hb + r) * SP + sc] = f2bf(g1[r]);
+      d1h[(hb + r) * SP + sc] = bf(g1[r]);
     }
   }
   // bias gradients: sums over the subtile's 16 samples
@@ -596,11 +599,11 @@ __global__ __launch_bounds__(256) void k_mlp_adam(AdamState s, const float* __re
   s.v[k] = v;
   const float p = s.p[k] - lr * (m / b1t) / (sqrtf(v / b2t) + eps);
   s.p[k] = p;
-  if (k >= OW1 && k < OB1) s.w1b[k - OW1] = f2bf(p);
+  if (k >= OW1 && k < OB1) s.w1b[k - OW1] = bf(p);
   if (k >= OW2 && k < OB2) {
     const int o = (k - OW2) / NH, i = (k - OW2) % NH;
-    s.w2b[o * NH + i] = f2bf(p);
-    s.w2tb[i * NH + o] = f2bf(p);
+    s.w2b[o * NH + i] = bf(p);
+    s.w2tb[i * NH + o] = bf(p);
   }
 }
 
@@ -608,11 +611,11 @@ __global__ void k_mlp_bf16(AdamState s) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= NP) return;
   const float p = s.p[k];
-  if (k >= OW1 && k < OB1) s.w1b[k - OW1] = f2bf(p);
+  if (k >= OW1 && k < OB1) s.w1b[k - OW1] = bf(p);
   if (k >= OW2 && k < OB2) {
     const int o = (k - OW2) / NH, i = (k - OW2) % NH;
-    s.w2b[o * NH + i] = f2bf(p);
-    s.w2tb[i * NH + o] = f2bf(p);
+    s.w2b[o * NH + i] = bf(p);
+    s.w2tb[i * NH + o] = bf(p);
   }
 }
 
@@ -756,7 +759,7 @@ __global__ __launch_bounds__(64) void k_curv_mlp(Weights w, const double* __rest
     {
       float f[NF];
       features(cv, kp, f);
-      for (int k = 0; k < NIP; k++) sx[l * NIP + k] = k < NF ? f2bf(f[k]) : 0;
+      for (int k = 0; k < NIP; k++) sx[l * NIP + k] = k < NF ? bf(f[k]) : 0;
     }
     __syncthreads();
     // lane l = 16 lg + l%16 keeps the prediction of subtile lg
