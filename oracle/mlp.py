@@ -22,6 +22,9 @@ convention (kappa > 0 with the gas, vof = 1, inside) the labels follow.
 * `plain_curvature`: the reference's curvature of the centre cell of a
   stencil with h = 1 (curvature_tensor.c:16-49 corner gradients, :100-135
   divergence). The tests pin it against the C oracle on a full grid.
+* `_gelu`: GELU in Hendrycks & Gimpel's tanh form, pinned against torch's
+  gelu(approximate="tanh"), the erf form x Phi(x) (within 4.8e-4) and the
+  identity gelu(-x) = gelu(x) - x.
 * `loss_grad64`: forward and backward of the weighted loss in float64 with
   no rounding, pinned by central finite differences.
 * `loss_grad_bf16`: the same with the device kernels' BF16 roundings (inputs,
@@ -31,7 +34,6 @@ convention (kappa > 0 with the gas, vof = 1, inside) the labels follow.
 from __future__ import annotations
 
 import numpy as np
-from scipy.special import erf
 
 NI, NF, NIP, NH = 25, 26, 32, 64  # stencil values, features used, padded, hidden
 OW1 = 0
@@ -175,12 +177,19 @@ def glorot(seed: int) -> np.ndarray:
     return p
 
 
+# GELU in the tanh form of Hendrycks & Gimpel (torch's gelu(approximate="tanh")):
+# x (1 + tanh u) / 2, u = sqrt(2/pi) (x + 0.044715 x^3)
+_GK0 = np.sqrt(2.0 / np.pi)
+_GK1 = 0.044715 * _GK0
+
+
 def _gelu(x):
-    return 0.5 * x * (1.0 + erf(x / np.sqrt(2.0)))
+    return 0.5 * x * (1.0 + np.tanh(x * (_GK0 + _GK1 * x * x)))
 
 
 def _dgelu(x):
-    return 0.5 * (1.0 + erf(x / np.sqrt(2.0))) + x * np.exp(-0.5 * x * x) / np.sqrt(2.0 * np.pi)
+    th = np.tanh(x * (_GK0 + _GK1 * x * x))
+    return 0.5 * (1.0 + th) + 0.5 * x * (1.0 - th * th) * (_GK0 + 3.0 * _GK1 * x * x)
 
 
 def _unpack(p):

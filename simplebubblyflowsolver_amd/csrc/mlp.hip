@@ -102,32 +102,25 @@ __host__ __device__ inline unsigned short f2bf(float x) {
 // even under the default mode; the same bits as f2bf for finite values)
 __device__ inline unsigned short bf(float x) { return __builtin_bit_cast(unsigned short, (__bf16)x); }
 
-// 1 + erf(x / sqrt 2) from erfc of z = |x| / sqrt 2: 2 - erfc(z) for
-// x >= 0, erfc(z) for x < 0, so that the negative side keeps its relative
-// accuracy (1 + erf there is a small difference of O(1) numbers).  erfc is
-// the Chebyshev fit of Numerical Recipes (erfcc, fractional error below
-// 1.2e-7), t exp(-z^2 + P(t)) with t = 1 / (1 + z/2); in FP32 the rounding
-// of the exponent keeps it within 1e-5 relative.  One reciprocal and one
-// exponential, where erff takes several branches' worth of work.
-__device__ inline float one_plus_erf(float x) {
-  const float z = fabsf(x) * 0.70710678118654752f;
-  const float t = __builtin_amdgcn_rcpf(fmaf(0.5f, z, 1.f));
-  float p = 0.17087277f;
-  p = fmaf(t, p, -0.82215223f);
-  p = fmaf(t, p, 1.48851587f);
-  p = fmaf(t, p, -1.13520398f);
-  p = fmaf(t, p, 0.27886807f);
-  p = fmaf(t, p, -0.18628806f);
-  p = fmaf(t, p, 0.09678418f);
-  p = fmaf(t, p, 0.37409196f);
-  p = fmaf(t, p, 1.00002368f);
-  p = fmaf(t, p, -1.26551223f);
-  const float c = t * __expf(fmaf(-z, z, p));
-  return x >= 0.f ? 2.f - c : c;
+// GELU in the tanh form of Hendrycks & Gimpel, x (1 + tanh u) / 2 with
+// u = sqrt(2/pi) (x + 0.044715 x^3), written as x s with s = sigma(2u) =
+// 1 / (1 + 2^(-2u log2 e)): one exponential and one reciprocal.  It stays
+// within 4.8e-4 of the erf form x Phi(x) (at x = 2.7), 20 times below the
+// BF16 step of the activations there; training and inference share it.
+// x -> -inf: 2^(...) overflows to inf and s to 0, so gelu -> -0.
+constexpr float kGeluA = 2.3022082f;    // 2 sqrt(2/pi) log2 e
+constexpr float kGeluB = 0.10294324f;   // 2 sqrt(2/pi) 0.044715 log2 e
+constexpr float kGeluC = 1.5957692f;    // 2 sqrt(2/pi)
+constexpr float kGeluD = 0.21406445f;   // 6 sqrt(2/pi) 0.044715
+__device__ inline float gelu_s(float x) {
+  const float e = __builtin_amdgcn_exp2f(-x * fmaf(kGeluB, x * x, kGeluA));
+  return __builtin_amdgcn_rcpf(1.f + e);
 }
-__device__ inline float gelu(float x) { return 0.5f * x * one_plus_erf(x); }
+__device__ inline float gelu(float x) { return x * gelu_s(x); }
+// d/dx x s = s + x s (1 - s) d(2u)/dx
 __device__ inline float dgelu(float x) {
-  return 0.5f * one_plus_erf(x) + x * 0.39894228040143268f * __expf(-0.5f * x * x);
+  const float sg = gelu_s(x);
+  return fmaf(x * fmaf(-sg, sg, sg), fmaf(kGeluD, x * x, kGeluC), sg);
 }
 
 __device__ inline f4 mma(s4 a, s4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
@@ -641,14 +634,15 @@ __global__ __launch_bounds__(64) void k_mlp_predict(Weights w, const unsigned sh
 }
 
 // ---------------------------------------------------------------------------
-// inference on the solver's vof, in two launches.  k_curv_gate (one thread
-// per cell, light) writes NaN to every cell and appends the cells whose
-// rows j-1..j+1, columns i-1..i+1 hold a cut cell (vof in [kUseLo, kUseHi])
-// and whose stencil stays off the wall ghosts to a list; k_curv_mlp walks
+// inference on the solver's vof.  A gate (k_curv_gate from the vof field,
+// k_curv_gate_cut from the stage kernel's cut-cell bitmask) lists the cells
+// whose rows j-1..j+1, columns i-1..i+1 hold a cut cell (vof in [kUseLo,
+// kUseHi]) and whose stencil stays off the wall ghosts; k_curv_mlp walks
 // that list 64 cells per wavefront with a fixed grid and writes
-// kappa = (kp + y) / h in the cells the rule above admits (NaN stays in the
-// others).  Layout of kappa and of the list entries: interior cells,
-// (j-1) Nx + (i-1).
+// kappa = (kp + y) / h in the cells the rule below admits (NaN stays in the
+// others).  The list's order follows the gates' atomics and differs from
+// run to run; each cell's value does not depend on it.  Layout of kappa
+// and of the list entries: interior cells, (j-1) Nx + (i-1).
 // ---------------------------------------------------------------------------
 // The kappa field and its list of cells (list[0] the count, then the cell
 // indices) are kept together: every cell outside the list holds NaN.  A
@@ -668,18 +662,44 @@ __global__ void k_curv_reset(unsigned* __restrict__ count) {
   if (threadIdx.x == 0) *count = 0u;
 }
 
-// one strip of kGateCols columns and kGateRows rows per wavefront, four
-// per workgroup: lane l holds column kGateCols bx + l, lanes 0 and 63 the
+// Reserves `total` (uniform over the wavefront) list slots for each of the
+// NW wavefronts of a workgroup with one atomic on the shared counter and
+// returns the wavefront's first slot; every wavefront of the workgroup calls
+// it.  Atomics on one address serialise in its L2 channel: with one per
+// wavefront the gates took 46-70 us on the 4096 x 8192 swarm, most of it
+// waiting on the counter.
+template <int NW>
+__device__ inline unsigned reserve_slots(unsigned* count, unsigned total) {
+  __shared__ unsigned part[NW + 1];
+  const int w = (int)(threadIdx.x >> 6);
+  if ((threadIdx.x & 63) == 0) part[w] = total;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned sum = 0u;
+    for (int k = 0; k < NW; k++) {
+      const unsigned t = part[k];
+      part[k] = sum;
+      sum += t;
+    }
+    part[NW] = sum ? atomicAdd(count, sum) : 0u;
+  }
+  __syncthreads();
+  return part[NW] + part[w];
+}
+
+// one strip of kGateCols columns and kGateRows rows per wavefront, NW per
+// workgroup: lane l holds column kGateCols bx + l, lanes 0 and 63 the
 // columns on either side of the strip, so the 3 x 3 block of every strip
 // cell lies in the wavefront's own loads (one per lane and row, all issued
 // before the first test)
-constexpr int kGateCols = 62, kGateRows = 32;
-__global__ __launch_bounds__(256) void k_curv_gate(const double* __restrict__ vof, int nx, int ny,
-                                                   unsigned* __restrict__ count, unsigned* __restrict__ list) {
+constexpr int kGateCols = 62, kGateRows = 32, kGateWaves = 8;
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void k_curv_gate(const double* __restrict__ vof, int nx, int ny,
+                                                       unsigned* __restrict__ count, unsigned* __restrict__ list) {
   const int l = threadIdx.x & 63;
   const int i = (int)blockIdx.x * kGateCols + l;
-  const int j0 = 1 + ((int)blockIdx.y * 4 + (int)(threadIdx.x >> 6)) * kGateRows;
-  if (j0 > ny) return;  // whole wavefronts
+  // wavefronts past row ny load the clamped last row and list nothing
+  const int j0 = 1 + ((int)blockIdx.y * NW + (int)(threadIdx.x >> 6)) * kGateRows;
   const size_t wc = (size_t)nx + 2;
   auto in_use = [](double v) { return (v >= kUseLo) & (v <= kUseHi); };
   // columns outside 1..nx never count; rows past ny + 1 of the last strip
@@ -705,12 +725,8 @@ __global__ __launch_bounds__(256) void k_curv_gate(const double* __restrict__ vo
     adm[k] = __ballot(j <= ny && inside && ((nb >> l) & 1ull));
     total += (unsigned)__popcll(adm[k]);
   }
+  unsigned base = reserve_slots<NW>(count, total);
   if (total == 0u) return;
-  // one atomic per wavefront reserves its cells' slots (a shared counter:
-  // finer atomics on it serialise)
-  unsigned base = 0u;
-  if (l == 0) base = atomicAdd(count, total);
-  base = __shfl(base, 0);
   const uint64_t below = l == 0 ? 0ull : (~0ull >> (64 - l));
 #pragma unroll
   for (int k = 0; k < kGateRows; k++) {
@@ -720,10 +736,13 @@ __global__ __launch_bounds__(256) void k_curv_gate(const double* __restrict__ vo
   }
 }
 
-__global__ __launch_bounds__(64) void k_curv_mlp(Weights w, const double* __restrict__ vof,
-                                                 float* __restrict__ kappa, int nx, int ny, float inv_h,
-                                                 const unsigned* __restrict__ count,
-                                                 const unsigned* __restrict__ list) {
+// Three wavefronts per SIMD: the body fits 168 VGPRs without spills (the FP64
+// stencil and its canonical image hold about 100 of them), and the third
+// wavefront hides more of the stencil gathers (measured 95 -> 88 us per
+// launch on the 4096 x 8192 swarm; four would spill about 40 registers).
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_curv_mlp(
+    Weights w, const double* __restrict__ vof, float* __restrict__ kappa, int nx, int ny, float inv_h,
+    const unsigned* __restrict__ count, const unsigned* __restrict__ list) {
   __shared__ __align__(16) unsigned short sx[64 * NIP];
   const int l = threadIdx.x, lg = l >> 4;
   const unsigned n = *count;
@@ -784,13 +803,16 @@ __global__ __launch_bounds__(64) void k_curv_mlp(Weights w, const double* __rest
 // rows, lane l row j0 + l: the lane dilates its row's word by one column
 // each way (with the edge bits of the neighbouring strips), the rows above
 // and below arrive by lane shuffles, and a prefix sum over the lanes places
-// each row's cells in the list behind one atomic per wavefront.
-__global__ __launch_bounds__(256) void k_curv_gate_cut(const unsigned long long* __restrict__ cut, int strips,
-                                                       int nx, int ny, unsigned* __restrict__ count,
-                                                       unsigned* __restrict__ list) {
+// each row's cells in the list behind one atomic per workgroup of NW
+// wavefronts.
+constexpr int kGateCutWaves = 16;
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void k_curv_gate_cut(const unsigned long long* __restrict__ cut, int strips,
+                                                           int nx, int ny, unsigned* __restrict__ count,
+                                                           unsigned* __restrict__ list) {
   const int l = threadIdx.x & 63, b = (int)blockIdx.x;
-  const int j0 = 1 + ((int)blockIdx.y * 4 + (int)(threadIdx.x >> 6)) * 64;
-  if (j0 > ny) return;  // whole wavefronts
+  const int j0 = 1 + ((int)blockIdx.y * NW + (int)(threadIdx.x >> 6)) * 64;
+  const bool live = j0 <= ny;  // uniform: wavefronts past row ny list nothing
   const int j = j0 + l;
   // columns of the strip within one column of a cut cell of row r (rows
   // wrap periodically)
@@ -803,8 +825,11 @@ __global__ __launch_bounds__(256) void k_curv_gate_cut(const unsigned long long*
     const uint64_t lw = b > 0 ? w[-1] : 0ull, rwd = b + 1 < strips ? w[1] : 0ull;
     return c | (c << 1) | (c >> 1) | ((lw >> (kStrip - 1)) & 1ull) | ((rwd & 1ull) << (kStrip - 1));
   };
-  const uint64_t mine = near(j);
-  const uint64_t edge = l == 0 ? near(j0 - 1) : (l == 63 ? near(j0 + 64) : 0ull);
+  uint64_t mine = 0ull, edge = 0ull;
+  if (live) {
+    mine = near(j);
+    edge = l == 0 ? near(j0 - 1) : (l == 63 ? near(j0 + 64) : 0ull);
+  }
   // both shuffles with every lane active (a lane masked off during a lane
   // permute would hand its neighbour zero), then the edge rows selected
   const uint64_t dn_ = __shfl_up(mine, 1), up_ = __shfl_down(mine, 1);
@@ -823,10 +848,9 @@ __global__ __launch_bounds__(256) void k_curv_gate_cut(const unsigned long long*
     if (l >= d) incl += v;
   }
   const unsigned total = __shfl(incl, 63);
+  const unsigned base = reserve_slots<NW>(count, total);
   if (total == 0u) return;
-  unsigned base = 0u;
-  if (l == 63) base = atomicAdd(count, total);
-  unsigned pos = __shfl(base, 63) + incl - n;
+  unsigned pos = base + incl - n;
   const unsigned row = (unsigned)(j - 1) * (unsigned)nx + (unsigned)(i0 - 1);
   while (adm) {
     list[pos++] = row + (unsigned)(__ffsll((unsigned long long)adm) - 1);
@@ -958,11 +982,12 @@ int curv_mlp_launch(const double* vof, const unsigned long long* cut, float* kap
   hipLaunchKernelGGL(k_curv_reset, dim3(1), dim3(64), 0, st, list);
   if (cut)
     SBFS_TIMED("k_curv_gate_cut",
-               hipLaunchKernelGGL(k_curv_gate_cut, dim3(march_strips(nx), cdiv(ny, 4 * 64)), dim3(256), 0, st,
-                                  cut, march_strips(nx), nx, ny, list, list + 1));
+               hipLaunchKernelGGL(k_curv_gate_cut<kGateCutWaves>, dim3(march_strips(nx), cdiv(ny, kGateCutWaves * 64)),
+                                  dim3(64 * kGateCutWaves), 0, st, cut, march_strips(nx), nx, ny, list, list + 1));
   else
-    SBFS_TIMED("k_curv_gate", hipLaunchKernelGGL(k_curv_gate, dim3(cdiv(nx, kGateCols), cdiv(ny, 4 * kGateRows)),
-                                                 dim3(256), 0, st, vof, nx, ny, list, list + 1));
+    SBFS_TIMED("k_curv_gate",
+               hipLaunchKernelGGL(k_curv_gate<kGateWaves>, dim3(cdiv(nx, kGateCols), cdiv(ny, kGateWaves * kGateRows)),
+                                  dim3(64 * kGateWaves), 0, st, vof, nx, ny, list, list + 1));
   SBFS_TIMED("k_curv_mlp", hipLaunchKernelGGL(k_curv_mlp, dim3(4096), dim3(64), 0, st, weights(), vof, kappa,
                                               nx, ny, (float)(1. / h), (const unsigned*)list,
                                               (const unsigned*)(list + 1)));

@@ -1,5 +1,8 @@
 """Pins of the learned-curvature restatement (oracle/mlp.py), on the CPU.
 
+* GELU is the tanh form: torch's gelu(approximate="tanh") to round-off,
+  within 4.8e-4 of the erf form x Phi(x), gelu(-x) = gelu(x) - x, and its
+  derivative equals central finite differences.
 * The float64 backward of the weighted loss equals central finite
   differences.
 * BF16 rounding is round-to-nearest-even on known values.
@@ -19,6 +22,22 @@ import pytest
 from oracle import mlp
 
 
+def test_gelu_is_the_tanh_form():
+    import torch
+    from scipy.special import erf
+    x = np.linspace(-12.0, 12.0, 240001)
+    g = mlp._gelu(x)
+    t = torch.nn.functional.gelu(torch.from_numpy(x), approximate="tanh").numpy()
+    assert np.abs(g - t).max() <= 1e-14
+    exact = 0.5 * x * (1.0 + erf(x / np.sqrt(2.0)))
+    assert np.abs(g - exact).max() < 4.8e-4
+    assert np.abs(g - exact).max() > 4.5e-4  # the tanh form, not the erf form
+    assert np.allclose(mlp._gelu(-x), g - x, rtol=0, atol=1e-14)
+    h = 1e-5
+    fd = (mlp._gelu(x + h) - mlp._gelu(x - h)) / (2 * h)
+    assert np.allclose(mlp._dgelu(x), fd, rtol=1e-8, atol=1e-9)
+
+
 def test_float64_backward_matches_finite_differences():
     rng = np.random.default_rng(3)
     p = (0.3 * rng.standard_normal(mlp.NP)).astype(np.float64)
@@ -31,7 +50,7 @@ def test_float64_backward_matches_finite_differences():
     idx = np.concatenate([rng.choice(np.arange(o, e), 8, replace=False) for o, e in
                           ((mlp.OW1, mlp.OW1 + mlp.NF), (mlp.OB1, mlp.OW2), (mlp.OW2, mlp.OB2),
                            (mlp.OB2, mlp.OW3), (mlp.OW3, mlp.OB3))] + [np.array([mlp.OB3])])
-    eps = 1e-6
+    eps = 1e-5  # truncation ~1e-10 relative; at 1e-6 round-off of the loss (~28) reaches 1e-6
     for k in idx:
         pp, pm = p.copy(), p.copy()
         pp[k] += eps
