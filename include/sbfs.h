@@ -73,8 +73,11 @@ extern int sbfs_set_tdm(int kind, int chunk);
  * curvature, THINC and mass fluxes) and the momentum sources (viscous
  * stresses, advection, surface tension, buoyancy) in FP32; the vof update,
  * the density-weighted velocity update, the pressure gradient, the Poisson
- * right-hand side and solve, the projection and every field in memory stay
- * FP64, so the gas volume is conserved to FP64 round-off. */
+ * right-hand side and solve, the projection and every field of the state
+ * stay FP64, so the gas volume is conserved to FP64 round-off.  The
+ * momentum sources that one Runge-Kutta stage passes to the next
+ * (fluid->srcux, srcuy) are FP32 values and are stored as such, in the
+ * first half of each buffer; they carry no meaning after a step. */
 #define SBFS_PRECISION_FP64  0
 #define SBFS_PRECISION_MIXED 1
 extern int sbfs_set_precision(int precision);

@@ -48,6 +48,9 @@ struct MarchArgs {
   const double* uy;
   const double* p;
   const double* psi0;
+  // Runge-Kutta sources: beta (the previous stage's, read) and alpha (this
+  // stage's, written).  The FP32 bodies keep the momentum sources (ux, uy)
+  // as float in the buffers' first halves; the vof source is always FP64.
   const double* srcb_vof;
   const double* srcb_ux;
   const double* srcb_uy;
