@@ -28,10 +28,10 @@
 //   * k_curv_mlp      inference on those cells, 64 per wavefront; writes
 //                     kappa (NaN where the plain curvature stays).
 //
-// MFMA: v_mfma_f32_16x16x16bf16_1k.  Lane l holds A[l%16][4(l/16)+r],
-// B[4(l/16)+r][l%16] and D[4(l/16)+r][l%16] (r = 0..3); training uses
-// gfx950's v_mfma_f32_16x16x32_bf16, whose lane l holds eight consecutive
-// k, A[l%16][8(l/16)+j] and B[8(l/16)+j][l%16].  Activations are
+// MFMA: gfx950's v_mfma_f32_16x16x32_bf16, whose lane l holds eight k,
+// A[l%16][8(l/16)+j] and B[8(l/16)+j][l%16] (j = 0..7), and
+// D[4(l/16)+r][l%16] (r = 0..3); one weight-gradient product uses
+// v_mfma_f32_16x16x16bf16_1k (four k per lane).  Activations are
 // kept as (hidden x samples) matrices, the B operand, with the weights as the
 // A operand (out x in), so one layer's D output is the next layer's B operand
 // lane for lane.
@@ -124,6 +124,11 @@ __device__ inline float dgelu(float x) {
 }
 
 __device__ inline f4 mma(s4 a, s4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+// gfx950's v_mfma_f32_16x16x32_bf16: lane l holds A[l%16][8(l/16)+j] and
+// B[8(l/16)+j][l%16] (j = 0..7); D as for the 16x16x16 form
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
+__device__ inline f4 mma32(b8 a, b8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+__device__ inline b8 ld8(const unsigned short* p) { return *(const b8*)p; }
 __device__ inline s4 ld4(const unsigned short* p) { return *(const s4*)p; }
 __device__ inline s4 pack4(float a, float b, float c, float d) {
   s4 r;
@@ -275,44 +280,62 @@ struct Weights {
   const float* p;             // FP32 parameters (biases, w3, b3)
 };
 
+// Inference uses the 16x16x32 form: layer 1 is one product per block of 16
+// hidden units (K = the 32 padded features), layer 2 two (K = 64 hidden
+// units in two chunks).  Layer 2's k order within chunk q: lane group lg's
+// slots j < 4 take hidden unit 32q + 4 lg + j and slots j >= 4 hidden unit
+// 32q + 16 + 4 lg + (j - 4), which is where layer 1's D layout leaves them
+// (blocks 2q and 2q + 1), so the activations become the B operand without a
+// shuffle; the weights are loaded in the same order.
 struct AOps {
-  s4 w1[4][2];
-  s4 w2[4][4];
+  b8 w1[4];     // [block of 16 hidden units]
+  b8 w2[4][2];  // [block of 16 outputs][chunk of 32 hidden units]
 };
+
+struct B8Halves {
+  s4 lo, hi;
+};
+__device__ inline b8 cat8(s4 lo, s4 hi) { return __builtin_bit_cast(b8, B8Halves{lo, hi}); }
 
 __device__ inline void load_aops(const Weights& w, AOps& a) {
   const int l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
 #pragma unroll
   for (int mt = 0; mt < 4; mt++) {
+    a.w1[mt] = ld8(w.w1 + (16 * mt + lr) * NIP + 8 * lg);
 #pragma unroll
-    for (int ks = 0; ks < 2; ks++) a.w1[mt][ks] = ld4(w.w1 + (16 * mt + lr) * NIP + 16 * ks + 4 * lg);
-#pragma unroll
-    for (int ks = 0; ks < 4; ks++) a.w2[mt][ks] = ld4(w.w2 + (16 * mt + lr) * NH + 16 * ks + 4 * lg);
+    for (int q = 0; q < 2; q++) {
+      const unsigned short* r = w.w2 + (16 * mt + lr) * NH + 32 * q + 4 * lg;
+      a.w2[mt][q] = cat8(ld4(r), ld4(r + 16));
+    }
   }
 }
 
-// x0, x1: the B operands of the subtile (features 0-15, 16-31); h1, h2: the
-// pre-activations (D layout: hidden 16 mt + 4 (l/16) + r, sample l%16);
-// returns y of sample l%16 of the subtile (in every lane group)
-__device__ inline float forward16(const Weights& w, const AOps& a, s4 x0, s4 x1, f4 (&h1)[4], f4 (&h2)[4]) {
+// x: the B operand of the subtile (features 8 (l/16) .. +7 of sample l%16);
+// h1, h2: the pre-activations (D layout: hidden 16 mt + 4 (l/16) + r,
+// sample l%16); returns y of sample l%16 of the subtile (in every lane group)
+__device__ inline float forward16(const Weights& w, const AOps& a, b8 x, f4 (&h1)[4], f4 (&h2)[4]) {
   const int lg = (threadIdx.x & 63) >> 4;
 #pragma unroll
   for (int mt = 0; mt < 4; mt++) {
-    f4 c = {0.f, 0.f, 0.f, 0.f};
-    c = mma(a.w1[mt][0], x0, c);
-    c = mma(a.w1[mt][1], x1, c);
+    f4 c = mma32(a.w1[mt], x, f4{0.f, 0.f, 0.f, 0.f});
 #pragma unroll
     for (int r = 0; r < 4; r++) c[r] += w.p[OB1 + 16 * mt + 4 * lg + r];
     h1[mt] = c;
   }
-  s4 ab[4];
+  b8 ab[2];
 #pragma unroll
-  for (int ks = 0; ks < 4; ks++) ab[ks] = pack4(gelu(h1[ks][0]), gelu(h1[ks][1]), gelu(h1[ks][2]), gelu(h1[ks][3]));
+  for (int q = 0; q < 2; q++) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      ab[q][r] = (__bf16)gelu(h1[2 * q][r]);
+      ab[q][4 + r] = (__bf16)gelu(h1[2 * q + 1][r]);
+    }
+  }
 #pragma unroll
   for (int mt = 0; mt < 4; mt++) {
     f4 c = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int ks = 0; ks < 4; ks++) c = mma(a.w2[mt][ks], ab[ks], c);
+    for (int q = 0; q < 2; q++) c = mma32(a.w2[mt][q], ab[q], c);
 #pragma unroll
     for (int r = 0; r < 4; r++) c[r] += w.p[OB2 + 16 * mt + 4 * lg + r];
     h2[mt] = c;
@@ -327,11 +350,10 @@ __device__ inline float forward16(const Weights& w, const AOps& a, s4 x0, s4 x1,
   return s + w.p[OB3];
 }
 
-// B operands of subtile nt (16 samples) from [sample][NIP] bf16 in LDS
-__device__ inline void load_x(const unsigned short* X, int nt, s4& x0, s4& x1) {
+// B operand of subtile nt (16 samples) from [sample][NIP] bf16 in LDS
+__device__ inline b8 load_x(const unsigned short* X, int nt) {
   const int l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
-  x0 = ld4(X + (16 * nt + lr) * NIP + 4 * lg);
-  x1 = ld4(X + (16 * nt + lr) * NIP + 16 + 4 * lg);
+  return ld8(X + (16 * nt + lr) * NIP + 8 * lg);
 }
 
 // sum over the 16 lanes l%16 of a group: a butterfly through DPP lane
@@ -350,11 +372,6 @@ __device__ inline float sum16(float v) {
   return v;
 }
 
-// gfx950's v_mfma_f32_16x16x32_bf16: lane l holds A[l%16][8(l/16)+j] and
-// B[8(l/16)+j][l%16] (j = 0..7); D as for the 16x16x16 form
-typedef __bf16 b8 __attribute__((ext_vector_type(8)));
-__device__ inline f4 mma32(b8 a, b8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
-__device__ inline b8 ld8(const unsigned short* p) { return *(const b8*)p; }
 
 // ---------------------------------------------------------------------------
 // training: one workgroup = one tile of S = 16 ST samples and 4 ST
@@ -624,10 +641,8 @@ __global__ __launch_bounds__(64) void k_mlp_predict(Weights w, const unsigned sh
   __syncthreads();
 #pragma unroll 1
   for (int nt = 0; nt < 4; nt++) {
-    s4 x0, x1;
-    load_x(sx, nt, x0, x1);
     f4 h1[4], h2[4];
-    const float v = forward16(w, a, x0, x1, h1, h2);
+    const float v = forward16(w, a, load_x(sx, nt), h1, h2);
     const long o = (long)blockIdx.x * 64 + 16 * nt + lr;
     if (lg == 0 && o < n) y[o] = v;
   }
@@ -785,10 +800,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     float y = 0.f;
 #pragma unroll 1
     for (int nt = 0; nt < 4; nt++) {
-      s4 x0, x1;
-      load_x(sx, nt, x0, x1);
       f4 h1[4], h2[4];
-      const float p = forward16(w, a, x0, x1, h1, h2);
+      const float p = forward16(w, a, load_x(sx, nt), h1, h2);
       if (lg == nt) y = p;
     }
     const double kh = kp + (double)y;
