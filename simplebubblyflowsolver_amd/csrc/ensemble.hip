@@ -189,7 +189,7 @@ static int plan(sbfs_ensemble* e) {
   e->off_post = (unsigned)map.size();
   for (int c = 0; c < n; c++) {
     const auto& q = steps[0][c].post;
-    add(c, cdiv(q.nx + 2, 64) * cdiv(q.ny, 4));
+    add(c, cdiv(q.nx + 2, 64) * cdiv(q.ny, 4 * kPostRows));
   }
   e->nb_post = (unsigned)map.size() - e->off_post;
   e->lds = lds_rows;

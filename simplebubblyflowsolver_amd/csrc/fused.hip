@@ -47,58 +47,149 @@ namespace sbfs {
 // ---------------------------------------------------------------------------
 // projection + pressure (correct_velocity/ux.c, uy.c; update_pressure.c)
 // ---------------------------------------------------------------------------
-// cell (i, j), i = 0..nx+1, j = 1..ny
-__device__ __forceinline__ void post_body(const PostArgs& a, int i, int j) {
+// Column i = 0..nx+1, rows j0 .. j0 + kPostRows - 1 (up to ny) of one
+// thread: the quotients that depend on the column or on nothing are formed
+// once per thread, and the potentials and the density of row j - 1 come from
+// the previous row.  Every cell's expressions are those of the reference
+// (left to right, the same roundings), so the rows a thread takes do not
+// change a bit.  O: the type of the element offsets (32 bits while every
+// field stays below 4 GiB, so the loads take a scalar base and a 32-bit
+// offset).
+
+template <typename O>
+__device__ __forceinline__ double post_ld(const double* f, O e) {
+  using G = __attribute__((address_space(1))) const char;
+  return *(const double*)((G*)f + e * (O)sizeof(double));
+}
+template <typename O>
+__device__ __forceinline__ void post_st(double* f, O e, double v) {
+  using G = __attribute__((address_space(1))) char;
+  *(double*)((G*)f + e * (O)sizeof(double)) = v;
+}
+
+// rows j0 .. j0 + N - 1 of column i; with All every load is issued before
+// the first row's arithmetic (the full tiles), otherwise row by row
+template <typename O, int N, bool All>
+__device__ __forceinline__ void post_column(const PostArgs& a, int i, int j0, int nj) {
   const int nx = a.nx, ny = a.ny;
-  if (i > nx + 1 || j > ny) return;
   const double dt = *a.dt;
   const double dtn = a.g_new * dt;
   const double dto = a.g_old * dt;
   const double coef = -1. / dtn * dto;
-  auto icc = [nx](int p, int q) { return L::cc(p, q, nx); };
-  auto ixf = [nx](int p, int q) { return L::xf(p, q, nx); };
-  auto DEN1 = [&](int p, int q) { return mix_clamp(a.denr, a.dlo, a.dhi, a.vofn[L::vof(p, q, nx)]); };
-  if (i >= 1) {
-    double u = 0.;
-    if (i != 1 && i != nx + 1) {
-      const double h = a.hxxf[i - 1];
-      u = a.uxs[ixf(i, j)];
-      u -= dtn / a.refden / h * (-a.psi1[icc(i - 1, j)] + a.psi1[icc(i, j)]);
-      const double d = 0.5 * DEN1(i - 1, j) + 0.5 * DEN1(i, j);
-      u += dtn * coef * (1. / d - 1. / a.refden) / h * (-a.psi0[icc(i - 1, j)] + a.psi0[icc(i, j)]);
+  // dtn / a.refden / h * dpsi is ((dtn / refden) / h) * dpsi, and
+  // dtn * coef * (1 / d - 1 / refden) / h * dpsi is
+  // (((dtn coef) (1 / d - 1 / refden)) / h) * dpsi
+  const double dtr = dtn / a.refden, cdc = dtn * coef, rr = 1. / a.refden;
+  const bool fx = i >= 2 && i <= nx;  // an interior x face
+  const bool cin = i >= 1 && i <= nx;  // an interior cell column
+  const double h = fx ? a.hxxf[i - 1] : 1.;
+  const double qx = dtr / h, qy = dtr / a.hy;
+  const O wc = (O)nx + 2, wf = (O)nx + 1;
+  // element offsets of (i, j0 - 1): cc and xf layouts (L::cc, L::xf); the
+  // vof of cell (i, j) is at its cc offset + wc (L::vof)
+  const O oc0 = (O)(j0 - 1) * wc + (O)i;
+  const O ox0 = (O)(j0 - 1) * wf + (O)(i - 1);
+  // potentials and vof of rows j0 - 1 .. j0 + N - 1 in this column (k = row
+  // - j0 + 1), and the inputs of rows j0 .. j0 + N - 1
+  double p1[N + 1] = {}, p0[N + 1] = {}, vf[N + 1] = {};
+  double p1l[N] = {}, p0l[N] = {}, vfl[N] = {}, us[N] = {}, vs[N] = {}, pp[N] = {};
+  auto load_row = [&](int k) {
+    const O oc = oc0 + (O)k * wc;
+    if (cin) {
+      p1[k] = post_ld(a.psi1, oc);
+      p0[k] = post_ld(a.psi0, oc);
+      vf[k] = post_ld(a.vofn, oc + wc);
     }
-    store_with_halo(a.ux, i, j, ny, 1, u, ixf);
-  }
-  {
-    double u;
-    if (i == 0) u = kUyXm;
-    else if (i == nx + 1) u = kUyXp;
-    else {
-      u = a.uys[icc(i, j)];
-      u -= dtn / a.refden / a.hy * (-a.psi1[icc(i, j - 1)] + a.psi1[icc(i, j)]);
-      const double d = 0.5 * DEN1(i, j - 1) + 0.5 * DEN1(i, j);
-      u += dtn * coef * (1. / d - 1. / a.refden) / a.hy * (-a.psi0[icc(i, j - 1)] + a.psi0[icc(i, j)]);
+    if (k == 0) return;
+    if (fx) {
+      p1l[k - 1] = post_ld(a.psi1, oc - 1);
+      p0l[k - 1] = post_ld(a.psi0, oc - 1);
+      vfl[k - 1] = post_ld(a.vofn, oc + wc - 1);
+      us[k - 1] = post_ld(a.uxs, ox0 + (O)k * wf);
     }
-    store_with_halo(a.uy, i, j, ny, 1, u, icc);
+    if (cin) {
+      vs[k - 1] = post_ld(a.uys, oc);
+      pp[k - 1] = post_ld(a.p, oc);
+    }
+  };
+  if (All) {
+#pragma unroll
+    for (int k = 0; k <= N; k++) load_row(k);
+  } else {
+    load_row(0);
   }
-  if (i >= 1 && i <= nx) {
-    const double v = a.p[icc(i, j)] + a.psi1[icc(i, j)];
-    store_with_halo(a.p, i, j, ny, 1, v, icc);
-    if (i == 1) store_with_halo(a.p, 0, j, ny, 1, v, icc);
-    if (i == nx) store_with_halo(a.p, nx + 1, j, ny, 1, v, icc);
+  double du = cin ? mix_clamp(a.denr, a.dlo, a.dhi, vf[0]) : 0.;
+#pragma unroll
+  for (int r = 0; r < N; r++) {
+    if (!All) {
+      if (r >= nj) break;
+      load_row(r + 1);
+    }
+    const int j = j0 + r;
+    const O oc = oc0 + (O)(r + 1) * wc, ox = ox0 + (O)(r + 1) * wf;
+    const double d = cin ? mix_clamp(a.denr, a.dlo, a.dhi, vf[r + 1]) : 0.;
+    if (i >= 1) {
+      double u = 0.;
+      if (fx) {
+        u = us[r];
+        u -= qx * (-p1l[r] + p1[r + 1]);
+        const double dd = 0.5 * mix_clamp(a.denr, a.dlo, a.dhi, vfl[r]) + 0.5 * d;
+        u += cdc * (1. / dd - rr) / h * (-p0l[r] + p0[r + 1]);
+      }
+      post_st(a.ux, ox, u);
+      if (j > ny - 1) post_st(a.ux, ox - (O)ny * wf, u);
+      if (j <= 1) post_st(a.ux, ox + (O)ny * wf, u);
+    }
+    {
+      double u;
+      if (i == 0) u = kUyXm;
+      else if (i == nx + 1) u = kUyXp;
+      else {
+        u = vs[r];
+        u -= qy * (-p1[r] + p1[r + 1]);
+        const double dd = 0.5 * du + 0.5 * d;
+        u += cdc * (1. / dd - rr) / a.hy * (-p0[r] + p0[r + 1]);
+      }
+      post_st(a.uy, oc, u);
+      if (j > ny - 1) post_st(a.uy, oc - (O)ny * wc, u);
+      if (j <= 1) post_st(a.uy, oc + (O)ny * wc, u);
+    }
+    if (cin) {
+      const double v = pp[r] + p1[r + 1];
+      auto put = [&](O o) {
+        post_st(a.p, o, v);
+        if (j > ny - 1) post_st(a.p, o - (O)ny * wc, v);
+        if (j <= 1) post_st(a.p, o + (O)ny * wc, v);
+      };
+      put(oc);
+      if (i == 1) put(oc - 1);
+      if (i == nx) put(oc + 1);
+    }
+    du = d;
   }
 }
 
+template <typename O>
+__device__ __forceinline__ void post_rows(const PostArgs& a, int i, int j0) {
+  if (i > a.nx + 1 || j0 > a.ny) return;
+  const int nj = min(kPostRows, a.ny - j0 + 1);
+  if (nj == kPostRows) post_column<O, kPostRows, true>(a, i, j0, nj);
+  else post_column<O, kPostRows, false>(a, i, j0, nj);
+}
+
+// tiles of 64 columns x 4 kPostRows rows, a thread per column and kPostRows rows
+template <typename O>
 __global__ __launch_bounds__(256) void k_post(PostArgs a) {
-  post_body(a, (int)(blockIdx.x * blockDim.x + threadIdx.x), 1 + (int)(blockIdx.y * blockDim.y + threadIdx.y));
+  post_rows<O>(a, (int)(blockIdx.x * 64 + threadIdx.x), 1 + (int)(blockIdx.y * 4 + threadIdx.y) * kPostRows);
 }
 
-// one workgroup of 64 x 4 cells per map entry (case, local tile)
+// one tile per map entry (case, local tile); ensemble fields stay below 4 GiB
 __global__ __launch_bounds__(256) void k_post_batch(const PostArgs* __restrict__ tab, const uint2* __restrict__ map) {
   const uint2 m = map[blockIdx.x];
   const PostArgs& a = tab[m.x];
   const int gx = (a.nx + 2 + 63) / 64;
-  post_body(a, (int)((m.y % gx) * 64 + threadIdx.x), 1 + (int)((m.y / gx) * 4 + threadIdx.y));
+  post_rows<unsigned>(a, (int)((m.y % gx) * 64 + threadIdx.x),
+                      1 + (int)((m.y / gx) * 4 + threadIdx.y) * kPostRows);
 }
 
 // ---------------------------------------------------------------------------
@@ -584,8 +675,11 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
     }
     PostArgs q;
     fill_post(c, fluid, interface, us[k], vs[k], dtp, k, q);
-    const dim3 pg(cdiv(nx + 2, 64), cdiv(ny, 4)), pb(64, 4);
-    SBFS_TIMED("k_post", hipLaunchKernelGGL(k_post, pg, pb, 0, stream(), q));
+    const dim3 pg(cdiv(nx + 2, 64), cdiv(ny, 4 * kPostRows)), pb(64, 4);
+    if (fp64_march_addressable(nx, ny))
+      SBFS_TIMED("k_post", hipLaunchKernelGGL(k_post<unsigned>, pg, pb, 0, stream(), q));
+    else
+      SBFS_TIMED("k_post", hipLaunchKernelGGL(k_post<size_t>, pg, pb, 0, stream(), q));
     if (check_launch("integrate: k_post")) return 1;
     swap_data(fluid->psi[0], fluid->psi[1]);
   }

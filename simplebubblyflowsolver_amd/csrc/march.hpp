@@ -110,7 +110,12 @@ __host__ __device__ inline int march_max_weight(int band) { return band + kWarmC
 // weights give a permutation of 0..n-1.
 __global__ void k_march_order(const unsigned* weight, unsigned* order, int n, int max_weight);
 
-// projection of u and the pressure update after the last stage (k_post)
+// projection of u and the pressure update after the last stage (k_post):
+// a thread takes kPostRows consecutive rows of one column, a workgroup
+// 64 columns and 4 kPostRows rows (4096 x 8192, us per launch: 1 row 496,
+// 2 rows 487-498, 4 rows 506-510, 8 rows 523-529; one cell per thread
+// without the shared quotients 558)
+constexpr int kPostRows = 2;
 struct PostArgs {
   const double* uxs;
   const double* uys;
