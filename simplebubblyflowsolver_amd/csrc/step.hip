@@ -34,14 +34,22 @@ __device__ __forceinline__ void dt_partial_body(const double* __restrict__ ux, c
   const int j0 = 1 + by * rows;
   if (i <= nx) {
     const double hx = hxxf[i - 1];
-    // rows past ny repeat row ny (a repeated value leaves a min unchanged),
+    // The column's minimum of h / (|u| + small) is h over its largest
+    // |u| + small, bit for bit: a correctly rounded quotient does not grow
+    // with its divisor, so the largest divisor gives the smallest quotient
+    // (and fmax passes over a NaN as fmin does).  One division per field
+    // and thread instead of one per cell.
+    double mx = 0., my = 0.;
+    // rows past ny repeat row ny (a repeated value leaves a max unchanged),
     // so the loads carry no branch and are all in flight together
 #pragma unroll 16
     for (int r = 0; r < rows; r++) {
       const int j = min(j0 + r, ny);
-      if (i >= 2) m = fmin(m, hx / (fabs(ux[L::xf(i, j, nx)]) + small));
-      m = fmin(m, hy / (fabs(uy[L::cc(i, j, nx)]) + small));
+      if (i >= 2) mx = fmax(mx, fabs(ux[L::xf(i, j, nx)]) + small);
+      my = fmax(my, fabs(uy[L::cc(i, j, nx)]) + small);
     }
+    if (i >= 2) m = fmin(m, hx / mx);
+    m = fmin(m, hy / my);
   }
   m = block_reduce<0>(m, sh);
   if (threadIdx.x == 0) partial[by * gx + bx] = m;
