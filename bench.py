@@ -69,10 +69,11 @@ BYTES_PER_CELL = {
     "k_dct3_psi": 16, "k_correct_ux": 40, "k_correct_uy": 40,
     "k_update_p": 24, "k_dt_partial": 16, "k_dt_final": 0,
     # fused path
-    # (the mixed kernels keep the momentum sources as FP32: 8 bytes fewer per
-    # cell in stage 0 and the mean of stages 1 and 2, 16 and 8 fewer, below)
+    # (stages 1 and 2: 9 reads + 7 writes each; the mixed kernels keep the
+    # momentum sources as FP32: 8 bytes fewer per cell in stage 0, and 16 and
+    # 8 fewer in stages 1 and 2)
     "k_march": 12 * 8, "k_march_mixed": 12 * 8 - 8, "k_dct2_rows": 16,
-    "k_post": 72, "k_march_corr": (17 + 16) * 8 / 2, "k_march_mixed_corr": (17 + 16) * 8 / 2 - 12,
+    "k_post": 72, "k_march_corr": 16 * 8, "k_march_mixed_corr": 16 * 8 - 12,
     # learned curvature: the stage-0 gate reads the vof field
     "k_curv_gate": 8,
 }

@@ -430,7 +430,8 @@ static int march_band(int nx, int ny, bool mixed) {
 
 // Arguments of stage k's stage kernel from the state's current pointers:
 // with the fold (corr) it reads the previous stage's predicted velocity
-// us/vs[k-1], writes us/vs[k], and writes the updated pressure to p_out.
+// us/vs[k-1] and writes us/vs[k]; stage 2 writes the pressure after stage
+// 1's update to p_out (stage 1 writes none, see MarchArgs::p_out).
 static void fill_march(const domain_t* domain, const DomainCtx* c, const fluid_t* fluid,
                        const interface_t* interface, const Coef& cf, double* rhs, int band, int k, bool corr,
                        double* const us[3], double* const vs[3], double* vofn, double* p_out,
@@ -442,7 +443,10 @@ static void fill_march(const domain_t* domain, const DomainCtx* c, const fluid_t
   m.p = D(fluid->p);
   m.psi0 = D(fluid->psi[0]);
   m.psi_old = corr ? D(fluid->psi[1]) : nullptr;
-  m.p_out = corr ? p_out : nullptr;
+  // stage 1's updated pressure is formed again by stage 2 (p_twice), so
+  // only stage 2 stores one
+  m.p_out = corr && k == 2 ? p_out : nullptr;
+  m.p_twice = corr && k == 2;
   m.pg_old = rk((k + 1) % 3).g;  // gammas of stage k-1's dt_old / dt_new
   m.pg_new = rk((k + 2) % 3).g;
   m.kappa = nullptr;
@@ -651,8 +655,8 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
       else
         SBFS_TIMED("k_march", hipLaunchKernelGGL(k_march, grid, block, 0, stream(), ma));
       if (check_launch("integrate: k_march")) return 1;
-      if (corr) {
-        // the stage kernel wrote the updated pressure of stage k-1
+      if (corr && k == 2) {
+        // the stage kernel wrote the pressure after stage 1's update
         double* t = (double*)fluid->p.data;
         fluid->p.data = g_scr.p;
         g_scr.p = t;
@@ -750,7 +754,9 @@ static StepKey step_key(const DomainCtx* c, fluid_t* f, interface_t* s, const Co
 struct StepGraph {
   StepKey key;
   hipGraphExec_t exec = nullptr;
-  void* after[kStateArrays + 1] = {};  // state pointers after the step, + scratch vof
+  // state pointers after the step, then the scratch vof and pressure, which
+  // a step swaps with the state's (the pressure once per step)
+  void* after[kStateArrays + 2] = {};
   uint64_t last_use = 0;
 };
 static StepGraph g_graph[2];
@@ -776,6 +782,7 @@ static int step_graph(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
       SBFS_HIP(hipGraphLaunch(g.exec, stream()));
       for (int n = 0; n < kStateArrays; n++) state_arrays(fluid, interface, n)->data = g.after[n];
       g_scr.vof = (double*)g.after[kStateArrays];
+      g_scr.p = (double*)g.after[kStateArrays + 1];
       g.last_use = g_clock;
       return 0;
     }
@@ -806,6 +813,7 @@ static int step_graph(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
   g.key = key;
   for (int n = 0; n < kStateArrays; n++) g.after[n] = state_arrays(fluid, interface, n)->data;
   g.after[kStateArrays] = g_scr.vof;
+  g.after[kStateArrays + 1] = g_scr.p;
   g.last_use = g_clock;
   SBFS_HIP(hipGraphLaunch(g.exec, stream()));
   return 0;
@@ -876,7 +884,7 @@ int ensemble_plan_step(const domain_t* domain, DomainCtx* c, fluid_t* fluid, int
       swap_data(fluid->srcuy[0], fluid->srcuy[1]);
     }
     fill_march(domain, c, fluid, interface, cf, rhs, band, k, corr, us, vs, scr.vof, scr.p, dtp, out.march[k]);
-    if (corr) {
+    if (corr && k == 2) {
       double* t = (double*)fluid->p.data;
       fluid->p.data = scr.p;
       scr.p = t;

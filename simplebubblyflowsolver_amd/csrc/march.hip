@@ -704,7 +704,7 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
           put(a.txy, 1, rx, double(txy_prev));
           if (c == nx) put_at(a.txy, 1, rx, 1, txy0r);
         }
-        if (CORR) {
+        if (CORR && a.p_out) {
           // the previous stage's updated pressure, with its Neumann ghosts
           put(a.p_out, 1, rc, p0);
           if (c == 1) put_at(a.p_out, 1, rc, -1, p0);
@@ -779,7 +779,9 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
         }
         nuy = u;
       }
-      if (c_in) np = np + psp1;
+      // (stage 2 of a folded step: the input pressure plus stage 0's
+      // potential, which stage 1 formed and did not store, then stage 1's)
+      if (c_in) np = (a.p_twice ? np + pop1 : np) + psp1;
     }
   }
   if (a.weight && lane == 0) a.weight[by * march_strips(nx) + bx] = weight;

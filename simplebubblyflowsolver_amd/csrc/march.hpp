@@ -76,8 +76,13 @@ struct MarchArgs {
   // before the update, psi0 its new potential and psi_old the potential it
   // started from; the updated pressure of the rows this wavefront owns is
   // written to p_out
+  // p_out is null in stage 1 of a folded step, whose updated pressure is not
+  // stored: stage 2 reads the step's input pressure and, with p_twice set,
+  // forms it again as p + psi_old (the same operands in the same order)
+  // before adding psi0
   const double* psi_old;
   double* p_out;
+  int p_twice;
   double pg_old, pg_new;  // Runge-Kutta gamma of the previous stage's dt_old / dt_new
   // learned curvature of the stage's input vof (interior cells, (j-1) Nx +
   // (i-1)); NaN where the plain curvature stays; null: plain everywhere.
