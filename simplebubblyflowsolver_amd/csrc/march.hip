@@ -367,15 +367,23 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
   // beta sources of row s (stages 1, 2): the vof's for every marched row,
   // uy's from the first output row on (the predicted uy of row je+1 enters
   // the Poisson rhs of row je), ux's on output rows
-  auto ld_sb = [&](int s_, int jw, double& bv, double& by, double& bx) {
-    bv = by = bx = 0.;
+  // (the FP32 stage-0 body keeps the prefetched momentum sources as float
+  // until they are used, after the fence: widened at the load, each waits
+  // for its load and with it for every load issued before it, which cost
+  // k_march_mixed 928 against 860 us at 4096 x 8192; the folded body
+  // measured faster with the early widening, 1158 against 1189-1205 us)
+  using MS = std::conditional_t<std::is_same<R, float>::value && !CORR, float, double>;
+  auto ld_sb = [&](int s_, int jw, double& bv, MS& by, MS& bx) {
+    bv = 0.;
+    by = bx = MS(0);
     if (has_b && c_in) {
       bv = AT(ci_row(a.srcb_vof, jw));
       if (s_ >= jb) by = msrc(a.srcb_uy, ci_row, jw);
       if (s_ >= jb && s_ <= je && c >= 2) bx = msrc(a.srcb_ux, fx_row, jw);
     }
   };
-  double nsbv, nsby, nsbx;
+  double nsbv;
+  MS nsby, nsbx;
   ld_sb(s_first, wrap_r(s_first, ny), nsbv, nsby, nsbx);
   // the fence below, for the first step's inputs: without it the loop entry
   // would carry outstanding loads into every step's first reads
@@ -407,7 +415,8 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
     const bool outrow = s >= jb && s <= je;
     if (a.weight) weight += __ballot(c_vof && in_band(v0)) != 0ull ? 1u : 0u;
     const float kap = nkap;
-    const double sbv = nsbv, sby = nsby, sbx = nsbx;
+    const double sbv = nsbv;
+    const MS sby = nsby, sbx = nsbx;
     nv = ld_vof(w3);
     nux = ld_ux(w2);
     nuy = ld_cc(a.uy, w2);
@@ -637,7 +646,7 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
           s_ += grav * rdfx;
           const double sd = double(s_);
           if (a.srca_ux && outrow && c_out) msrc(a.srca_ux, fx_row, s) = s_;
-          uxs = dfx * ux0 + dtca * sd + dtcb * sbx + dtcg * g;
+          uxs = dfx * ux0 + dtca * sd + dtcb * double(sbx) + dtcg * g;
           uxs /= d1x;
         }
         // -- predicted uy, row s (predict/uy.c, force.c) -----------------------
@@ -656,7 +665,7 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
           }
           const double sd = double(s_);
           if (a.srca_uy && outrow && c_out) msrc(a.srca_uy, ci_row, s) = s_;
-          uys = dfy * uy0 + dtca * sd + dtcb * sby + dtcg * g;
+          uys = dfy * uy0 + dtca * sd + dtcb * double(sby) + dtcg * g;
           uys /= d1y;
         }
       }
