@@ -20,7 +20,8 @@
  * In this library `data` is a DEVICE pointer (HBM of the current GPU).  Use
  * sbfs_array_download / sbfs_array_upload (sbfs.h) to move it.  Like the
  * reference, the solver swaps `data` pointers between arrays (the scalar
- * potentials and the Runge-Kutta sources), so callers must not cache `data`
+ * potentials and the Runge-Kutta sources; on the fused path also the vof and
+ * the pressure with scratch buffers), so callers must not cache `data`
  * across calls.
  */
 typedef struct {
