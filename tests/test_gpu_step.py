@@ -1,6 +1,10 @@
 """Whole time steps on the GPU against the CPU oracle, and the step's
 invariants at full size."""
 import ctypes as C
+import os
+import subprocess
+import sys
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -108,6 +112,46 @@ def test_fused_path_matches_staged_bitwise(oracle_mod, product, nx, ny):
         a.close()
         b.close()
         set_path(product, 1)
+
+
+UNFOLDED = """
+import sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+sys.path.insert(0, sys.argv[1] + "/tests")
+from harness import copy_state, initial_state
+from simplebubblyflowsolver_amd import solver
+be = solver.product_backend()
+be.set_dt_coefs(0.2, 0.5, 0.95)
+a = solver.Case(be, 64, 128, 1.0, 2.0, 35.0, 1.0, 1e-3, 1e-2, 10.0)
+b = solver.Case(be, 64, 128, 1.0, 2.0, 35.0, 1.0, 1e-3, 1e-2, 10.0)
+initial_state(a, 7, velocity=0.05)
+copy_state(a, b)
+for _ in range(8):
+    assert be.lib.sbfs_set_path(0) == 0
+    dta = a.integrate()
+    assert be.lib.sbfs_set_path(1) == 0
+    dtb = b.integrate()
+    assert dta == dtb
+for n in ("vof", "ux", "uy", "p", "psi0", "den1", "txy"):
+    x, y = a.get(n), b.get(n)
+    assert np.array_equal(x, y), (n, np.abs(x - y).max())
+print("unfolded fused path: 8 steps bitwise equal to the staged path", flush=True)
+"""
+
+
+def test_unfolded_fused_path_matches_staged_bitwise():
+    """With SBFS_MARCH_FOLD=0 the fused step runs k_post after every stage
+    instead of folding each projection into the next stage kernel, and no
+    stage kernel writes a pressure: 8 steps bitwise equal to the staged
+    path.  The switch is read once per process, so the comparison runs in a
+    child process."""
+    env = dict(os.environ, SBFS_MARCH_FOLD="0")
+    root = str(Path(__file__).resolve().parents[1])
+    r = subprocess.run([sys.executable, "-c", UNFOLDED, root], env=env, capture_output=True, text=True,
+                       timeout=240)
+    print(r.stdout.strip())
+    assert r.returncode == 0, r.stdout + r.stderr
 
 
 @pytest.mark.parametrize("nx,ny,bubbles,steps,bands", [
