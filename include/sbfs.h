@@ -96,7 +96,8 @@ extern int sbfs_set_precision(int precision);
 extern int sbfs_set_curvature(int mode);
 
 /*
- * The curvature network: a 26 -> 64 -> 64 -> 1 MLP with GELU, BF16 weights
+ * The curvature network: a 26 -> 64 -> 64 -> 1 MLP with GELU (the tanh
+ * form, x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))) / 2), BF16 weights
  * and FP32 accumulation, a learned correction of the plain curvature: its
  * inputs are a 5 x 5 vof stencil (the differences to the centre value and
  * the centre value minus 1/2) and the plain curvature kp of the centre cell
