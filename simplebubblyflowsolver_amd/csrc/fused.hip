@@ -204,7 +204,8 @@ struct FusedScratch {
   double* uy2 = nullptr;
   double* p = nullptr;    // pressure written by a stage kernel with the folded projection
   float* kappa = nullptr;  // learned curvature of a stage's input vof (interior cells)
-  unsigned* klist = nullptr;  // its listed cells (count first)
+  unsigned* klist[3] = {};  // its listed cells (count first), one list per stage
+  int klast = -1;            // the list of the last launch (-1: none since the fill)
   unsigned long long* cut = nullptr;  // cut-cell bitmask of a stage kernel's new vof
   // the stage kernels' dispatch order and unit weights (march.hpp)
   unsigned* weight = nullptr;
@@ -219,7 +220,7 @@ static int scratch(const domain_t* d, const interface_t* s, const fluid_t* f) {
   graph_epoch_bump();
   for (double* q : {g_scr.vof, g_scr.ux, g_scr.uy, g_scr.ux2, g_scr.uy2, g_scr.p}) (void)hipFree(q);
   (void)hipFree(g_scr.kappa);
-  (void)hipFree(g_scr.klist);
+  for (unsigned* l : g_scr.klist) (void)hipFree(l);
   (void)hipFree(g_scr.cut);
   (void)hipFree(g_scr.weight);
   (void)hipFree(g_scr.order);
@@ -233,8 +234,9 @@ static int scratch(const domain_t* d, const interface_t* s, const fluid_t* f) {
   }
   const size_t cells = d->mysizes[0] * d->mysizes[1];
   SBFS_HIP(hipMalloc(&g_scr.kappa, sizeof(float) * cells));
-  SBFS_HIP(hipMalloc(&g_scr.klist, sizeof(unsigned) * (cells + 1)));
-  if (curv_field_init(g_scr.kappa, g_scr.klist, (long)cells, stream())) return 1;
+  for (unsigned*& l : g_scr.klist) SBFS_HIP(hipMalloc(&l, sizeof(unsigned) * (cells + 1)));
+  if (curv_field_init(g_scr.kappa, g_scr.klist, 3, (long)cells, stream())) return 1;
+  g_scr.klast = -1;
   SBFS_HIP(hipMalloc(&g_scr.cut, sizeof(unsigned long long) * d->mysizes[1] * march_strips((int)d->mysizes[0])));
   g_scr.nx = d->mysizes[0];
   g_scr.ny = d->mysizes[1];
@@ -589,10 +591,25 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
   // bitmask of the vof it wrote; stage 0 from the state's vof, which the
   // caller may have changed since the last step)
   auto cut_of = [](int k) -> const unsigned long long* { return k > 0 ? g_scr.cut : nullptr; };
+  // stage k lists into list k; the launch before it wrote list klast, and
+  // the launch after it writes list (k + 1) % 3
+  auto curv_launch = [&](int k, const double* vof, hipStream_t st) -> int {
+    if (g_scr.klast == k) {
+      // only after a step that stopped part way: start the field afresh
+      if (curv_field_init(g_scr.kappa, g_scr.klist, 3, (long)nx * ny, st)) return 1;
+      g_scr.klast = -1;
+    }
+    const unsigned* prev = g_scr.klast >= 0 ? g_scr.klist[g_scr.klast] : nullptr;
+    if (curv_mlp_launch(vof, cut_of(k), g_scr.kappa, g_scr.klist[k], prev, g_scr.klist[(k + 1) % 3], nx, ny,
+                        learned_h, st))
+      return 1;
+    g_scr.klast = k;
+    return 0;
+  };
   auto fork_curv = [&](int k, const double* vof) -> int {
     SBFS_HIP(hipEventRecord(fork_event(k), stream()));
     SBFS_HIP(hipStreamWaitEvent(side_stream(), fork_event(k), 0));
-    if (curv_mlp_launch(vof, cut_of(k), g_scr.kappa, g_scr.klist, nx, ny, learned_h, side_stream())) return 1;
+    if (curv_launch(k, vof, side_stream())) return 1;
     SBFS_HIP(hipEventRecord(join_event(k), side_stream()));
     return 0;
   };
@@ -631,12 +648,11 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
       if (learned) {
         if (overlap) {
           SBFS_HIP(hipStreamWaitEvent(stream(), join_event(k), 0));
-        } else if (curv_mlp_launch(D(interface->vof), cut_of(k), g_scr.kappa, g_scr.klist, nx, ny, learned_h,
-                                   stream())) {
+        } else if (curv_launch(k, D(interface->vof), stream())) {
           return 1;
         }
         m.kappa = g_scr.kappa;
-        m.kappa_n = g_scr.klist;
+        m.kappa_n = g_scr.klist[k];
         if (k < 2) m.cut = g_scr.cut;
       }
       if (ordered) {
@@ -743,7 +759,8 @@ static StepKey step_key(const DomainCtx* c, fluid_t* f, interface_t* s, const Co
                 ((uint64_t)options().chunk << 16) | ((uint64_t)options().precision << 48) |
                 ((uint64_t)options().curvature << 56));
   key_add(k, g_scr.kappa);
-  key_add(k, g_scr.klist);
+  for (const unsigned* l : g_scr.klist) key_add(k, l);
+  k.v.push_back((uint64_t)(int64_t)g_scr.klast);
   key_add(k, g_scr.cut);
   k.v.push_back(march_ordered(options().precision == SBFS_PRECISION_MIXED) ? 1 : 0);
   key_add(k, g_scr.order);

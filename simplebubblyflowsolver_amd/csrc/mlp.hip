@@ -659,22 +659,33 @@ __global__ __launch_bounds__(64) void k_mlp_predict(Weights w, const unsigned sh
 // run to run; each cell's value does not depend on it.  Layout of kappa
 // and of the list entries: interior cells, (j-1) Nx + (i-1).
 // ---------------------------------------------------------------------------
-// The kappa field and its list of cells (list[0] the count, then the cell
-// indices) are kept together: every cell outside the list holds NaN.  A
-// new field is filled with NaN and its list emptied once; each launch then
-// puts the listed cells of the previous launch back to NaN before it lists
-// and fills the new ones, so the field is never written whole again.
-__global__ void k_curv_fill(float* __restrict__ kappa, long n, unsigned* __restrict__ list) {
+// The kappa field and its lists of cells (list[0] the count, then the cell
+// indices) are kept together: every cell outside the last launch's list
+// holds NaN.  A new field is filled with NaN and its lists emptied once.
+// The launches of a step cycle through three lists (stage k writes list
+// k); a launch's gate, besides listing its own cells, puts the cells of the
+// previous launch's list back to NaN and empties the list the next launch
+// will write, which was last read two launches before, so the field is
+// never written whole again and no launch of its own does either.
+__global__ void k_curv_fill(float* __restrict__ kappa, long n, unsigned* l0, unsigned* l1, unsigned* l2) {
   const long stride = (long)gridDim.x * blockDim.x;
   for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) kappa[c] = __int_as_float(0x7FC00000);
-  if (blockIdx.x == 0 && threadIdx.x == 0) list[0] = 0u;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    l0[0] = 0u;
+    if (l1) l1[0] = 0u;
+    if (l2) l2[0] = 0u;
+  }
 }
-__global__ void k_curv_restore(float* __restrict__ kappa, const unsigned* __restrict__ list) {
-  const unsigned n = list[0], stride = gridDim.x * blockDim.x;
-  for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) kappa[list[1 + q]] = __int_as_float(0x7FC00000);
-}
-__global__ void k_curv_reset(unsigned* __restrict__ count) {
-  if (threadIdx.x == 0) *count = 0u;
+// the gates' share of the turnover above (prev, next: null for none)
+__device__ inline void curv_turnover(float* __restrict__ kappa, const unsigned* __restrict__ prev,
+                                     unsigned* __restrict__ next) {
+  const unsigned nt = gridDim.x * gridDim.y * blockDim.x;
+  const unsigned t = (blockIdx.y * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x;
+  if (prev) {
+    const unsigned n = prev[0];
+    for (unsigned q = t; q < n; q += nt) kappa[prev[1 + q]] = __int_as_float(0x7FC00000);
+  }
+  if (next && t == 0) next[0] = 0u;
 }
 
 // Reserves `total` (uniform over the wavefront) list slots for each of the
@@ -711,7 +722,10 @@ __device__ inline unsigned reserve_slots(unsigned* count, unsigned total) {
 constexpr int kGateCols = 62, kGateRows = 16, kGateWaves = 8;
 template <int NW>
 __global__ __launch_bounds__(64 * NW) void k_curv_gate(const double* __restrict__ vof, int nx, int ny,
-                                                       unsigned* __restrict__ count, unsigned* __restrict__ list) {
+                                                       unsigned* __restrict__ count, unsigned* __restrict__ list,
+                                                       float* __restrict__ kappa, const unsigned* __restrict__ prev,
+                                                       unsigned* __restrict__ next) {
+  curv_turnover(kappa, prev, next);
   const int l = threadIdx.x & 63;
   const int i = (int)blockIdx.x * kGateCols + l;
   // wavefronts past row ny load the clamped last row and list nothing
@@ -823,7 +837,10 @@ constexpr int kGateCutWaves = 16;
 template <int NW>
 __global__ __launch_bounds__(64 * NW) void k_curv_gate_cut(const unsigned long long* __restrict__ cut, int strips,
                                                            int nx, int ny, unsigned* __restrict__ count,
-                                                           unsigned* __restrict__ list) {
+                                                           unsigned* __restrict__ list, float* __restrict__ kappa,
+                                                           const unsigned* __restrict__ prev,
+                                                           unsigned* __restrict__ next) {
+  curv_turnover(kappa, prev, next);
   const int l = threadIdx.x & 63, b = (int)blockIdx.x;
   const int j0 = 1 + ((int)blockIdx.y * NW + (int)(threadIdx.x >> 6)) * 64;
   const bool live = j0 <= ny;  // uniform: wavefronts past row ny list nothing
@@ -979,29 +996,30 @@ static int grad_pass(const Samples& s, long first, long stride, int tiles, float
 
 // the kappa field of `cells` cells with its list (cells + 1 entries):
 // NaN everywhere, empty list
-int curv_field_init(float* kappa, unsigned* list, long cells, hipStream_t st) {
-  hipLaunchKernelGGL(mlp::k_curv_fill, dim3(1024), dim3(256), 0, st, kappa, cells, list);
+int curv_field_init(float* kappa, unsigned* const* lists, int nlists, long cells, hipStream_t st) {
+  SBFS_REQUIRE(nlists >= 1 && nlists <= 3, "curv_field_init: 1 to 3 lists");
+  hipLaunchKernelGGL(mlp::k_curv_fill, dim3(1024), dim3(256), 0, st, kappa, cells, lists[0],
+                     nlists > 1 ? lists[1] : nullptr, nlists > 2 ? lists[2] : nullptr);
   return check_launch("k_curv_fill");
 }
 
-// the learned curvature of a stage's input vof: the previous launch's cells
-// back to NaN, then the gate lists the cells of this vof and the network
-// fills them
-int curv_mlp_launch(const double* vof, const unsigned long long* cut, float* kappa, unsigned* list, int nx, int ny,
-                    double h, hipStream_t st) {
+// the learned curvature of a stage's input vof: the gate lists the cells of
+// this vof into `list` (whose count is zero), puts the cells of `prev` back
+// to NaN and empties `next`; then the network fills the listed cells
+int curv_mlp_launch(const double* vof, const unsigned long long* cut, float* kappa, unsigned* list,
+                    const unsigned* prev, unsigned* next, int nx, int ny, double h, hipStream_t st) {
   using namespace mlp;
   SBFS_REQUIRE(g_net.trained, "learned curvature: no trained network (sbfs_mlp_train or sbfs_mlp_set_params)");
-  SBFS_TIMED("k_curv_restore", hipLaunchKernelGGL(k_curv_restore, dim3(256), dim3(256), 0, st, kappa,
-                                                  (const unsigned*)list));
-  hipLaunchKernelGGL(k_curv_reset, dim3(1), dim3(64), 0, st, list);
+  SBFS_REQUIRE(prev != list && next != list && (!prev || prev != next), "curv_mlp_launch: lists must differ");
   if (cut)
     SBFS_TIMED("k_curv_gate_cut",
                hipLaunchKernelGGL(k_curv_gate_cut<kGateCutWaves>, dim3(march_strips(nx), cdiv(ny, kGateCutWaves * 64)),
-                                  dim3(64 * kGateCutWaves), 0, st, cut, march_strips(nx), nx, ny, list, list + 1));
+                                  dim3(64 * kGateCutWaves), 0, st, cut, march_strips(nx), nx, ny, list, list + 1,
+                                  kappa, prev, next));
   else
     SBFS_TIMED("k_curv_gate",
                hipLaunchKernelGGL(k_curv_gate<kGateWaves>, dim3(cdiv(nx, kGateCols), cdiv(ny, kGateWaves * kGateRows)),
-                                  dim3(64 * kGateWaves), 0, st, vof, nx, ny, list, list + 1));
+                                  dim3(64 * kGateWaves), 0, st, vof, nx, ny, list, list + 1, kappa, prev, next));
   SBFS_TIMED("k_curv_mlp", hipLaunchKernelGGL(k_curv_mlp, dim3(4096), dim3(64), 0, st, weights(), vof, kappa,
                                               nx, ny, (float)(1. / h), (const unsigned*)list,
                                               (const unsigned*)(list + 1)));
@@ -1228,8 +1246,10 @@ extern "C" int sbfs_mlp_coverage(const domain_t* domain, const interface_t* inte
   SBFS_HIP(hipMalloc(&list, sizeof(unsigned) * (n + 1)));
   SBFS_HIP(hipMalloc(&cnt, sizeof(unsigned long long)));
   SBFS_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), stream()));
-  int rc = curv_field_init(k, list, n, stream());
-  if (!rc) rc = curv_mlp_launch((const double*)interface->vof.data, nullptr, k, list, nx, ny, domain->hy, stream());
+  int rc = curv_field_init(k, &list, 1, n, stream());
+  if (!rc)
+    rc = curv_mlp_launch((const double*)interface->vof.data, nullptr, k, list, nullptr, nullptr, nx, ny, domain->hy,
+                         stream());
   unsigned long long h = 0;
   if (!rc) {
     hipLaunchKernelGGL(k_count_learned, dim3(1024), dim3(256), 0, stream(), (const float*)k, n, cnt);

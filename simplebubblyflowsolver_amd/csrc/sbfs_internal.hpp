@@ -163,15 +163,19 @@ inline bool fp64_march_addressable(size_t nx, size_t ny) { return (ny + 4) * (nx
 constexpr double kCutLo = 0.02, kCutHi = 0.98;
 // learned curvature (mlp.hip): kappa of the interior cells of `vof`, NaN
 // where the plain curvature stays; h is the (uniform, square) cell size.
-// `list` (nx ny + 1 entries: the count, then the listed cells) belongs to
-// `kappa`: curv_field_init fills a new field with NaN and empties its list,
-// and each curv_mlp_launch first returns the cells its predecessor listed
-// to NaN, so only listed cells ever hold a value.
-int curv_field_init(float* kappa, unsigned* list, long cells, hipStream_t st);
+// The field keeps NaN outside the cells of the last launch's list (each
+// list: nx ny + 1 entries, the count, then the listed cells).
+// curv_field_init fills a new field with NaN and empties up to three lists;
+// curv_mlp_launch lists the cells of `vof` into `list`, whose count must be
+// zero, returns the cells of `prev` (the previous launch's list, null for
+// none) to NaN and empties `next` (the list of the launch after this one,
+// null for none), then fills the listed cells.  A step's stages cycle
+// through three lists.
+int curv_field_init(float* kappa, unsigned* const* lists, int nlists, long cells, hipStream_t st);
 // `cut`, when not null, is the cut-cell bitmask that the stage kernel wrote
 // with this vof (march.hpp), which the gate then reads instead of the vof.
-int curv_mlp_launch(const double* vof, const unsigned long long* cut, float* kappa, unsigned* list, int nx, int ny,
-                    double h, hipStream_t st);
+int curv_mlp_launch(const double* vof, const unsigned long long* cut, float* kappa, unsigned* list,
+                    const unsigned* prev, unsigned* next, int nx, int ny, double h, hipStream_t st);
 // Captured step graphs hold device addresses that their keys do not list
 // (Poisson plan buffers, domain metrics).
 // Whatever frees such memory bumps this epoch, which is part of every key,
