@@ -39,20 +39,23 @@
 
 namespace sbfs {
 
-// value of the neighbouring lane (l-1 / l+1) through DPP wave shifts
+// value of the neighbouring lane (l-1 / l+1) through DPP wave shifts; the
+// lane shifted in from outside the wavefront reads 0 (bound_ctrl), so the
+// destination needs no clearing move first (every shift runs with all lanes
+// active, where this equals keeping an old value of 0)
 __device__ __forceinline__ double from_left(double v) {
   const unsigned long long b = (unsigned long long)__double_as_longlong(v);
   int lo = (int)(unsigned)(b & 0xffffffffull), hi = (int)(unsigned)(b >> 32);
-  lo = __builtin_amdgcn_update_dpp(0, lo, 0x138, 0xf, 0xf, false);  // wave_shr:1
-  hi = __builtin_amdgcn_update_dpp(0, hi, 0x138, 0xf, 0xf, false);
+  lo = __builtin_amdgcn_update_dpp(0, lo, 0x138, 0xf, 0xf, true);  // wave_shr:1
+  hi = __builtin_amdgcn_update_dpp(0, hi, 0x138, 0xf, 0xf, true);
   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
 __device__ __forceinline__ double from_right(double v) {
   const unsigned long long b = (unsigned long long)__double_as_longlong(v);
   int lo = (int)(unsigned)(b & 0xffffffffull), hi = (int)(unsigned)(b >> 32);
-  lo = __builtin_amdgcn_update_dpp(0, lo, 0x130, 0xf, 0xf, false);  // wave_shl:1
-  hi = __builtin_amdgcn_update_dpp(0, hi, 0x130, 0xf, 0xf, false);
+  lo = __builtin_amdgcn_update_dpp(0, lo, 0x130, 0xf, 0xf, true);  // wave_shl:1
+  hi = __builtin_amdgcn_update_dpp(0, hi, 0x130, 0xf, 0xf, true);
   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
@@ -150,10 +153,10 @@ struct Arith<double> {
 template <>
 struct Arith<float> {
   static __device__ __forceinline__ float left(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, true));
   }
   static __device__ __forceinline__ float right(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, true));
   }
   static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
   static __device__ __forceinline__ float sqr(float x) { return __builtin_amdgcn_sqrtf(x); }
