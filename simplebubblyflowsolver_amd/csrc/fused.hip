@@ -61,10 +61,12 @@ __device__ __forceinline__ double post_ld(const double* f, O e) {
   using G = __attribute__((address_space(1))) const char;
   return *(const double*)((G*)f + e * (O)sizeof(double));
 }
+// (streaming stores: the velocities and the pressure are read again only by
+// the next step, 487 -> 463 us at 4096 x 8192)
 template <typename O>
 __device__ __forceinline__ void post_st(double* f, O e, double v) {
   using G = __attribute__((address_space(1))) char;
-  *(double*)((G*)f + e * (O)sizeof(double)) = v;
+  __builtin_nontemporal_store(v, (__attribute__((address_space(1))) double*)((G*)f + e * (O)sizeof(double)));
 }
 
 // rows j0 .. j0 + N - 1 of column i; with All every load is issued before

@@ -101,6 +101,13 @@ __device__ __forceinline__ T& lane_at(RowRef<T> r, unsigned loff) {
   return *(T*)(base + (r.off + loff));
 }
 
+// the lane's element of a row as a global pointer (for streaming stores)
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* lane_gp(RowRef<T> r, unsigned loff) {
+  GlobalBytes<T>* base = (GlobalBytes<T>*)r.base;
+  return (__attribute__((address_space(1))) T*)(base + (r.off + loff));
+}
+
 // A wave-uniform value pinned to SGPRs.  FP64 arithmetic has no scalar
 // form, so a constant such as dt * gamma is formed on the vector unit and
 // would otherwise hold a VGPR pair for the whole kernel.
@@ -289,6 +296,14 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
   auto fx_row = [&](auto* f, int j) { return row(f, (ptrdiff_t)(j - 1) * (nx - 1) + cl - 2); };  // L::fx
   const unsigned loff = (unsigned)lane * 8u;
   auto AT = [&](auto r) -> auto& { return lane_at(r, loff); };
+  // Output rows: the FP64 bodies store them streaming (non-temporal), which
+  // measured k_march 1198 -> 1164 us and k_march_corr 1502 -> 1446 us at
+  // 4096 x 8192; the FP32 bodies slowed by 4-5 % with it and keep ordinary
+  // stores.
+  auto ST = [&](auto r, double v) {
+    if constexpr (std::is_same<R, double>::value) __builtin_nontemporal_store(v, lane_gp(r, loff));
+    else AT(r) = v;
+  };
   // loads of row jw, already wrapped into 1..ny
   auto ld_vof = [&](int jw) { return c_vof ? AT(vof_row(a.vof, jw)) : 0.; };
   auto ld_ux = [&](int jw) { return c_fx ? AT(xf_row(a.ux, jw)) : 0.; };
@@ -690,14 +705,14 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
       }
       if (outrow && c_out) {
         auto put = [&](double* f, int nh, auto row, double v) {
-          AT(row(f, s)) = v;
-          if (s > ny - nh) AT(row(f, s - ny)) = v;
-          if (s <= nh) AT(row(f, s + ny)) = v;
+          ST(row(f, s), v);
+          if (s > ny - nh) ST(row(f, s - ny), v);
+          if (s <= nh) ST(row(f, s + ny), v);
         };
         auto put_at = [&](double* f, int nh, auto row, int off, double v) {
-          AT(row(f, s) + off) = v;
-          if (s > ny - nh) AT(row(f, s - ny) + off) = v;
-          if (s <= nh) AT(row(f, s + ny) + off) = v;
+          ST(row(f, s) + off, v);
+          if (s > ny - nh) ST(row(f, s - ny) + off, v);
+          if (s <= nh) ST(row(f, s + ny) + off, v);
         };
         auto rv = [&](double* f, int j) { return vof_row(f, j); };
         auto rc = [&](double* f, int j) { return cc_row(f, j); };
@@ -705,7 +720,7 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
         put(a.vofn, 2, rv, vn0);
         if (c == 1) put_at(a.vofn, 2, rv, -1, 0.);
         if (c == nx) put_at(a.vofn, 2, rv, 1, 0.);
-        if (a.srca_vof) AT(ci_row(a.srca_vof, s)) = src;
+        if (a.srca_vof) ST(ci_row(a.srca_vof, s), src);
         put(a.uxs, 1, rx, c == 1 ? 0. : uxs);
         if (c == nx) put_at(a.uxs, 1, rx, 1, 0.);
         put(a.uys, 1, rc, uys);
@@ -750,7 +765,7 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
       if (r >= jb && r <= je && c_out) {
         const double div = TB(T_RJC) * (-q_uxs + q_uxsr - ring_ld(RG_CYCUYS, cyc_uys_m1) + cyc_uys0);
         const double add = TB(T_RJC) * (-qgx + qgxr - ring_ld(RG_CGY, cg_ym) + cg_yp);
-        AT(ci_row(a.rhs, r)) = c0 * div + c1 * add;
+        ST(ci_row(a.rhs, r), c0 * div + c1 * add);
       }
       ring_st(RG_CGY, cg_ym, cg_yp);
       ring_st(RG_CYCUYS, cyc_uys_m1, cyc_uys0);

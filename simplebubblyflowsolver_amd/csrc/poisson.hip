@@ -217,6 +217,20 @@ __global__ void k_dct2_rhs(const double* __restrict__ ux, const double* __restri
 // ---------------------------------------------------------------------------
 
 // REDFT10 of rows row0 .. row0 + R - 1 (of `rows`) of `in` into `out`
+// Streaming (non-temporal) loads and stores for the rows the transforms read
+// and write once, so that they do not evict the twiddles, the spike tables
+// and the separators, which every row reads again from L2.  At 4096 x 8192:
+// k_dct3_psi 226 -> 173 us, k_dct2_rows 128 -> 119 us.  (The same hints on the
+// partitioned solve's rows slowed the DCT-III that follows it more than they
+// sped the solve up, so it keeps ordinary accesses.)
+__device__ __forceinline__ double2 ld_stream(const double2* p) {
+  return make_double2(__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y));
+}
+__device__ __forceinline__ void st_stream(double2* p, double2 v) {
+  __builtin_nontemporal_store(v.x, &p->x);
+  __builtin_nontemporal_store(v.y, &p->y);
+}
+
 // the unit's rows in LDS: row rr at lds + rr PM, doubles through zd().
 // Rows past the last (rr >= nrows >= 1) load the last row again, so every
 // load is unconditional and all are in flight before the first LDS write.
@@ -256,7 +270,7 @@ __device__ __forceinline__ void dct2_wave(const double* __restrict__ in, double*
   if (nrows <= 0) return;  // a second unit past the last row (one wavefront, no barrier)
   const auto tw0 = pass_tw<LOG, 0>(lt, tw);
   const double2* src = (const double2*)(in + (size_t)row0 * N);
-  rows_to_lds<LOG>(lds, lane, nrows, [src](int rr, int h) { return src[rr * (N / 2) + h]; });
+  rows_to_lds<LOG>(lds, lane, nrows, [src](int rr, int h) { return ld_stream(src + rr * (N / 2) + h); });
   F::sync();
   double2* Z = lds + rg * F::PM;
   // v[k] = x[2k], v[n-1-k] = x[2k+1]; z[k] = v[2k] + i v[2k+1]
@@ -284,8 +298,8 @@ __device__ __forceinline__ void dct2_wave(const double* __restrict__ in, double*
     const double2 tt = wmul(od, t1[k]);
     const double2 vv = make_double2(er + tt.x, ei + tt.y);
     const double2 w = wmul(vv, t2[k]);
-    dst[k] = 2. * w.x;
-    if (k != 0 && k != M) dst[N - k] = -2. * w.y;
+    __builtin_nontemporal_store(2. * w.x, dst + k);
+    if (k != 0 && k != M) __builtin_nontemporal_store(-2. * w.y, dst + (N - k));
   }
 }
 
@@ -308,10 +322,12 @@ struct ModeSrc {
 
 // rows_to_lds for the partitioned solve's modes (L != 0).  The chunk rows g
 // of all the lane's points are loaded first (from HBM), then the separators
-// and spikes in groups of four points (from L2), so that the unit waits for
-// one HBM latency and PER / 4 L2 latencies rather than one of each per
-// point.  With one row per unit (R = 1) the row, its chunk and whether it
-// is the chunk's last row (the separator itself) are the same for every
+// and spikes in groups of points (from L2), so that the unit waits for one
+// HBM latency and PER / GRP L2 latencies rather than one of each per point.
+// GRP is 8 where the unit holds 2048 complex points (the group's 32 loads fit
+// the registers the FFT leaves; 233 -> 226 us at 4096 x 8192), else 4.  With
+// one row per unit (R = 1) the row, its chunk and whether it is the chunk's
+// last row (the separator itself) are the same for every
 // lane; with several, the loads are unconditional (a chunk's last row
 // reads spike row 0, whose product it discards) and the value a select.
 template <int LOG>
@@ -319,7 +335,7 @@ __device__ __forceinline__ void expanded_to_lds(double2* lds, int lane, int nrow
   using F = WaveFft<LOG>;
   constexpr int H = F::N / 2;  // double2 per row
   static_assert(F::R * H >= F::U, "every lane loads at least one point");
-  constexpr int PER = F::R * H / F::U, GRP = PER < 4 ? PER : 4;
+  constexpr int PER = F::R * H / F::U, GMAX = LOG >= 11 ? 8 : 4, GRP = PER < GMAX ? PER : GMAX;
   static_assert(PER % GRP == 0, "groups of points");
   const int nx = src.nx, sh = 31 - __clz(src.L);
   auto row = [&](int i) { return F::R == 1 ? row0 : row0 + min((lane + i * F::U) / H, nrows - 1); };
@@ -342,7 +358,7 @@ __device__ __forceinline__ void expanded_to_lds(double2* lds, int lane, int nrow
   }
   double2 g[PER];
 #pragma unroll
-  for (int i = 0; i < PER; i++) g[i] = *(const double2*)(src.buf + (size_t)row(i) * nx + col(i));
+  for (int i = 0; i < PER; i++) g[i] = ld_stream((const double2*)(src.buf + (size_t)row(i) * nx + col(i)));
 #pragma unroll
   for (int i0 = 0; i0 < PER; i0 += GRP) {
     double2 xu[GRP], xl[GRP], wk[GRP], zk[GRP];
@@ -474,8 +490,8 @@ __device__ __forceinline__ void dct3_wave(const ModeSrc& src, int rows, int row0
       const int q = lt + u * G;
       if (q > N / 2) break;
       const double2 v = make_double2(y(max(2 * q - 1, 0)), y(min(2 * q, N - 1)));
-      row[q] = v;
-      if (halo) halo[q] = v;
+      st_stream(row + q, v);
+      if (halo) st_stream(halo + q, v);
     }
   } else {
     double* dst = out + (size_t)r * N;
