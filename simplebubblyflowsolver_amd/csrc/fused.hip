@@ -61,17 +61,19 @@ __device__ __forceinline__ double post_ld(const double* f, O e) {
   using G = __attribute__((address_space(1))) const char;
   return *(const double*)((G*)f + e * (O)sizeof(double));
 }
-// (streaming stores: the velocities and the pressure are read again only by
-// the next step, 487 -> 463 us at 4096 x 8192)
-template <typename O>
+// (with NT, streaming stores: the velocities and the pressure are read again
+// only by the next step; 487 -> 463 us at 4096 x 8192, see stream_fields)
+template <typename O, bool NT>
 __device__ __forceinline__ void post_st(double* f, O e, double v) {
   using G = __attribute__((address_space(1))) char;
-  __builtin_nontemporal_store(v, (__attribute__((address_space(1))) double*)((G*)f + e * (O)sizeof(double)));
+  auto* q = (__attribute__((address_space(1))) double*)((G*)f + e * (O)sizeof(double));
+  if constexpr (NT) __builtin_nontemporal_store(v, q);
+  else *q = v;
 }
 
 // rows j0 .. j0 + N - 1 of column i; with All every load is issued before
 // the first row's arithmetic (the full tiles), otherwise row by row
-template <typename O, int N, bool All>
+template <typename O, int N, bool All, bool NT>
 __device__ __forceinline__ void post_column(const PostArgs& a, int i, int j0, int nj) {
   const int nx = a.nx, ny = a.ny;
   const double dt = *a.dt;
@@ -138,9 +140,9 @@ __device__ __forceinline__ void post_column(const PostArgs& a, int i, int j0, in
         const double dd = 0.5 * mix_clamp(a.denr, a.dlo, a.dhi, vfl[r]) + 0.5 * d;
         u += cdc * (1. / dd - rr) / h * (-p0l[r] + p0[r + 1]);
       }
-      post_st(a.ux, ox, u);
-      if (j > ny - 1) post_st(a.ux, ox - (O)ny * wf, u);
-      if (j <= 1) post_st(a.ux, ox + (O)ny * wf, u);
+      post_st<O, NT>(a.ux, ox, u);
+      if (j > ny - 1) post_st<O, NT>(a.ux, ox - (O)ny * wf, u);
+      if (j <= 1) post_st<O, NT>(a.ux, ox + (O)ny * wf, u);
     }
     {
       double u;
@@ -152,16 +154,16 @@ __device__ __forceinline__ void post_column(const PostArgs& a, int i, int j0, in
         const double dd = 0.5 * du + 0.5 * d;
         u += cdc * (1. / dd - rr) / a.hy * (-p0[r] + p0[r + 1]);
       }
-      post_st(a.uy, oc, u);
-      if (j > ny - 1) post_st(a.uy, oc - (O)ny * wc, u);
-      if (j <= 1) post_st(a.uy, oc + (O)ny * wc, u);
+      post_st<O, NT>(a.uy, oc, u);
+      if (j > ny - 1) post_st<O, NT>(a.uy, oc - (O)ny * wc, u);
+      if (j <= 1) post_st<O, NT>(a.uy, oc + (O)ny * wc, u);
     }
     if (cin) {
       const double v = pp[r] + p1[r + 1];
       auto put = [&](O o) {
-        post_st(a.p, o, v);
-        if (j > ny - 1) post_st(a.p, o - (O)ny * wc, v);
-        if (j <= 1) post_st(a.p, o + (O)ny * wc, v);
+        post_st<O, NT>(a.p, o, v);
+        if (j > ny - 1) post_st<O, NT>(a.p, o - (O)ny * wc, v);
+        if (j <= 1) post_st<O, NT>(a.p, o + (O)ny * wc, v);
       };
       put(oc);
       if (i == 1) put(oc - 1);
@@ -171,18 +173,18 @@ __device__ __forceinline__ void post_column(const PostArgs& a, int i, int j0, in
   }
 }
 
-template <typename O>
+template <typename O, bool NT>
 __device__ __forceinline__ void post_rows(const PostArgs& a, int i, int j0) {
   if (i > a.nx + 1 || j0 > a.ny) return;
   const int nj = min(kPostRows, a.ny - j0 + 1);
-  if (nj == kPostRows) post_column<O, kPostRows, true>(a, i, j0, nj);
-  else post_column<O, kPostRows, false>(a, i, j0, nj);
+  if (nj == kPostRows) post_column<O, kPostRows, true, NT>(a, i, j0, nj);
+  else post_column<O, kPostRows, false, NT>(a, i, j0, nj);
 }
 
 // tiles of 64 columns x 4 kPostRows rows, a thread per column and kPostRows rows
-template <typename O>
+template <typename O, bool NT>
 __global__ __launch_bounds__(256) void k_post(PostArgs a) {
-  post_rows<O>(a, (int)(blockIdx.x * 64 + threadIdx.x), 1 + (int)(blockIdx.y * 4 + threadIdx.y) * kPostRows);
+  post_rows<O, NT>(a, (int)(blockIdx.x * 64 + threadIdx.x), 1 + (int)(blockIdx.y * 4 + threadIdx.y) * kPostRows);
 }
 
 // one tile per map entry (case, local tile); ensemble fields stay below 4 GiB
@@ -190,7 +192,7 @@ __global__ __launch_bounds__(256) void k_post_batch(const PostArgs* __restrict__
   const uint2 m = map[blockIdx.x];
   const PostArgs& a = tab[m.x];
   const int gx = (a.nx + 2 + 63) / 64;
-  post_rows<unsigned>(a, (int)((m.y % gx) * 64 + threadIdx.x),
+  post_rows<unsigned, false>(a, (int)((m.y % gx) * 64 + threadIdx.x),
                       1 + (int)((m.y / gx) * 4 + threadIdx.y) * kPostRows);
 }
 
@@ -354,6 +356,8 @@ static void swap_data(array_t& x, array_t& y) {
 
 __global__ void k_march(const MarchArgs* a);             // march.hip, plain FP64
 __global__ void k_march_corr(const MarchArgs* a);        // + the previous stage's projection
+__global__ void k_march_nt(const MarchArgs* a);          // k_march with streaming output stores
+__global__ void k_march_corr_nt(const MarchArgs* a);     // k_march_corr with streaming output stores
 __global__ void k_march_mixed(const MarchArgs* a);       // FP32 stencils
 __global__ void k_march_mixed_corr(const MarchArgs* a);  // FP32 stencils + projection
 
@@ -619,6 +623,7 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
   if (compute_dt_device(domain, c, fluid, interface)) return 1;
   const double* dtp = c->dt_dev;
   const bool fold = march_fold();
+  const bool nt = stream_fields(nx, ny);  // streaming accesses of the rows touched once
   // the units in order of the interface rows the last stage kernel found in
   // them (the previous step's; the bubbles move a fraction of a cell per
   // step), for all three stages of this step
@@ -669,9 +674,9 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
       else if (mixed)
         SBFS_TIMED("k_march_mixed", hipLaunchKernelGGL(k_march_mixed, grid, block, 0, stream(), ma));
       else if (corr)
-        SBFS_TIMED("k_march_corr", hipLaunchKernelGGL(k_march_corr, grid, block, 0, stream(), ma));
+        SBFS_TIMED("k_march_corr", hipLaunchKernelGGL((nt ? k_march_corr_nt : k_march_corr), grid, block, 0, stream(), ma));
       else
-        SBFS_TIMED("k_march", hipLaunchKernelGGL(k_march, grid, block, 0, stream(), ma));
+        SBFS_TIMED("k_march", hipLaunchKernelGGL((nt ? k_march_nt : k_march), grid, block, 0, stream(), ma));
       if (check_launch("integrate: k_march")) return 1;
       if (corr && k == 2) {
         // the stage kernel wrote the pressure after stage 1's update
@@ -698,10 +703,13 @@ static int fused_body(const domain_t* domain, DomainCtx* c, fluid_t* fluid, inte
     PostArgs q;
     fill_post(c, fluid, interface, us[k], vs[k], dtp, k, q);
     const dim3 pg(cdiv(nx + 2, 64), cdiv(ny, 4 * kPostRows)), pb(64, 4);
-    if (fp64_march_addressable(nx, ny))
-      SBFS_TIMED("k_post", hipLaunchKernelGGL(k_post<unsigned>, pg, pb, 0, stream(), q));
+    // (fields of 4 GiB or more are always streamed)
+    if (!fp64_march_addressable(nx, ny))
+      SBFS_TIMED("k_post", hipLaunchKernelGGL((k_post<size_t, true>), pg, pb, 0, stream(), q));
+    else if (stream_fields(nx, ny))
+      SBFS_TIMED("k_post", hipLaunchKernelGGL((k_post<unsigned, true>), pg, pb, 0, stream(), q));
     else
-      SBFS_TIMED("k_post", hipLaunchKernelGGL(k_post<size_t>, pg, pb, 0, stream(), q));
+      SBFS_TIMED("k_post", hipLaunchKernelGGL((k_post<unsigned, false>), pg, pb, 0, stream(), q));
     if (check_launch("integrate: k_post")) return 1;
     swap_data(fluid->psi[0], fluid->psi[1]);
   }

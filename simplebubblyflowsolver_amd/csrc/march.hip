@@ -220,7 +220,8 @@ __device__ __forceinline__ R mixr(double ratio, double lo, double hi, R v) {
 // buoyancy) in FP32; the vof update, the density-weighted velocity update,
 // the pressure gradient and the Poisson right-hand side stay FP64.
 // (bx, by): the strip and band of this wavefront
-template <typename R, bool CORR>
+// NT: the FP64 bodies store their output rows streaming (see ST below)
+template <typename R, bool CORR, bool NT = false>
 __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
   using A = Arith<R>;
   using V3 = V3R<R>;
@@ -296,12 +297,14 @@ __device__ __forceinline__ void march_body(const MarchArgs& a, int bx, int by) {
   auto fx_row = [&](auto* f, int j) { return row(f, (ptrdiff_t)(j - 1) * (nx - 1) + cl - 2); };  // L::fx
   const unsigned loff = (unsigned)lane * 8u;
   auto AT = [&](auto r) -> auto& { return lane_at(r, loff); };
-  // Output rows: the FP64 bodies store them streaming (non-temporal), which
+  // Output rows: with NT (fields too large for the Infinity Cache to keep,
+  // stream_fields) the FP64 bodies store them streaming (non-temporal), which
   // measured k_march 1198 -> 1164 us and k_march_corr 1502 -> 1446 us at
   // 4096 x 8192; the FP32 bodies slowed by 4-5 % with it and keep ordinary
   // stores.
+  static_assert(!NT || std::is_same<R, double>::value, "streaming stores in the FP64 bodies only");
   auto ST = [&](auto r, double v) {
-    if constexpr (std::is_same<R, double>::value) __builtin_nontemporal_store(v, lane_gp(r, loff));
+    if constexpr (NT) __builtin_nontemporal_store(v, lane_gp(r, loff));
     else AT(r) = v;
   };
   // loads of row jw, already wrapped into 1..ny
@@ -923,5 +926,24 @@ SBFS_MARCH_KERNEL(k_march_corr, double, true, SBFS_FP64_WAVES)
 SBFS_MARCH_KERNEL(k_march_mixed, float, false, SBFS_MIXED_WAVES)
 SBFS_MARCH_KERNEL(k_march_mixed_corr, float, true, SBFS_MIXED_WAVES)
 #undef SBFS_MARCH_KERNEL
+// the FP64 stage kernels with streaming output stores (single cases whose
+// fields the Infinity Cache cannot keep, stream_fields)
+#define SBFS_MARCH_NT_KERNEL(name, CORR)                                                         \
+  __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SBFS_FP64_WAVES))) void name( \
+      const MarchArgs* __restrict__ args) {                                                      \
+    const MarchArgs& a = *args;                                                                  \
+    int bx = blockIdx.x, by = blockIdx.y;                                                        \
+    if (a.order) {                                                                               \
+      const unsigned u = a.order[blockIdx.x];                                                    \
+      if (u >= (unsigned)march_units(a.nx, a.ny, a.band)) return;                                \
+      const int strips = march_strips(a.nx);                                                     \
+      bx = (int)u % strips;                                                                      \
+      by = (int)u / strips;                                                                      \
+    }                                                                                            \
+    march_body<double, CORR, true>(a, bx, by);                                                   \
+  }
+SBFS_MARCH_NT_KERNEL(k_march_nt, false)
+SBFS_MARCH_NT_KERNEL(k_march_corr_nt, true)
+#undef SBFS_MARCH_NT_KERNEL
 
 }  // namespace sbfs

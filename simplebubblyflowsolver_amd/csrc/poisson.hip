@@ -217,18 +217,31 @@ __global__ void k_dct2_rhs(const double* __restrict__ ux, const double* __restri
 // ---------------------------------------------------------------------------
 
 // REDFT10 of rows row0 .. row0 + R - 1 (of `rows`) of `in` into `out`
-// Streaming (non-temporal) loads and stores for the rows the transforms read
-// and write once, so that they do not evict the twiddles, the spike tables
-// and the separators, which every row reads again from L2.  At 4096 x 8192:
-// k_dct3_psi 226 -> 173 us, k_dct2_rows 128 -> 119 us.  (The same hints on the
-// partitioned solve's rows slowed the DCT-III that follows it more than they
-// sped the solve up, so it keeps ordinary accesses.)
-__device__ __forceinline__ double2 ld_stream(const double2* p) {
-  return make_double2(__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y));
+// With NT, streaming (non-temporal) loads and stores for the rows the
+// transforms read and write once, so that they do not evict the twiddles, the
+// spike tables and the separators, which every row reads again from L2: at
+// 4096 x 8192, k_dct3_psi 226 -> 173 us and k_dct2_rows 128 -> 119 us.  The
+// host picks NT for fields too large to stay in the Infinity Cache between
+// kernels (stream_fields); the ensemble's batched kernels never stream.  (The
+// same hints on the partitioned solve's rows slowed the DCT-III that follows
+// it more than they sped the solve up, so it keeps ordinary accesses.)
+template <bool NT>
+__device__ __forceinline__ double2 ld_row(const double2* p) {
+  if constexpr (NT) return make_double2(__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y));
+  else return *p;
 }
-__device__ __forceinline__ void st_stream(double2* p, double2 v) {
-  __builtin_nontemporal_store(v.x, &p->x);
-  __builtin_nontemporal_store(v.y, &p->y);
+template <bool NT, typename T>
+__device__ __forceinline__ void st_row(T* p, T v) {
+  if constexpr (NT) {
+    if constexpr (std::is_same<T, double2>::value) {
+      __builtin_nontemporal_store(v.x, &p->x);
+      __builtin_nontemporal_store(v.y, &p->y);
+    } else {
+      __builtin_nontemporal_store(v, p);
+    }
+  } else {
+    *p = v;
+  }
 }
 
 // the unit's rows in LDS: row rr at lds + rr PM, doubles through zd().
@@ -258,7 +271,7 @@ __device__ __forceinline__ void rows_to_lds(double2* lds, int lane, int nrows, S
 }
 
 // (the unit's LDS rows at `lds`, its thread `lane` in [0, U))
-template <int LOG>
+template <int LOG, bool NT>
 __device__ __forceinline__ void dct2_wave(const double* __restrict__ in, double* __restrict__ out, int rows,
                                           int row0, const double2* __restrict__ tw,
                                           const double2* __restrict__ t1, const double2* __restrict__ t2,
@@ -270,7 +283,7 @@ __device__ __forceinline__ void dct2_wave(const double* __restrict__ in, double*
   if (nrows <= 0) return;  // a second unit past the last row (one wavefront, no barrier)
   const auto tw0 = pass_tw<LOG, 0>(lt, tw);
   const double2* src = (const double2*)(in + (size_t)row0 * N);
-  rows_to_lds<LOG>(lds, lane, nrows, [src](int rr, int h) { return ld_stream(src + rr * (N / 2) + h); });
+  rows_to_lds<LOG>(lds, lane, nrows, [src](int rr, int h) { return ld_row<NT>(src + rr * (N / 2) + h); });
   F::sync();
   double2* Z = lds + rg * F::PM;
   // v[k] = x[2k], v[n-1-k] = x[2k+1]; z[k] = v[2k] + i v[2k+1]
@@ -298,8 +311,8 @@ __device__ __forceinline__ void dct2_wave(const double* __restrict__ in, double*
     const double2 tt = wmul(od, t1[k]);
     const double2 vv = make_double2(er + tt.x, ei + tt.y);
     const double2 w = wmul(vv, t2[k]);
-    __builtin_nontemporal_store(2. * w.x, dst + k);
-    if (k != 0 && k != M) __builtin_nontemporal_store(-2. * w.y, dst + (N - k));
+    st_row<NT>(dst + k, 2. * w.x);
+    if (k != 0 && k != M) st_row<NT>(dst + (N - k), -2. * w.y);
   }
 }
 
@@ -330,7 +343,7 @@ struct ModeSrc {
 // last row (the separator itself) are the same for every
 // lane; with several, the loads are unconditional (a chunk's last row
 // reads spike row 0, whose product it discards) and the value a select.
-template <int LOG>
+template <int LOG, bool NT>
 __device__ __forceinline__ void expanded_to_lds(double2* lds, int lane, int nrows, const ModeSrc& src, int row0) {
   using F = WaveFft<LOG>;
   constexpr int H = F::N / 2;  // double2 per row
@@ -358,7 +371,7 @@ __device__ __forceinline__ void expanded_to_lds(double2* lds, int lane, int nrow
   }
   double2 g[PER];
 #pragma unroll
-  for (int i = 0; i < PER; i++) g[i] = ld_stream((const double2*)(src.buf + (size_t)row(i) * nx + col(i)));
+  for (int i = 0; i < PER; i++) g[i] = ld_row<NT>((const double2*)(src.buf + (size_t)row(i) * nx + col(i)));
 #pragma unroll
   for (int i0 = 0; i0 < PER; i0 += GRP) {
     double2 xu[GRP], xl[GRP], wk[GRP], zk[GRP];
@@ -387,12 +400,12 @@ __device__ __forceinline__ void expanded_to_lds(double2* lds, int lane, int nrow
 // psi[1] rows j = row + 1 with the Neumann x ghosts and the periodic y halo
 // (compute_potential.c:365-380, src/fluid/boundary/psi.c; out must be
 // 16-byte aligned); otherwise plain rows of N
-template <int LOG, bool PSI, int AHEAD>
+template <int LOG, bool PSI, int AHEAD, bool NT>
 __device__ __forceinline__ void dct3_wave(const ModeSrc& src, int rows, int row0, const double2* __restrict__ tw,
                                           const double2* __restrict__ t1, const double2* __restrict__ t2,
                                           double* __restrict__ out, double2* lds, int lane) {
   using F = WaveFft<LOG>;
-  constexpr int M = F::M, N = F::N, G = F::G, U = F::U;
+  constexpr int M = F::M, N = F::N, G = F::G;
   const int rg = lane / G, lt = lane % G;
   const int nrows = min(F::R, rows - row0);
   if (nrows <= 0) return;  // a second unit past the last row (one wavefront, no barrier)
@@ -417,7 +430,7 @@ __device__ __forceinline__ void dct3_wave(const ModeSrc& src, int rows, int row0
   if (src.L == 0)
     rows_to_lds<LOG>(lds, lane, nrows, [&src, row0](int rr, int h) { return src.solved(row0 + rr, h); });
   else
-    expanded_to_lds<LOG>(lds, lane, nrows, src, row0);
+    expanded_to_lds<LOG, NT>(lds, lane, nrows, src, row0);
   F::sync();
   double2* Z = lds + rg * F::PM;
   // pre-twiddle in place: the values of k and M - k (0 <= k <= M/2) need
@@ -490,8 +503,8 @@ __device__ __forceinline__ void dct3_wave(const ModeSrc& src, int rows, int row0
       const int q = lt + u * G;
       if (q > N / 2) break;
       const double2 v = make_double2(y(max(2 * q - 1, 0)), y(min(2 * q, N - 1)));
-      st_stream(row + q, v);
-      if (halo) st_stream(halo + q, v);
+      st_row<NT>(row + q, v);
+      if (halo) st_row<NT>(halo + q, v);
     }
   } else {
     double* dst = out + (size_t)r * N;
@@ -509,12 +522,12 @@ __device__ __forceinline__ void dct3_wave(const ModeSrc& src, int rows, int row0
 // hoisting every load
 #define SBFS_ROWS_ATTR(LG) __launch_bounds__(wave_fft_u(LG)) __attribute__((amdgpu_waves_per_eu(2)))
 
-template <int LOG>
+template <int LOG, bool NT>
 __global__ SBFS_ROWS_ATTR(LOG) void k_dct2_rows(const double* __restrict__ in, double* __restrict__ out, int rows,
                                                 const double2* __restrict__ tw, const double2* __restrict__ t1,
                                                 const double2* __restrict__ t2) {
   extern __shared__ double2 lds[];
-  dct2_wave<LOG>(in, out, rows, (int)blockIdx.x * WaveFft<LOG>::R, tw, t1, t2, lds, (int)threadIdx.x);
+  dct2_wave<LOG, NT>(in, out, rows, (int)blockIdx.x * WaveFft<LOG>::R, tw, t1, t2, lds, (int)threadIdx.x);
 }
 
 template <int LOG>
@@ -524,10 +537,10 @@ __global__ SBFS_ROWS_ATTR(LOG) void k_dct3_rows(const double* __restrict__ in, d
   constexpr int N = WaveFft<LOG>::N;
   extern __shared__ double2 lds[];
   const ModeSrc src{in, nullptr, nullptr, nullptr, N, 0, 0};
-  dct3_wave<LOG, false, 1>(src, rows, (int)blockIdx.x * WaveFft<LOG>::R, tw, t1, t2, out, lds, (int)threadIdx.x);
+  dct3_wave<LOG, false, 1, false>(src, rows, (int)blockIdx.x * WaveFft<LOG>::R, tw, t1, t2, out, lds, (int)threadIdx.x);
 }
 
-template <int LOG>
+template <int LOG, bool NT>
 __global__ SBFS_ROWS_ATTR(LOG) void k_dct3_psi(ModeSrc src, double* __restrict__ psi, int ny,
                                                const double2* __restrict__ tw, const double2* __restrict__ t1,
                                                const double2* __restrict__ t2) {
@@ -545,7 +558,7 @@ __global__ SBFS_ROWS_ATTR(LOG) void k_dct3_psi(ModeSrc src, double* __restrict__
   } else {
     unit = xcd_group(b, n);
   }
-  dct3_wave<LOG, true, 1>(src, ny, unit * WaveFft<LOG>::R, tw, t1, t2, psi, lds, (int)threadIdx.x);
+  dct3_wave<LOG, true, 1, NT>(src, ny, unit * WaveFft<LOG>::R, tw, t1, t2, psi, lds, (int)threadIdx.x);
 }
 
 #define SBFS_LOG_CASES(X) \
@@ -627,7 +640,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void k
   case LG: {                                                                                            \
     using F = WaveFft<LG>;                                                                              \
     const int units = (int)blockDim.x / F::U, u = (int)threadIdx.x / F::U;                              \
-    dct2_wave<LG>(a.rhs, a.buf, a.ny, ((int)mp.y * units + u) * F::R, a.tw, a.t1, a.t2,                   \
+    dct2_wave<LG, false>(a.rhs, a.buf, a.ny, ((int)mp.y * units + u) * F::R, a.tw, a.t1, a.t2,                   \
                   lds + (size_t)u * F::R * F::PM, (int)threadIdx.x % F::U);                              \
   } break;
     SBFS_LOG_CASES(SBFS_DCT2_CASE)
@@ -654,7 +667,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void k
   case LG: {                                                                                            \
     using F = WaveFft<LG>;                                                                              \
     const int units = (int)blockDim.x / F::U, u = (int)threadIdx.x / F::U;                              \
-    dct3_wave<LG, true, 0>(src, a.ny, ((int)mp.y * units + u) * F::R, a.tw, a.t1, a.t2, a.psi,              \
+    dct3_wave<LG, true, 0, false>(src, a.ny, ((int)mp.y * units + u) * F::R, a.tw, a.t1, a.t2, a.psi,              \
                         lds + (size_t)u * F::R * F::PM, (int)threadIdx.x % F::U);                        \
   } break;
     SBFS_LOG_CASES(SBFS_DCT3_CASE)
@@ -770,10 +783,12 @@ static int poisson_tail(const domain_t* d, PoissonPlan* p, fluid_t* f) {
   }
   double* psi = (double*)f->psi[1].data;
   SBFS_REQUIRE(((uintptr_t)psi & 15) == 0, "poisson: psi[1] must be 16-byte aligned (hipMalloc'ed)");
+  const bool nt = stream_fields(nx, ny);
   switch (ilog2(m)) {
 #define SBFS_TAIL_CASE(LG)                                                                          \
   case LG:                                                                                          \
-    SBFS_TIMED("k_dct3_psi", hipLaunchKernelGGL(k_dct3_psi<LG>, dim3(cdiv(ny, WaveFft<LG>::R)), dim3(WaveFft<LG>::U), \
+    SBFS_TIMED("k_dct3_psi", hipLaunchKernelGGL((nt ? k_dct3_psi<LG, true> : k_dct3_psi<LG, false>),      \
+                                                dim3(cdiv(ny, WaveFft<LG>::R)), dim3(WaveFft<LG>::U),      \
                                                 wave_fft_lds(LG), stream(), src, psi, ny, p->tw, p->t1,    \
                                                 p->t2));                                            \
     break;
@@ -795,10 +810,12 @@ int poisson_transform_fused(const domain_t* d, DomainCtx* c, fluid_t* f, double*
     return 0;
   }
   const int ny = p->ny;
+  const bool nt = stream_fields(p->nx, ny);
   switch (ilog2(p->m)) {
 #define SBFS_ROWS_CASE(LG)                                                                              \
   case LG:                                                                                              \
-    SBFS_TIMED("k_dct2_rows", hipLaunchKernelGGL(k_dct2_rows<LG>, dim3(cdiv(ny, WaveFft<LG>::R)), dim3(WaveFft<LG>::U), \
+    SBFS_TIMED("k_dct2_rows", hipLaunchKernelGGL((nt ? k_dct2_rows<LG, true> : k_dct2_rows<LG, false>),    \
+                                                 dim3(cdiv(ny, WaveFft<LG>::R)), dim3(WaveFft<LG>::U),     \
                                                  wave_fft_lds(LG), stream(), (const double*)p->rhs, p->buf,  \
                                                  ny, p->tw, p->t1, p->t2));                             \
     break;
@@ -885,7 +902,7 @@ extern "C" int sbfs_dct_rows(int inverse, size_t n, size_t howmany, const double
       hipLaunchKernelGGL(k_dct3_rows<LG>, dim3(cdiv(rows, WaveFft<LG>::R)), dim3(WaveFft<LG>::U), wave_fft_lds(LG), stream(), \
                          din, dout, rows, dtw, dt1, dt2);                                                  \
     else                                                                                                   \
-      hipLaunchKernelGGL(k_dct2_rows<LG>, dim3(cdiv(rows, WaveFft<LG>::R)), dim3(WaveFft<LG>::U), wave_fft_lds(LG), stream(), \
+      hipLaunchKernelGGL((k_dct2_rows<LG, false>), dim3(cdiv(rows, WaveFft<LG>::R)), dim3(WaveFft<LG>::U), wave_fft_lds(LG), stream(), \
                          din, dout, rows, dtw, dt1, dt2);                                                  \
     break;
     SBFS_LOG_CASES(SBFS_TEST_CASE)

@@ -158,6 +158,12 @@ int load_dt_coefs();
 // offset (march.hip): every field of the grid, the vof with its two halo
 // rows on each side the largest, must stay below 4 GiB.
 inline bool fp64_march_addressable(size_t nx, size_t ny) { return (ny + 4) * (nx + 2) * 8 < (1ull << 32); }
+// Whether the single-case kernels stream the rows they touch once
+// (non-temporal loads and stores): only when an FP64 field outgrows what the
+// 256 MB Infinity Cache keeps between one kernel and the next; below that,
+// ordinary accesses leave the producer's rows there for the consumer.
+constexpr size_t kStreamFieldBytes = 128ull << 20;
+inline bool stream_fields(size_t nx, size_t ny) { return (ny + 4) * (nx + 2) * 8 >= kStreamFieldBytes; }
 // the learned curvature's band: cells within one cell (3 x 3 block) of a
 // cut cell, vof in [kCutLo, kCutHi]
 constexpr double kCutLo = 0.02, kCutHi = 0.98;
@@ -204,6 +210,7 @@ struct DtArgs {
   double* partial;
   double* dt;
   int nx, ny, rows, gx, nblk;
+  int stream;  // streaming loads (stream_fields)
 };
 int dt_args(const domain_t* d, DomainCtx* c, const fluid_t* f, const interface_t* s, DtArgs* a);
 __global__ void k_dt_partial_batch(const DtArgs* tab, const uint2* map);

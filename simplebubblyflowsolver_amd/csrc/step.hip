@@ -24,6 +24,7 @@ static const double kDtMax = 1.;                              // decide_dt.c:25
 // thread one column, so the loads of a thread are independent and in flight
 // together; a min does not depend on the combination order
 
+template <bool NT>
 __device__ __forceinline__ void dt_partial_body(const double* __restrict__ ux, const double* __restrict__ uy,
                                                 const double* __restrict__ hxxf, double hy, int nx, int ny,
                                                 int rows, double* __restrict__ partial, int bx, int by, int gx) {
@@ -41,13 +42,18 @@ __device__ __forceinline__ void dt_partial_body(const double* __restrict__ ux, c
     // and thread instead of one per cell.
     double mx = 0., my = 0.;
     // rows past ny repeat row ny (a repeated value leaves a max unchanged),
-    // so the loads carry no branch and are all in flight together; they are
-    // streaming loads (read once, 128 -> 88 us at 4096 x 8192)
+    // so the loads carry no branch and are all in flight together; with NT
+    // they are streaming loads (read once; 124 -> 88 us at 4096 x 8192, see
+    // stream_fields)
+    auto ld = [](const double* p) {
+      if constexpr (NT) return __builtin_nontemporal_load(p);
+      else return *p;
+    };
 #pragma unroll 16
     for (int r = 0; r < rows; r++) {
       const int j = min(j0 + r, ny);
-      if (i >= 2) mx = fmax(mx, fabs(__builtin_nontemporal_load(ux + L::xf(i, j, nx))) + small);
-      my = fmax(my, fabs(__builtin_nontemporal_load(uy + L::cc(i, j, nx))) + small);
+      if (i >= 2) mx = fmax(mx, fabs(ld(ux + L::xf(i, j, nx))) + small);
+      my = fmax(my, fabs(ld(uy + L::cc(i, j, nx))) + small);
     }
     if (i >= 2) m = fmin(m, hx / mx);
     m = fmin(m, hy / my);
@@ -56,12 +62,13 @@ __device__ __forceinline__ void dt_partial_body(const double* __restrict__ ux, c
   if (threadIdx.x == 0) partial[by * gx + bx] = m;
 }
 
+template <bool NT>
 __global__ __launch_bounds__(kRedThreads) void k_dt_partial(const double* __restrict__ ux,
                                                            const double* __restrict__ uy,
                                                            const double* __restrict__ hxxf,
                                                            double hy, int nx, int ny, int rows,
                                                            double* __restrict__ partial) {
-  dt_partial_body(ux, uy, hxxf, hy, nx, ny, rows, partial, blockIdx.x, blockIdx.y, gridDim.x);
+  dt_partial_body<NT>(ux, uy, hxxf, hy, nx, ny, rows, partial, blockIdx.x, blockIdx.y, gridDim.x);
 }
 
 // combine the limits (decide_dt.c:157-167)
@@ -91,7 +98,7 @@ __global__ __launch_bounds__(kRedThreads) void k_dt_partial_batch(const DtArgs* 
                                                                  const uint2* __restrict__ map) {
   const uint2 m = map[blockIdx.x];
   const DtArgs& a = tab[m.x];
-  dt_partial_body(a.ux, a.uy, a.hxxf, a.hy, a.nx, a.ny, a.rows, a.partial, (int)(m.y % a.gx), (int)(m.y / a.gx), a.gx);
+  dt_partial_body<false>(a.ux, a.uy, a.hxxf, a.hy, a.nx, a.ny, a.rows, a.partial, (int)(m.y % a.gx), (int)(m.y / a.gx), a.gx);
 }
 
 __global__ __launch_bounds__(kRedThreads) void k_dt_final_batch(const DtArgs* __restrict__ tab) {
@@ -172,7 +179,8 @@ int dt_launch(const domain_t* d, const DomainCtx* c, const fluid_t* f, const int
   if (dt_grid(nx, ny, capacity, &rows, &nblk)) return 1;
   const dim3 grd(cdiv(nx, kRedThreads), cdiv(ny, rows));
   SBFS_TIMED("k_dt_partial",
-             hipLaunchKernelGGL(k_dt_partial, grd, dim3(kRedThreads), 0, stream(),
+             hipLaunchKernelGGL((stream_fields(nx, ny) ? k_dt_partial<true> : k_dt_partial<false>), grd,
+                                dim3(kRedThreads), 0, stream(),
                                 (const double*)f->ux.data, (const double*)f->uy.data, c->hxxf,
                                 c->hy, nx, ny, rows, partial));
   SBFS_TIMED("k_dt_final",
