@@ -773,6 +773,7 @@ static StepKey step_key(const DomainCtx* c, fluid_t* f, interface_t* s, const Co
   k.v.push_back((uint64_t)(int64_t)g_scr.klast);
   key_add(k, g_scr.cut);
   k.v.push_back(march_ordered(options().precision == SBFS_PRECISION_MIXED) ? 1 : 0);
+  k.v.push_back(stream_fields(c->nx, c->ny) ? 1 : 0);
   key_add(k, g_scr.order);
   k.v.push_back(graph_epoch());
   return k;

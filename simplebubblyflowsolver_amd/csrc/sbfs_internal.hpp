@@ -162,8 +162,15 @@ inline bool fp64_march_addressable(size_t nx, size_t ny) { return (ny + 4) * (nx
 // (non-temporal loads and stores): only when an FP64 field outgrows what the
 // 256 MB Infinity Cache keeps between one kernel and the next; below that,
 // ordinary accesses leave the producer's rows there for the consumer.
+// SBFS_STREAM=0 / =1 forces either (the results are bitwise the same).
 constexpr size_t kStreamFieldBytes = 128ull << 20;
-inline bool stream_fields(size_t nx, size_t ny) { return (ny + 4) * (nx + 2) * 8 >= kStreamFieldBytes; }
+inline bool stream_fields(size_t nx, size_t ny) {
+  if (const char* e = getenv("SBFS_STREAM")) {
+    if (e[0] == '0') return false;
+    if (e[0] == '1') return true;
+  }
+  return (ny + 4) * (nx + 2) * 8 >= kStreamFieldBytes;
+}
 // the learned curvature's band: cells within one cell (3 x 3 block) of a
 // cut cell, vof in [kCutLo, kCutHi]
 constexpr double kCutLo = 0.02, kCutHi = 0.98;

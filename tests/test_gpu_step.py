@@ -196,6 +196,41 @@ def test_dispatch_order_does_not_change_results(product, nx, ny, bubbles, steps,
             g.close()
 
 
+@pytest.mark.parametrize("mixed", [0, 1])
+def test_streaming_variants_match_ordinary_bitwise(product, mixed):
+    """Large grids take the kernels' streaming variants (non-temporal loads and
+    stores of the rows a kernel touches once); SBFS_STREAM forces either
+    choice.  A 1024 x 2048 swarm stepped with the streaming variants and with
+    ordinary accesses agrees bitwise in every state field, in FP64 and in
+    mixed precision."""
+    import os
+    w = configs.bubble_swarm(1024, 2048, 16, steps=4, seed=5)
+    print(f"{w.name}: {w.steps} steps, mixed={mixed}", flush=True)
+    cases = [solver.Case(product, w.nx, w.ny, w.lx, w.ly, w.Re, w.Fr, w.denr, w.visr, w.We) for _ in range(2)]
+    saved = os.environ.get("SBFS_STREAM")
+    assert product.lib.sbfs_set_precision(mixed) == 0
+    try:
+        for g in cases:
+            g.load_initial_condition(w.vof())
+        for _ in range(w.steps):
+            dts = []
+            for g, flag in zip(cases, ("1", "0")):
+                os.environ["SBFS_STREAM"] = flag
+                dts.append(g.integrate())
+            assert dts[0] == dts[1]
+        for n in ("vof", "ux", "uy", "p", "psi0", "den1", "txy"):
+            x, y = cases[0].get(n), cases[1].get(n)
+            assert np.array_equal(x, y), (n, np.abs(x - y).max())
+    finally:
+        assert product.lib.sbfs_set_precision(0) == 0
+        if saved is None:
+            os.environ.pop("SBFS_STREAM", None)
+        else:
+            os.environ["SBFS_STREAM"] = saved
+        for g in cases:
+            g.close()
+
+
 def test_repo_default_rising_bubble_100_steps(oracle_mod, product):
     """The 128 x 256 rising bubble (the reference's CPU-runnable case), 100
     steps on the default (fused) path against the oracle: velocity and vof
