@@ -900,19 +900,26 @@ __global__ __launch_bounds__(1024) void k_march_order(const unsigned* __restrict
 // MarchArgs outgrow the scalar registers, and the compiler spills them into
 // vector lanes (v_writelane / v_readlane in the row loop, up to 185 of them
 // in the FP64 folded body); from memory it reloads them with scalar loads.
+// the (strip, band) of this workgroup: from the grid, or from the dispatch
+// order (false: a slot past the last unit)
+__device__ __forceinline__ bool march_unit(const MarchArgs& a, int& bx, int& by) {
+  bx = blockIdx.x;
+  by = blockIdx.y;
+  if (a.order) {
+    const unsigned u = a.order[blockIdx.x];
+    if (u >= (unsigned)march_units(a.nx, a.ny, a.band)) return false;
+    const int strips = march_strips(a.nx);
+    bx = (int)u % strips;
+    by = (int)u / strips;
+  }
+  return true;
+}
 #define SBFS_MARCH_KERNEL(name, R, CORR, WAVES)                                            \
   __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES))) void name( \
       const MarchArgs* __restrict__ args) {                                               \
     const MarchArgs& a = *args;                                                           \
-    int bx = blockIdx.x, by = blockIdx.y;                                                 \
-    if (a.order) {                                                                        \
-      const unsigned u = a.order[blockIdx.x];                                             \
-      if (u >= (unsigned)march_units(a.nx, a.ny, a.band)) return;                         \
-      const int strips = march_strips(a.nx);                                              \
-      bx = (int)u % strips;                                                               \
-      by = (int)u / strips;                                                               \
-    }                                                                                     \
-    march_body<R, CORR>(a, bx, by);                                                       \
+    int bx, by;                                                                           \
+    if (march_unit(a, bx, by)) march_body<R, CORR>(a, bx, by);                            \
   }                                                                                       \
   __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES))) void name##_batch( \
       const MarchArgs* __restrict__ tab, const uint2* __restrict__ map) {                \
@@ -932,15 +939,8 @@ SBFS_MARCH_KERNEL(k_march_mixed_corr, float, true, SBFS_MIXED_WAVES)
   __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SBFS_FP64_WAVES))) void name( \
       const MarchArgs* __restrict__ args) {                                                      \
     const MarchArgs& a = *args;                                                                  \
-    int bx = blockIdx.x, by = blockIdx.y;                                                        \
-    if (a.order) {                                                                               \
-      const unsigned u = a.order[blockIdx.x];                                                    \
-      if (u >= (unsigned)march_units(a.nx, a.ny, a.band)) return;                                \
-      const int strips = march_strips(a.nx);                                                     \
-      bx = (int)u % strips;                                                                      \
-      by = (int)u / strips;                                                                      \
-    }                                                                                            \
-    march_body<double, CORR, true>(a, bx, by);                                                   \
+    int bx, by;                                                                                  \
+    if (march_unit(a, bx, by)) march_body<double, CORR, true>(a, bx, by);                        \
   }
 SBFS_MARCH_NT_KERNEL(k_march_nt, false)
 SBFS_MARCH_NT_KERNEL(k_march_corr_nt, true)
