@@ -76,7 +76,19 @@ __device__ __forceinline__ void dt_final_body(const double* __restrict__ partial
                                               double dif1, double dint, double* __restrict__ dt) {
   __shared__ double sh[kRedThreads / 64];
   double m = kDtMax;
-  for (int k = threadIdx.x; k < n; k += blockDim.x) m = fmin(m, partial[k]);
+  // four running minima, so that the loads are in flight together (a
+  // minimum does not depend on the combination order)
+  double m1 = kDtMax, m2 = kDtMax, m3 = kDtMax;
+  const int s = (int)blockDim.x;
+  int k = threadIdx.x;
+  for (; k + 3 * s < n; k += 4 * s) {
+    m = fmin(m, partial[k]);
+    m1 = fmin(m1, partial[k + s]);
+    m2 = fmin(m2, partial[k + 2 * s]);
+    m3 = fmin(m3, partial[k + 3 * s]);
+  }
+  for (; k < n; k += s) m = fmin(m, partial[k]);
+  m = fmin(fmin(m, m1), fmin(m2, m3));
   m = block_reduce<0>(m, sh);
   if (threadIdx.x == 0) {
     double r = m * cadv;
